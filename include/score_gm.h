@@ -1,0 +1,221 @@
+/*
+ * score_gm.h -- C ABI of the MI355X growth machine.
+ *
+ * The library replaces the body of SCORE's recrystallization growth loop on
+ * one domain, caHdl::solve_RXGROWTH (SCORE_Kernel.h:1081, SCORE_Kernel.cpp:6497-6792).
+ * The caller hands over the domain exactly as SCORE holds it after
+ * caHdl::solve_REPLACE_CA_STRUCTURE (SCORE_Kernel.cpp:6353-6445): the cell grid
+ * holds deformed-grain ids, the pools hold orientation indices, dislocation
+ * densities and nucleus sites.  The library places the site-saturated nuclei,
+ * runs the step loop on the GPU and returns the grain map, the grain counts and
+ * the per-step trajectory.
+ *
+ * All functions return 0 on success and a negative SCORE_GM_ERR_* code on
+ * failure; nothing throws across the ABI.  Host buffers are caller-owned and
+ * copied; no caller pointer is retained after a call returns.  A handle is used
+ * by one host thread at a time; distinct handles are independent.
+ */
+#ifndef SCORE_GM_H
+#define SCORE_GM_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCORE_GM_ABI_VERSION 1
+
+/* status codes */
+#define SCORE_GM_OK                 0
+#define SCORE_GM_ERR_ARG           -1  /* invalid argument or domain */
+#define SCORE_GM_ERR_OOM           -2  /* device or host allocation failed */
+#define SCORE_GM_ERR_HIP           -3  /* HIP runtime error */
+#define SCORE_GM_ERR_CAPACITY      -4  /* a fixed-size front list overflowed (SCORE would write out of bounds) */
+#define SCORE_GM_ERR_UNSUPPORTED   -5  /* option not implemented by this build */
+#define SCORE_GM_ERR_STATE         -6  /* call not valid in the handle's current state */
+#define SCORE_GM_ERR_NODEVICE      -7  /* no usable gfx950 device */
+
+/* mobility models, same numbering as SCORE_Defs_Functionality.h:200-201 */
+#define SCORE_GM_MOB_SEBALDGOTTSTEIN 1
+#define SCORE_GM_MOB_ROLLETTHOLM     2
+
+/* grain-map encodings for score_gm_get_grainmap */
+#define SCORE_GM_MAP_CELLGRID   0  /* raw cell-grid values: defg id, ndefg+rxg id, or 4096002000 (infected) */
+#define SCORE_GM_MAP_RAW2GID    1  /* 1 = infected, else 2+gid (SCORE_Io_Characterization.cpp:583-676) */
+
+/* sentinel, SCORE_Defs_Functionality.h:95 */
+#define SCORE_GM_CURRENTLY_INFECTED 4096002000u
+
+/* front lists, SCORE's myRXFrontInside / myRXFrontBorder (SCORE_Kernel.h:1403-1585) */
+#define SCORE_GM_LIST_INSIDE 0
+#define SCORE_GM_LIST_BORDER 1
+
+/* Domain after solve_REPLACE_CA_STRUCTURE (SCORE_Kernel.cpp:6353-6445). */
+typedef struct score_gm_domain {
+    uint32_t nx, ny, nz;          /* edge lengths in cells along RD (x), TD (y), ND (z) */
+    double cellsize;              /* m (myCAGeometry.cellsize) */
+    const uint32_t *cellgrid;     /* nx*ny*nz deformed-grain ids, index x + y*nx + z*nx*ny */
+    uint32_t n_ori;               /* orientation pool (myoripool) */
+    const double *ori_quat;       /* n_ori*4: q0,q1,q2,q3 per orientation */
+    uint32_t n_defg;              /* deformed-grain pool (mydefgpool) */
+    const uint32_t *defg_ori;     /* n_defg orientation indices (caori) */
+    const double *defg_rho;       /* n_defg dislocation densities, m^-2 (rho) */
+    uint32_t n_rxg;               /* nucleus pool (myrxgpool) */
+    const uint32_t *rxg_ori;      /* n_rxg orientation indices (caori) */
+    const uint32_t *rxg_site;     /* n_rxg global linear sites (nucsite) */
+} score_gm_domain;
+
+/* Physics and integrator settings (caHdl::init_parameter, SCORE_Kernel.cpp:5747-5880). */
+typedef struct score_gm_params {
+    int32_t mobility_model;       /* SCORE_GM_MOB_* */
+    int32_t zener_mode;           /* 0 = no dispersoid drag (only mode implemented) */
+    int32_t recovery_mode;        /* 0 = no recovery (only mode implemented) */
+    int32_t reserved0;
+    /* Rollett-Holm (MobilityModel 2) */
+    double rh_hagb_m0;            /* m^4/(J s) */
+    double rh_hagb_hact;          /* J (eV * 1.602176565e-19, as SCORE_Kernel.cpp:579) */
+    double rh_cut, rh_trans, rh_exponent;
+    /* Sebald-Gottstein (MobilityModel 1) */
+    double lagb_m0, lagb_hact, hagb_m0, hagb_hact, gs_m0, gs_hact;   /* m^4/(J s), J */
+    /* material */
+    double G0;                    /* ZeroKelvinShearModulusG0, Pa */
+    double b0;                    /* ZeroCelsiusBurgersVector, m */
+    /* annealing schedule: time (s, strictly increasing from 0) and temperature (K) */
+    uint32_t n_sched;
+    uint32_t reserved1;
+    const double *sched_time;
+    const double *sched_temp;
+    /* integrator */
+    double maxfill;               /* MAXFILLPERSTEP */
+    double xmax;                  /* XMAX */
+    double tmax;                  /* TIMEMAX, s */
+    int64_t nmax;                 /* NMAX */
+    double rel_cell_caching;      /* InitialCacheSizeRXCells */
+    double rel_cell_recaching;    /* ReCacheSizeRXCells */
+    /* on-the-fly defragmentation thresholds, compared with Xcells (SCORE_Kernel.cpp:6625-6630) */
+    uint32_t n_defrag;
+    uint32_t reserved2;
+    const double *defrag_at;
+} score_gm_params;
+
+/* One row per executed growth step (the values SCORE's master section holds,
+ * SCORE_Kernel.cpp:6688-6742).  t and dt are the time at the start of the step
+ * and the increment used in it; X, xcells and rhomax are the values after it. */
+typedef struct score_gm_traj_row {
+    uint32_t step;
+    uint32_t sv;                  /* ACTIVE front cells advanced in the step (Sv) */
+    uint32_t nfull_inside;        /* cells that completed, Inside list */
+    uint32_t nfull_border;        /* cells that completed, Border list */
+    uint32_t nslots_inside;       /* nextSlotNeverActiveRXFrontInside after the step */
+    uint32_t nslots_border;       /* nextSlotNeverActiveRXFrontBorder after the step */
+    uint32_t ninfected;           /* new infections in the step */
+    uint32_t npending;            /* mobility weights resolved on the host after the step */
+    double t;
+    double dt;
+    double X;
+    double xcells;
+    double temperature;
+    double rhomax;
+} score_gm_traj_row;
+
+typedef struct score_gm_result {
+    double t;                     /* tsimend */
+    double X;
+    double xcells;
+    uint32_t step;                /* stepsimend */
+    int32_t completed;            /* 1 once the termination test of SCORE_Kernel.cpp:6619 fired */
+    uint64_t sum_sv;              /* sum of Sv over all steps (cell updates) */
+    uint32_t nuclei_placed;       /* nmynuclei */
+    uint32_t reserved;
+    double pmax;                  /* myMobilityWeightMax as tracked by the host */
+    double loop_seconds;          /* host wall time spent inside step/run calls */
+} score_gm_result;
+
+/* Per-phase device time and work counters, the counterpart of SCORE's per-thread
+ * phase timers (omp_log_rxfrontstats, SCORE_Kernel.cpp:5019-5063; written to
+ * ThreadProfilingGrowth.csv when OutputThreadProfiling is set).  Times are HIP
+ * events recorded on the handle's stream around each phase of every step. */
+typedef struct score_gm_profile {
+    uint64_t steps;               /* steps recorded while profiling was on */
+    double ms_sweep;              /* front sweep (calcGrowthStep Inside+Border) */
+    double ms_compact;            /* FullRX / recycling list compaction */
+    double ms_infect;             /* propose, decide, rank, commit, retire (updateFullRX) */
+    double ms_tail;               /* rhomax reduction and stats */
+    double ms_host;               /* host wall time of the profiled steps */
+    uint64_t sweep_slots;         /* front slots scanned */
+    uint64_t sweep_active;        /* ACTIVE slots advanced (sum of Sv) */
+    uint64_t sweep_bytes;         /* algorithmic HBM bytes of the sweep */
+    uint64_t infect_parents;      /* completed cells (sum of FullRX lengths) */
+    uint64_t infect_new;          /* new infections */
+    uint64_t infect_bytes;        /* algorithmic HBM bytes of the infection phase */
+    uint64_t pending_resolved;    /* mobility weights resolved on the host */
+} score_gm_profile;
+
+typedef struct score_gm_handle score_gm_handle;
+
+/* ABI version of the loaded library. */
+int score_gm_abi_version(void);
+
+/* Human-readable text for a status code. */
+const char *score_gm_strerror(int status);
+
+/* Number of usable gfx950 devices (0 when none). */
+int score_gm_device_count(void);
+
+/* Copy the domain to device `device`, allocate the front lists, place the
+ * site-saturated nuclei (ompshar_sim_myCA_sitesaturatedNucleation,
+ * SCORE_GrowthMachine.cpp:50-189) and compute the first time step
+ * (SCORE_Kernel.cpp:6497-6530).  Replaces the set-up part of solve_RXGROWTH. */
+int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm,
+                    int device, score_gm_handle **out);
+
+/* Advance by up to `max_steps` growth steps of the main loop
+ * (SCORE_Kernel.cpp:6615-6747).  *completed is set to 1 once the termination
+ * test fires.  Replaces iterations of solve_RXGROWTH's loop. */
+int score_gm_step(score_gm_handle *h, uint32_t max_steps, int32_t *completed);
+
+/* Run to completion (XMAX, TIMEMAX, NMAX, end of schedule or all cells
+ * transformed) and fill *res.  Replaces the whole of solve_RXGROWTH. */
+int score_gm_run(score_gm_handle *h, score_gm_result *res);
+
+/* Current scalars (tsimend, X, Xcells, step, ...). */
+int score_gm_get_result(const score_gm_handle *h, score_gm_result *res);
+
+/* Grain map in the requested SCORE_GM_MAP_* encoding, nx*ny*nz values. */
+int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding);
+
+/* Cell counts per deformed grain (mydefgpool[].cellcount, n_defg values) and
+ * per nucleus (myrxgpool[].cellcount, n_rxg values). Either pointer may be NULL. */
+int score_gm_get_counts(score_gm_handle *h, uint32_t *defg_counts, uint32_t *rxg_counts);
+
+/* Per-step trajectory: copies min(cap, rows recorded) rows and sets *n to the
+ * number of rows recorded. */
+int score_gm_get_trajectory(const score_gm_handle *h, score_gm_traj_row *rows,
+                            size_t cap, size_t *n);
+
+/* Slot-level copy of one front list (SCORE_GM_LIST_*), slots [0, *n) where *n
+ * is nextSlotNeverActive of that list (SCORE_Kernel.h:124-137 record):
+ * activity, infector, linear cell index, rxFrac, P, deformed and nucleus ids.
+ * Each array may be NULL; otherwise it must hold `cap` entries. */
+int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t *n,
+                       uint8_t *activity, uint8_t *infector, uint32_t *cell,
+                       double *rxfrac, double *P, uint32_t *defg, uint32_t *rxg);
+
+/* Switch per-phase event timing on (1) or off (0); switching on clears the
+ * counters.  Replaces SCORE's OutputThreadProfiling switch (SCORE_Kernel.cpp:6677-6683). */
+int score_gm_set_profiling(score_gm_handle *h, int32_t enable);
+
+/* Counters accumulated since profiling was switched on. */
+int score_gm_get_profile(const score_gm_handle *h, score_gm_profile *out);
+
+/* Release all device memory of the handle (omp_cleanupMemoryUsedForGrowthMachine,
+ * SCORE_Kernel.cpp:5066-5133). NULL is accepted. */
+void score_gm_destroy(score_gm_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCORE_GM_H */

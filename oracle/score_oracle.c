@@ -1,0 +1,752 @@
+/*
+ * score_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A serial CPU restatement of SCORE's growth machine on one domain
+ * (caHdl::solve_RXGROWTH, SCORE_Kernel.cpp:6497-6792) with the semantics of a
+ * run with OMP_NUM_THREADS=1: one region spanning the whole domain, an Inside
+ * and a Border front list, recycle-then-append slot assignment, first-come
+ * infection in list order and fixed direction order.  It is the checker for the
+ * GPU library; it is never linked into the library, and only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py load it.
+ *
+ * Each function names the reference lines it follows.  It consumes the same
+ * score_gm_domain / score_gm_params structs as the library (include/score_gm.h)
+ * so both receive byte-identical inputs.
+ *
+ * Arithmetic: FP64, compiled with -ffp-contract=off and without -march so that
+ * no FMA is formed, matching SCORE's GCC -O3 build (CMakeLists.txt:23,89-91).
+ * Transcendentals come from the host glibc (acos, pow, exp, cos, sin, sqrt), as
+ * in the reference.
+ *
+ * Parity of this file against the reference is pinned by tests/test_oracle_*.py
+ * (closed forms, brute force over the cubic group, invariants, hand-derived
+ * first steps, and the statistics of the reference's shipped example/jmak run).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "score_gm.h"
+
+/* ---- constants (SCORE_Defs_Functionality.h:83-101, SCORE_Defs_Physics.h:51-160, SCORE_Math.h:54-70) ---- */
+#define O_CURRENTLY_INFECTED 4096002000u
+#define O_INVALID_ADDRESS 4294967290u
+#define O_KBOLTZMAN (1.3806488e-23)
+#define O_PI (3.1415926535897932384626433832795)
+#define O_RHO_RX (1.0e11)
+#define O_DOUBLE_ACCURACY (5.0e-16)
+#define O_INITIAL_DELTAT (1e-6)
+#define O_INFINITE (1e36)
+#define O_SMALL_HEATRATE (0.000277777777777777)
+#define O_SMALL_HEAT (1.0)
+#define O_RHOMAX_WELLANNEALED (1e8)
+#define O_DEFAULT_PMAX (0.0)
+#define O_DEFAULT_SMALL_NUMBER (1e-6)
+#define O_MAXDISORI_LAGB2HAGB ((15.0) / (180.0) * (O_PI))
+#define O_MAXDISORI_TO_40DEG111 ((10.0) / (180.0) * (O_PI))
+#define O_FULLRXCACHING_ATT_FACTOR (1.5)
+#define O_FULLRECYCLING_ATT_FACTOR (3.0)
+#define O_FGEOFACE (1.0000)
+#define O_FGEOEDGE (0.7957)
+#define O_FGEODIAG (0.6494)
+
+/* shape factors indexed by infector code (SCORE_Kernel.cpp:4051-4053) */
+static const double o_shape[27] = {
+    O_FGEODIAG, O_FGEOEDGE, O_FGEODIAG, O_FGEOEDGE, O_FGEOFACE, O_FGEOEDGE, O_FGEODIAG, O_FGEOEDGE, O_FGEODIAG,
+    O_FGEOEDGE, O_FGEOFACE, O_FGEOEDGE, O_FGEOFACE, O_FGEOFACE, O_FGEOEDGE, O_FGEOFACE, O_FGEOEDGE,
+    O_FGEODIAG, O_FGEOEDGE, O_FGEODIAG, O_FGEOEDGE, O_FGEOFACE, O_FGEOEDGE, O_FGEODIAG, O_FGEOEDGE, O_FGEODIAG, (1.0)};
+
+/* The 26 infection attempts in call order (SCORE_Kernel.cpp:4611-4638 and
+ * :4719-4748): offset, infector code, carry class (1 face, 2 edge, 3 corner). */
+typedef struct { int dx, dy, dz, code, carry; } o_dir;
+static const o_dir o_dirs[26] = {
+    {-1, -1, 0, 16, 2}, {0, -1, 0, 15, 1}, {+1, -1, 0, 14, 2}, {-1, 0, 0, 13, 1}, {+1, 0, 0, 12, 1},
+    {-1, +1, 0, 11, 2}, {0, +1, 0, 10, 1}, {+1, +1, 0, 9, 2},
+    {-1, +1, -1, 19, 3}, {0, +1, -1, 18, 2}, {+1, +1, -1, 17, 3}, {-1, -1, -1, 25, 3}, {0, -1, -1, 24, 2},
+    {+1, -1, -1, 23, 3}, {-1, 0, -1, 22, 2}, {0, 0, -1, 21, 1}, {+1, 0, -1, 20, 2},
+    {-1, -1, +1, 8, 3}, {0, -1, +1, 7, 2}, {+1, -1, +1, 6, 3}, {-1, 0, +1, 5, 2}, {0, 0, +1, 4, 1},
+    {+1, 0, +1, 3, 2}, {-1, +1, +1, 2, 3}, {0, +1, +1, 1, 2}, {+1, +1, +1, 0, 3}};
+
+/* front record, the 32-byte `cell` of SCORE_Kernel.h:124-137 */
+typedef struct {
+    uint8_t act;
+    uint8_t inf;
+    int16_t x, y, z;
+    double frac;
+    double P;
+    uint32_t defg;
+    uint32_t rxg;
+} o_cell;
+
+/* one front list with its recycling and FullRX bookkeeping (SCORE_Kernel.cpp:3587-3710) */
+typedef struct {
+    o_cell *c;
+    uint32_t ntotal;        /* allocated slots */
+    uint32_t nnever;        /* nextSlotNeverActive */
+    int growable;           /* Inside grows (:3795-3843); Border is fixed (:3847-3874) */
+    uint32_t *rec;
+    uint32_t rec_cap;       /* ntotalRecyclingList, fixed */
+    uint32_t rec_next;      /* nextSlotThatBecomesRecycled */
+    uint32_t rec_first_not; /* firstNotRecycledYet */
+    uint32_t *full;
+    uint32_t full_cap;
+    uint32_t full_n;        /* nextSlotToFullRX */
+    uint32_t sv;
+    uint32_t nupdated;
+} o_list;
+
+typedef struct oracle {
+    uint32_t nx, ny, nz, nxy;
+    uint64_t N;
+    double cellsize, inv_dx;
+    uint32_t *grid;
+    uint32_t n_defg, n_rxg;
+    double *defq, *rxq, *rho;
+    uint32_t *site;
+    uint32_t *defcnt, *rxcnt;
+    score_gm_params p;
+    double *sched_t, *sched_T, *defrag_at;
+    double tprocessingend;
+    o_list in, bd;
+    /* caHdl scalars */
+    double t, X, Xcells, dt, T, G, b, Gbhalfsq;
+    double mLAGB, mHAGB, mGS, mRHHAGB, pzmin, rhomax, Pmax, reg_Pmax;
+    uint32_t step;
+    uint32_t nuclei;
+    uint32_t defrag_cnt;
+    int completed;
+    int err;
+    uint64_t sum_sv;
+    score_gm_traj_row *traj;
+    size_t ntraj, captraj;
+    uint32_t last_infected;
+} oracle;
+
+/* ---------------- orientation math ---------------- */
+
+/* Hamilton product, multiply_quat_degraef (SCORE_Math.cpp:230-239) with P_EIJK = +1 */
+static void o_qmul(const double *p, const double *q, double *r) {
+    r[0] = p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3];
+    r[1] = q[0] * p[1] + p[0] * q[1] + (p[2] * q[3] - p[3] * q[2]);
+    r[2] = q[0] * p[2] + p[0] * q[2] + (-(p[1] * q[3]) + p[3] * q[1]);
+    r[3] = q[0] * p[3] + p[0] * q[3] + (p[1] * q[2] - p[2] * q[1]);
+}
+
+/* The six Grimmer candidate quaternions of the misorientation r
+ * (disori_angle_oriquat_cubic_imm, SCORE_Math.cpp:592-602). */
+static void o_grimmer(const double *r, double r0[6][4]) {
+    double a = r[0], b = r[1], c = r[2], d = r[3];
+    double fac = 1.0 / sqrt(2.0);
+    r0[0][0] = (a + b) * fac; r0[0][1] = (a - b) * fac; r0[0][2] = (c + d) * fac; r0[0][3] = (c - d) * fac;
+    r0[1][0] = (a + c) * fac; r0[1][1] = (a - c) * fac; r0[1][2] = (b + d) * fac; r0[1][3] = (b - d) * fac;
+    r0[2][0] = (a + d) * fac; r0[2][1] = (a - d) * fac; r0[2][2] = (b + c) * fac; r0[2][3] = (b - c) * fac;
+    r0[3][0] = (a + b + c + d) * 0.5; r0[3][1] = (a + b - c - d) * 0.5; r0[3][2] = (a - b + c - d) * 0.5; r0[3][3] = (a - b - c + d) * 0.5;
+    r0[4][0] = (a + b + c - d) * 0.5; r0[4][1] = (a + b - c + d) * 0.5; r0[4][2] = (a - b + c + d) * 0.5; r0[4][3] = (a - b - c - d) * 0.5;
+    r0[5][0] = a; r0[5][1] = b; r0[5][2] = c; r0[5][3] = d;
+}
+
+/* misorientation q^-1 p (SCORE_Math.cpp:564-572) */
+static void o_misori(const double *p, const double *q, double *r) {
+    double qm1[4];
+    qm1[0] = q[0];
+    qm1[1] = -1.0 * q[1];
+    qm1[2] = -1.0 * q[2];
+    qm1[3] = -1.0 * q[3];
+    o_qmul(qm1, p, r);
+}
+
+/* largest |component| over the Grimmer candidates (SCORE_Math.cpp:604-608) */
+double oracle_disori_omega(const double *p, const double *q) {
+    double r[4], r0[6][4];
+    o_misori(p, q, r);
+    o_grimmer(r, r0);
+    double omega = 0.0;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 4; j++)
+            if (fabs(r0[i][j]) > omega) omega = fabs(r0[i][j]);
+    return omega;
+}
+
+/* disori_angle_oriquat_cubic_imm (SCORE_Math.cpp:558-622), radians */
+double oracle_disori_angle(const double *p, const double *q) {
+    double omega = oracle_disori_omega(p, q);
+    if (omega < (1.0 - O_DOUBLE_ACCURACY)) return 2.0 * acos(omega);
+    return 0.0;
+}
+
+/* disori_fromoriquat_cubic_imm (SCORE_Math.cpp:1067-1131) */
+static void o_disori_quat(const double *p, const double *q, double *quat) {
+    double r[4], r0[6][4];
+    o_misori(p, q, r);
+    o_grimmer(r, r0);
+    int mi = 0;
+    double mx = 0.0;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 4; j++)
+            if (fabs(r0[i][j]) > mx) { mx = fabs(r0[i][j]); mi = i; }
+    double rq[4] = {fabs(r0[mi][0]), fabs(r0[mi][1]), fabs(r0[mi][2]), fabs(r0[mi][3])};
+    for (int i = 0; i < 3; i++) /* ascending sort of four values */
+        for (int j = 0; j < 3 - i; j++)
+            if (rq[j] > rq[j + 1]) { double tmp = rq[j]; rq[j] = rq[j + 1]; rq[j + 1] = tmp; }
+    quat[0] = rq[3]; quat[1] = rq[2]; quat[2] = rq[1]; quat[3] = rq[0];
+}
+
+/* Mobility weight P of a nucleus/deformed pair: Rollett-Holm sigmoid
+ * (SCORE_Kernel.cpp:2490-2496, SCORE_GrainBoundaryPhysics.cpp:110-135) or
+ * Sebald-Gottstein classes (SCORE_Kernel.cpp:2498-2526, SCORE_GrainBoundaryPhysics.cpp:59-107). */
+double oracle_mobility_weight(const score_gm_params *p, const double *qrx, const double *qdef) {
+    double theta = oracle_disori_angle(qrx, qdef);
+    if (p->mobility_model == SCORE_GM_MOB_ROLLETTHOLM) {
+        return 1.0 - (p->rh_cut * exp(-1.0 * p->rh_trans * pow((theta / O_MAXDISORI_LAGB2HAGB), p->rh_exponent)));
+    }
+    if (theta <= O_MAXDISORI_LAGB2HAGB) return -1.0;
+    double qdis[4] = {1.0, 0.0, 0.0, 0.0};
+    o_disori_quat(qrx, qdef, qdis);
+    double maxDev40_111 = O_MAXDISORI_TO_40DEG111;
+    double _sqrt3 = 1 / sqrt(3.0);
+    double oneNinth = 1.0 / 9.0;
+    double m40_111[4] = {cos(oneNinth * O_PI), _sqrt3 * sin(oneNinth * O_PI), _sqrt3 * sin(oneNinth * O_PI),
+                         _sqrt3 * sin(oneNinth * O_PI)};
+    double dev = oracle_disori_angle(qdis, m40_111);
+    double w = 0.0;
+    if (dev < maxDev40_111) {
+        double cc = cos(0.5 * O_PI * dev / maxDev40_111);
+        w = cc * cc;
+    }
+    return w;
+}
+
+/* ---------------- per-step physics scalars ---------------- */
+
+/* update_temperature (SCORE_Kernel.cpp:5318-5337) */
+static void o_update_temperature(oracle *o) {
+    uint32_t i = 0, n = o->p.n_sched;
+    while (i < n && o->sched_t[i] < o->t) i++;
+    if (i == 0) { o->T = o->sched_T[0]; return; }
+    if (i >= n) { o->T = o->sched_T[n - 1]; return; }
+    o->T = o->sched_T[i - 1] + ((o->sched_T[i] - o->sched_T[i - 1]) / (o->sched_t[i] - o->sched_t[i - 1])) * (o->t - o->sched_t[i - 1]);
+}
+
+/* get_shearmodulus, get_burgersvector (SCORE_GeneralPhysics.cpp:65-90);
+ * update_atomisticproperties (SCORE_Kernel.cpp:5340-5352) */
+static void o_update_atomistic(oracle *o) {
+    double T = o->T;
+    o->G = o->p.G0 * (-1.38e-9 * (T * T * T) + 1.21e-6 * (T * T) - 4.58e-4 * T + 1.00);
+    o->b = o->p.b0;
+    o->Gbhalfsq = 0.5 * o->G * (o->b * o->b);
+}
+
+/* update_intrinsicmobilities (SCORE_Kernel.cpp:5355-5368) */
+static void o_update_mobilities(oracle *o) {
+    double kT = O_KBOLTZMAN * o->T;
+    o->mLAGB = o->p.lagb_m0 * exp(-1.0 * o->p.lagb_hact / kT);
+    o->mHAGB = o->p.hagb_m0 * exp(-1.0 * o->p.hagb_hact / kT);
+    o->mGS = o->p.gs_m0 * exp(-1.0 * o->p.gs_hact / kT);
+    o->mRHHAGB = o->p.rh_hagb_m0 * exp(-1.0 * o->p.rh_hagb_hact / (O_KBOLTZMAN * o->T));
+}
+
+/* get_dtmax_instantslope_temp (SCORE_Kernel.cpp:5561-5591) */
+static double o_dt_temp(const oracle *o) {
+    uint32_t i = 0, n = o->p.n_sched;
+    while (i < n && o->sched_t[i] < o->t) i++;
+    if (i == 0) return O_INITIAL_DELTAT;
+    if (i < n) {
+        double dTdt = (fabs(o->sched_T[i] - o->sched_T[i - 1]) / (o->sched_t[i] - o->sched_t[i - 1]));
+        if (dTdt <= O_SMALL_HEATRATE) dTdt = O_SMALL_HEATRATE;
+        double dtmax = (O_SMALL_HEAT / dTdt);
+        double dtintvl = o->sched_t[i] - o->sched_t[i - 1];
+        if (dtmax >= dtintvl) dtmax = dtintvl;
+        return dtmax;
+    }
+    return O_INFINITE;
+}
+
+/* get_dtmax_instantslope_cells (SCORE_Kernel.cpp:5519-5558) */
+static double o_dt_cells(const oracle *o) {
+    double kT = O_KBOLTZMAN * o->T;
+    if (o->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) {
+        double mmax = o->p.rh_hagb_m0 * exp(-1.0 * o->p.rh_hagb_hact / (O_KBOLTZMAN * o->T));
+        double vmax = mmax * ((o->Gbhalfsq * o->rhomax) - o->pzmin);
+        return ((o->p.maxfill * o->cellsize) / vmax);
+    }
+    double mgs = o->p.gs_m0 * exp(-1.0 * o->p.gs_hact / kT);
+    double mhagb = o->p.hagb_m0 * exp(-1.0 * o->p.hagb_hact / kT);
+    double m_max = (o->Pmax * mgs) + ((1.0 - o->Pmax) * mhagb);
+    if (o->Pmax < (0.0 - O_DEFAULT_SMALL_NUMBER)) m_max = o->p.lagb_m0 * exp(-1.0 * o->p.lagb_hact / kT);
+    double v_max = m_max * ((o->Gbhalfsq * o->rhomax) - o->pzmin);
+    return ((o->p.maxfill * o->cellsize) / v_max);
+}
+
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery, Zener and
+ * nucleation rules return INFINITE for the modes implemented here. */
+static double o_dt_minimum(const oracle *o) {
+    double mn = O_INFINITE;
+    double d = o_dt_temp(o);
+    if (d < mn) mn = d;
+    double dm = o_dt_cells(o);
+    if (dm < mn) mn = dm;
+    return mn;
+}
+
+/* update_mydgoptimize_* + update_myrhomax (SCORE_RecoveryModels.cpp:128-165):
+ * the largest rho among deformed grains that still own cells, floor 1e8 */
+static void o_update_rhomax(oracle *o) {
+    double rmax = O_RHOMAX_WELLANNEALED;
+    double best = -1.0;
+    int found = 0;
+    for (uint32_t d = 0; d < o->n_defg; d++)
+        if (o->defcnt[d] > 0 && (!found || o->rho[d] > best)) { best = o->rho[d]; found = 1; }
+    if (found && best >= rmax) rmax = best;
+    o->rhomax = rmax;
+}
+
+/* ---------------- front lists ---------------- */
+
+static int o_list_init(o_list *L, uint32_t nslots, uint32_t rec_cap, uint32_t full_cap, int growable) {
+    memset(L, 0, sizeof(*L));
+    L->ntotal = nslots > 0 ? nslots : 1;
+    L->c = (o_cell *)calloc(L->ntotal, sizeof(o_cell));
+    L->rec_cap = rec_cap;
+    L->rec = (uint32_t *)malloc(sizeof(uint32_t) * (rec_cap > 0 ? rec_cap : 1));
+    L->full_cap = full_cap > 0 ? full_cap : 1;
+    L->full = (uint32_t *)malloc(sizeof(uint32_t) * L->full_cap);
+    L->growable = growable;
+    if (!L->c || !L->rec || !L->full) return -1;
+    for (uint32_t i = 0; i < L->ntotal; i++) {
+        L->c[i].act = 0; L->c[i].inf = 26; L->c[i].x = L->c[i].y = L->c[i].z = -1;
+        L->c[i].frac = 0.0; L->c[i].P = -1.0; L->c[i].defg = 4096000001u; L->c[i].rxg = 4096000001u;
+    }
+    return 0;
+}
+
+static void o_list_free(o_list *L) {
+    free(L->c); free(L->rec); free(L->full);
+    memset(L, 0, sizeof(*L));
+}
+
+/* omp_getNextFreeSlotInRXFrontInside / Border with CELLRECYCLE (SCORE_Kernel.cpp:3757-3874);
+ * CELLAPPEND skips the recycling branch. Inside grows; Border returns INVALID_ADDRESS when full. */
+static uint32_t o_next_free(o_list *L, int recycle) {
+    if (recycle && L->rec_next < L->rec_first_not) return L->rec[L->rec_next++];
+    if (L->nnever < L->ntotal) return L->nnever++;
+    if (!L->growable) return O_INVALID_ADDRESS;
+    uint32_t nsz = L->ntotal * 2u;
+    o_cell *nc = (o_cell *)realloc(L->c, sizeof(o_cell) * nsz);
+    if (!nc) return O_INVALID_ADDRESS;
+    L->c = nc;
+    for (uint32_t i = L->ntotal; i < nsz; i++) {
+        L->c[i].act = 0; L->c[i].inf = 26; L->c[i].x = L->c[i].y = L->c[i].z = -1;
+        L->c[i].frac = 0.0; L->c[i].P = -1.0; L->c[i].defg = 4096000001u; L->c[i].rxg = 4096000001u;
+    }
+    L->ntotal = nsz;
+    return L->nnever++;
+}
+
+/* append_to_fullrx_*_list (SCORE_Kernel.cpp:3994-4030, :4133-4170): grows */
+static int o_append_full(o_list *L, uint32_t c) {
+    if (L->full_n >= L->full_cap) {
+        uint32_t nsz = L->full_cap * 2u;
+        uint32_t *nf = (uint32_t *)realloc(L->full, sizeof(uint32_t) * nsz);
+        if (!nf) return -1;
+        L->full = nf;
+        L->full_cap = nsz;
+    }
+    L->full[L->full_n++] = c;
+    return 0;
+}
+
+/* append_to_recycling_*_list (SCORE_Kernel.cpp:4034-4042, :4173-4180): silently capped */
+static void o_append_rec(o_list *L, uint32_t c) {
+    if (L->rec_first_not < L->rec_cap) L->rec[L->rec_first_not++] = c;
+}
+
+/* ompshar_sim_myCA_calcGrowthStepInside / Border (SCORE_Kernel.cpp:4046-4130, :4183-4249) */
+static int o_sweep(oracle *o, o_list *L) {
+    uint32_t active = 0;
+    L->full_n = 0;
+    L->rec_next = 0;
+    L->rec_first_not = 0;
+    for (uint32_t c = 0; c < L->nnever; c++) {
+        o_cell *e = &L->c[c];
+        if (e->act) {
+            active++;
+            double m;
+            if (o->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) m = e->P * o->mRHHAGB;
+            else if (e->P < 0.0) m = o->mLAGB;
+            else m = (e->P * o->mGS) + ((1.0 - e->P) * o->mHAGB);
+            double p = (o->Gbhalfsq * (o->rho[e->defg] - O_RHO_RX)) - 0.0;
+            if (p > O_DOUBLE_ACCURACY) {
+                double v = m * p;
+                v *= o_shape[e->inf];
+                e->frac = e->frac + (v * o->dt * o->inv_dx);
+                if (e->frac >= 1.0)
+                    if (o_append_full(L, c)) return SCORE_GM_ERR_OOM;
+            }
+            continue;
+        }
+        o_append_rec(L, c);
+    }
+    L->sv = active;
+    return 0;
+}
+
+static void o_write_cell(o_cell *d, const o_cell *parent, uint8_t code, int x, int y, int z, double carry,
+                         uint32_t defg_t, double Pnew) {
+    d->act = 1;
+    d->inf = code;
+    d->x = (int16_t)x; d->y = (int16_t)y; d->z = (int16_t)z;
+    d->frac = carry;
+    d->P = Pnew;
+    d->rxg = parent->rxg;
+    d->defg = defg_t;
+}
+
+static double o_pair_weight(oracle *o, uint32_t rxg, uint32_t defg) {
+    return oracle_mobility_weight(&o->p, &o->rxq[4 * (size_t)rxg], &o->defq[4 * (size_t)defg]);
+}
+
+/* one attempt; omp_infect_OutOfRXFrontInside (SCORE_Kernel.cpp:4252-4344) when from_border == 0
+ * and omp_infect_OutOfRXFrontBorder (:4347-4556) restricted to one region when from_border == 1. */
+static int o_infect(oracle *o, o_list *src, uint32_t pslot, int from_border, int intoborder, const o_dir *d,
+                    double carry) {
+    const o_cell par = src->c[pslot];
+    int x = par.x + d->dx, y = par.y + d->dy, z = par.z + d->dz;
+    o_list *dst;
+    if (from_border) {
+        if (x < 0) x += (int)o->nx;
+        if (x >= (int)o->nx) x -= (int)o->nx;
+        if (y < 0) y += (int)o->ny;
+        if (y >= (int)o->ny) y -= (int)o->ny;
+        if (z < 0) return 0;
+        if (z >= (int)o->nz) return 0;
+        int inside = (x > 0 && x < (int)o->nx - 1 && y > 0 && y < (int)o->ny - 1 && z > 0 && z < (int)o->nz - 1);
+        dst = inside ? &o->in : &o->bd;
+    } else {
+        dst = intoborder ? &o->bd : &o->in;
+    }
+    uint64_t cxyz = (uint64_t)x + (uint64_t)y * o->nx + (uint64_t)z * o->nxy;
+    uint32_t st = o->grid[cxyz];
+    if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) return 0;
+    o->grid[cxyz] = O_CURRENTLY_INFECTED;
+    uint32_t slot = o_next_free(dst, 1);
+    if (slot == O_INVALID_ADDRESS) return SCORE_GM_ERR_CAPACITY;
+    double Pn = par.P;
+    if (par.defg != st) Pn = o_pair_weight(o, par.rxg, st);
+    if (Pn >= o->reg_Pmax) o->reg_Pmax = Pn;
+    o_write_cell(&dst->c[slot], &par, (uint8_t)d->code, x, y, z, carry, st, Pn);
+    o->last_infected++;
+    return 0;
+}
+
+/* ompshar_sim_myCA_updateFullRXInside / Border (SCORE_Kernel.cpp:4559-4664, :4678-4774) */
+static int o_update_fullrx(oracle *o, o_list *L, int from_border) {
+    const double overFac1 = 0.0682, overFac2 = 0.0542, overFac3 = 0.0484;
+    uint32_t xmx = o->nx - 1, ymx = o->ny - 1, zmx = o->nz - 1;
+    uint32_t nupd = 0;
+    for (uint32_t k = 0; k < L->full_n; k++) {
+        uint32_t c = L->full[k];
+        uint32_t rgpid = L->c[c].rxg;
+        double overshoot = L->c[c].frac - 1.0;
+        double carry[4];
+        carry[1] = overshoot * overFac1;
+        carry[2] = overshoot * overFac2;
+        carry[3] = overshoot * overFac3;
+        int intoborder = 1;
+        if (!from_border) {
+            int x = L->c[c].x, y = L->c[c].y, z = L->c[c].z;
+            if (x > (int)(0 + 1) && x < (int)(xmx - 1) && y > (int)(0 + 1) && y < (int)(ymx - 1) && z > (int)(0 + 1) &&
+                z < (int)(zmx - 1))
+                intoborder = 0;
+        }
+        for (int dd = 0; dd < 26; dd++) {
+            int rc = o_infect(o, L, c, from_border, intoborder, &o_dirs[dd], carry[o_dirs[dd].carry]);
+            if (rc) return rc;
+        }
+        uint64_t cxyz = (uint64_t)L->c[c].x + (uint64_t)L->c[c].y * o->nx + (uint64_t)L->c[c].z * o->nxy;
+        o->grid[cxyz] = o->n_defg + rgpid;
+        o->rxcnt[L->c[c].rxg] += 1;
+        o->defcnt[L->c[c].defg] -= 1;
+        L->c[c].act = 0;
+        L->c[c].frac = 0.0;
+        nupd++;
+    }
+    L->nupdated = nupd;
+    return 0;
+}
+
+/* ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934): stable compaction of ACTIVE slots.
+ * Slots past the compacted block keep their previous contents, as in the reference. */
+static void o_defrag_list(o_list *L) {
+    uint32_t first = 0;
+    while (first < L->ntotal && L->c[first].act) first++;
+    uint32_t contiguous = first;
+    for (uint32_t c = first + 1; c < L->nnever; c++)
+        if (L->c[c].act) L->c[contiguous++] = L->c[c];
+    L->nnever = contiguous;
+}
+
+/* ---------------- set-up ---------------- */
+
+static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
+    if (!d || !p) return SCORE_GM_ERR_ARG;
+    if (d->nx < 3 || d->ny < 3 || d->nz < 3) return SCORE_GM_ERR_ARG;
+    if (d->nx > 1600 || d->ny > 1600 || d->nz > 1600) return SCORE_GM_ERR_ARG;
+    if (!d->cellgrid || !d->ori_quat || !d->defg_ori || !d->defg_rho || d->n_defg == 0) return SCORE_GM_ERR_ARG;
+    if (d->n_rxg && (!d->rxg_ori || !d->rxg_site)) return SCORE_GM_ERR_ARG;
+    if (d->cellsize <= 0.0) return SCORE_GM_ERR_ARG;
+    if (p->mobility_model != SCORE_GM_MOB_ROLLETTHOLM && p->mobility_model != SCORE_GM_MOB_SEBALDGOTTSTEIN)
+        return SCORE_GM_ERR_ARG;
+    if (p->zener_mode != 0 || p->recovery_mode != 0) return SCORE_GM_ERR_UNSUPPORTED;
+    if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
+    if (p->n_defrag && !p->defrag_at) return SCORE_GM_ERR_ARG;
+    uint64_t N = (uint64_t)d->nx * d->ny * d->nz;
+    for (uint32_t i = 0; i < d->n_defg; i++)
+        if (d->defg_ori[i] >= d->n_ori) return SCORE_GM_ERR_ARG;
+    for (uint32_t i = 0; i < d->n_rxg; i++)
+        if (d->rxg_ori[i] >= d->n_ori || d->rxg_site[i] >= N) return SCORE_GM_ERR_ARG;
+    for (uint64_t i = 0; i < N; i++)
+        if (d->cellgrid[i] >= d->n_defg) return SCORE_GM_ERR_ARG;
+    return 0;
+}
+
+void oracle_destroy(oracle *o) {
+    if (!o) return;
+    free(o->grid); free(o->defq); free(o->rxq); free(o->rho); free(o->site);
+    free(o->defcnt); free(o->rxcnt); free(o->sched_t); free(o->sched_T); free(o->defrag_at);
+    o_list_free(&o->in); o_list_free(&o->bd);
+    free(o->traj);
+    free(o);
+}
+
+/* solve_RXGROWTH set-up (SCORE_Kernel.cpp:6497-6612) including site-saturated
+ * nucleation (SCORE_GrowthMachine.cpp:50-189) */
+int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **out) {
+    int rc = o_validate(d, p);
+    if (rc) return rc;
+    oracle *o = (oracle *)calloc(1, sizeof(oracle));
+    if (!o) return SCORE_GM_ERR_OOM;
+    o->nx = d->nx; o->ny = d->ny; o->nz = d->nz; o->nxy = d->nx * d->ny;
+    o->N = (uint64_t)d->nx * d->ny * d->nz;
+    o->cellsize = d->cellsize;
+    o->inv_dx = (1.0 / d->cellsize);
+    o->n_defg = d->n_defg; o->n_rxg = d->n_rxg;
+    o->p = *p;
+    o->grid = (uint32_t *)malloc(sizeof(uint32_t) * o->N);
+    o->defq = (double *)malloc(sizeof(double) * 4 * (size_t)o->n_defg);
+    o->rxq = (double *)malloc(sizeof(double) * 4 * (size_t)(o->n_rxg + 1));
+    o->rho = (double *)malloc(sizeof(double) * o->n_defg);
+    o->site = (uint32_t *)malloc(sizeof(uint32_t) * (o->n_rxg + 1));
+    o->defcnt = (uint32_t *)calloc(o->n_defg, sizeof(uint32_t));
+    o->rxcnt = (uint32_t *)calloc(o->n_rxg + 1, sizeof(uint32_t));
+    o->sched_t = (double *)malloc(sizeof(double) * p->n_sched);
+    o->sched_T = (double *)malloc(sizeof(double) * p->n_sched);
+    o->defrag_at = (double *)malloc(sizeof(double) * (p->n_defrag + 1));
+    if (!o->grid || !o->defq || !o->rxq || !o->rho || !o->site || !o->defcnt || !o->rxcnt || !o->sched_t ||
+        !o->sched_T || !o->defrag_at) {
+        oracle_destroy(o);
+        return SCORE_GM_ERR_OOM;
+    }
+    memcpy(o->grid, d->cellgrid, sizeof(uint32_t) * o->N);
+    for (uint32_t i = 0; i < o->n_defg; i++) {
+        memcpy(&o->defq[4 * (size_t)i], &d->ori_quat[4 * (size_t)d->defg_ori[i]], 4 * sizeof(double));
+        o->rho[i] = d->defg_rho[i];
+    }
+    for (uint32_t i = 0; i < o->n_rxg; i++) {
+        memcpy(&o->rxq[4 * (size_t)i], &d->ori_quat[4 * (size_t)d->rxg_ori[i]], 4 * sizeof(double));
+        o->site[i] = d->rxg_site[i];
+    }
+    memcpy(o->sched_t, p->sched_time, sizeof(double) * p->n_sched);
+    memcpy(o->sched_T, p->sched_temp, sizeof(double) * p->n_sched);
+    if (p->n_defrag) memcpy(o->defrag_at, p->defrag_at, sizeof(double) * p->n_defrag);
+    o->p.sched_time = o->sched_t;
+    o->p.sched_temp = o->sched_T;
+    o->p.defrag_at = o->defrag_at;
+    /* init_processing (SCORE_Kernel.cpp:5888-5901) */
+    o->tprocessingend = 0.0;
+    for (uint32_t i = 0; i < p->n_sched; i++)
+        if (o->sched_t[i] >= o->tprocessingend) o->tprocessingend = o->sched_t[i];
+    /* counts after solve_REPLACE_CA_STRUCTURE (SCORE_Kernel.cpp:6390-6420) */
+    for (uint64_t i = 0; i < o->N; i++) o->defcnt[o->grid[i]]++;
+
+    /* solve_RXGROWTH prologue (SCORE_Kernel.cpp:6501-6530) */
+    o->t = 0.0; o->X = 0.0; o->Xcells = 0.0; o->step = 0;
+    o->Pmax = O_DEFAULT_PMAX;
+    o->reg_Pmax = O_DEFAULT_PMAX;
+    o->pzmin = 0.0;
+    o_update_rhomax(o);
+    o_update_temperature(o);
+    o_update_atomistic(o);
+    o_update_mobilities(o);
+    o->dt = o_dt_minimum(o);
+
+    /* ompshar_init_myRXFrontInside / Border (SCORE_Kernel.cpp:3587-3710) */
+    uint32_t nin = (uint32_t)(p->rel_cell_caching * (double)o->N);
+    uint32_t nin_rec = (uint32_t)(p->rel_cell_caching * (double)o->N * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
+    uint32_t nin_full = (uint32_t)(p->rel_cell_caching * (double)o->N * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
+    uint32_t nbd = 2 * ((o->nx * o->ny) + (o->nx * o->nz) + (o->ny * o->nz));
+    uint32_t nbd_rec = (uint32_t)((double)nbd * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
+    uint32_t nbd_full = (uint32_t)((double)nbd * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
+    if (o_list_init(&o->in, nin, nin_rec, nin_full, 1) || o_list_init(&o->bd, nbd, nbd_rec, nbd_full, 0)) {
+        oracle_destroy(o);
+        return SCORE_GM_ERR_OOM;
+    }
+
+    /* ompshar_sim_myCA_sitesaturatedNucleation + infect_JuvenileNucleation (SCORE_GrowthMachine.cpp:50-189) */
+    for (uint32_t nuc = 0; nuc < o->n_rxg; nuc++) {
+        uint32_t w = o->site[nuc];
+        uint32_t z = w / o->nxy;
+        uint32_t rem = w - (o->nxy * z);
+        uint32_t y = rem / o->nx;
+        uint32_t x = rem - (o->nx * y);
+        int inside = (x > 0 && x < o->nx - 1 && y > 0 && y < o->ny - 1 && z > 0 && z < o->nz - 1);
+        uint32_t st = o->grid[w];
+        if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) continue;
+        o->grid[w] = O_CURRENTLY_INFECTED;
+        double PP = o_pair_weight(o, nuc, st);
+        o_list *L = inside ? &o->in : &o->bd;
+        uint32_t slot = o_next_free(L, 0);
+        if (slot == O_INVALID_ADDRESS) { oracle_destroy(o); return SCORE_GM_ERR_CAPACITY; }
+        o_cell *e = &L->c[slot];
+        e->act = 1; e->inf = 26; e->x = (int16_t)x; e->y = (int16_t)y; e->z = (int16_t)z;
+        e->frac = 1.0; e->defg = st; e->rxg = nuc; e->P = PP;
+        if (PP >= o->reg_Pmax) o->reg_Pmax = PP;
+        o->nuclei++;
+    }
+    /* master reduction of myMobilityWeightMax (SCORE_Kernel.cpp:6598-6605) */
+    if (o->reg_Pmax >= o->Pmax) o->Pmax = o->reg_Pmax;
+    *out = o;
+    return 0;
+}
+
+static int o_push_traj(oracle *o, const score_gm_traj_row *r) {
+    if (o->ntraj == o->captraj) {
+        size_t nc = o->captraj ? 2 * o->captraj : 256;
+        score_gm_traj_row *nt = (score_gm_traj_row *)realloc(o->traj, nc * sizeof(*nt));
+        if (!nt) return SCORE_GM_ERR_OOM;
+        o->traj = nt;
+        o->captraj = nc;
+    }
+    o->traj[o->ntraj++] = *r;
+    return 0;
+}
+
+/* termination test (SCORE_Kernel.cpp:6619) */
+static int o_done(const oracle *o) {
+    return (o->Xcells >= (double)o->N) || (o->X > o->p.xmax) || ((int64_t)o->step > o->p.nmax) ||
+           (o->t > o->tprocessingend) || (o->t > o->p.tmax);
+}
+
+/* one iteration of the main loop (SCORE_Kernel.cpp:6615-6747); returns 1 when
+ * the termination test fired, 0 after a step, <0 on error */
+int oracle_step(oracle *o) {
+    if (o->err) return o->err;
+    if (o->completed) return 1;
+    if (o_done(o)) { o->completed = 1; return 1; }
+    if (o->defrag_cnt < o->p.n_defrag && o->Xcells >= o->defrag_at[o->defrag_cnt]) {
+        o_defrag_list(&o->in);
+        o_defrag_list(&o->bd);
+        o->defrag_cnt++;
+    }
+    int rc;
+    if ((rc = o_sweep(o, &o->in)) || (rc = o_sweep(o, &o->bd))) { o->err = rc; return rc; }
+    o->last_infected = 0;
+    if ((rc = o_update_fullrx(o, &o->in, 0)) || (rc = o_update_fullrx(o, &o->bd, 1))) { o->err = rc; return rc; }
+    score_gm_traj_row r;
+    memset(&r, 0, sizeof(r));
+    r.step = o->step;
+    r.t = o->t;
+    r.dt = o->dt;
+    r.temperature = o->T;
+    /* master section (SCORE_Kernel.cpp:6688-6742) */
+    uint32_t sv = o->in.sv + o->bd.sv;
+    double mymobmax = O_DEFAULT_PMAX;
+    if (o->reg_Pmax >= mymobmax) mymobmax = o->reg_Pmax;
+    o->Pmax = mymobmax;
+    o->Xcells = o->Xcells + ((double)(o->in.nupdated + o->bd.nupdated));
+    o->X = (o->Xcells / ((double)o->N));
+    o->sum_sv += sv;
+    o_update_rhomax(o);
+    r.sv = sv;
+    r.nfull_inside = o->in.nupdated;
+    r.nfull_border = o->bd.nupdated;
+    r.nslots_inside = o->in.nnever;
+    r.nslots_border = o->bd.nnever;
+    r.ninfected = o->last_infected;
+    r.X = o->X;
+    r.xcells = o->Xcells;
+    r.rhomax = o->rhomax;
+    o->t += o->dt;
+    o_update_temperature(o);
+    o_update_atomistic(o);
+    o->pzmin = 0.0;
+    o_update_mobilities(o);
+    o->dt = o_dt_minimum(o);
+    o->step++;
+    if ((rc = o_push_traj(o, &r))) { o->err = rc; return rc; }
+    return 0;
+}
+
+int oracle_run(oracle *o, uint32_t max_steps) {
+    for (uint32_t k = 0; k < max_steps; k++) {
+        int rc = oracle_step(o);
+        if (rc != 0) return rc;
+    }
+    return 0;
+}
+
+/* ---------------- accessors ---------------- */
+
+int oracle_get_grainmap(const oracle *o, uint32_t *out, int encoding) {
+    for (uint64_t i = 0; i < o->N; i++) {
+        uint32_t v = o->grid[i];
+        if (encoding == SCORE_GM_MAP_RAW2GID) v = (v == O_CURRENTLY_INFECTED) ? 1u : 2u + v;
+        out[i] = v;
+    }
+    return 0;
+}
+
+int oracle_get_counts(const oracle *o, uint32_t *defg, uint32_t *rxg) {
+    if (defg) memcpy(defg, o->defcnt, sizeof(uint32_t) * o->n_defg);
+    if (rxg) memcpy(rxg, o->rxcnt, sizeof(uint32_t) * o->n_rxg);
+    return 0;
+}
+
+size_t oracle_get_trajectory(const oracle *o, score_gm_traj_row *rows, size_t cap) {
+    size_t n = o->ntraj < cap ? o->ntraj : cap;
+    if (rows && n) memcpy(rows, o->traj, n * sizeof(*rows));
+    return o->ntraj;
+}
+
+int oracle_get_result(const oracle *o, score_gm_result *r) {
+    memset(r, 0, sizeof(*r));
+    r->t = o->t; r->X = o->X; r->xcells = o->Xcells; r->step = o->step; r->completed = o->completed;
+    r->sum_sv = o->sum_sv; r->nuclei_placed = o->nuclei; r->pmax = o->Pmax;
+    return o->err;
+}
+
+double oracle_get_dt(const oracle *o) { return o->dt; }
+
+int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8_t *act, uint8_t *inf, uint32_t *cell,
+                     double *frac, double *P, uint32_t *defg, uint32_t *rxg) {
+    const o_list *L = list == SCORE_GM_LIST_BORDER ? &o->bd : &o->in;
+    *n = L->nnever;
+    uint32_t m = L->nnever < cap ? L->nnever : cap;
+    for (uint32_t i = 0; i < m; i++) {
+        const o_cell *e = &L->c[i];
+        if (act) act[i] = e->act;
+        if (inf) inf[i] = e->inf;
+        if (cell) cell[i] = (uint32_t)((int64_t)e->x + (int64_t)e->y * o->nx + (int64_t)e->z * o->nxy);
+        if (frac) frac[i] = e->frac;
+        if (P) P[i] = e->P;
+        if (defg) defg[i] = e->defg;
+        if (rxg) rxg[i] = e->rxg;
+    }
+    return 0;
+}
+
+/* list capacities the reference derives (SCORE_Kernel.cpp:3589,3615,3631,3679,3682,3697):
+ * inside, inside recycle cap, border, border recycle cap */
+void oracle_capacities(const oracle *o, uint32_t *out4) {
+    out4[0] = o->in.ntotal; out4[1] = o->in.rec_cap; out4[2] = o->bd.ntotal; out4[3] = o->bd.rec_cap;
+}

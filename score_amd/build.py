@@ -1,0 +1,87 @@
+"""Build the native parts in-tree.
+
+    python -m score_amd.build            # libscore_gm.so (gfx950) + libsynth.so
+    python -m score_amd.build --oracle   # also oracle/liboracle.so (test infrastructure)
+
+hipcc compiles device and host code of the growth machine for gfx950 only,
+with -ffp-contract=off so no multiply-add is fused (the results are compared
+bit for bit with a serial FP64 run).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+    "-I", INCLUDE,
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_gm(out: str | None = None) -> str:
+    out = out or os.path.join(HERE, "libscore_gm.so")
+    srcs = [os.path.join(CSRC, "gm_kernels.hip"), os.path.join(CSRC, "gm_host.cpp")]
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_synth(out: str | None = None) -> str:
+    out = out or os.path.join(HERE, "libsynth.so")
+    cc = shutil.which("gcc") or "cc"
+    tmp = out + ".tmp"
+    _run([cc, "-O2", "-std=c99", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+          os.path.join(CSRC, "synth.c"), "-o", tmp, "-lm"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> str:
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "-B"])
+    return os.path.join(odir, "liboracle.so")
+
+
+def build_all(oracle: bool = True) -> None:
+    build_synth()
+    build_gm()
+    if oracle:
+        build_oracle()
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--oracle", action="store_true", help="also build the oracle (test infrastructure)")
+    ap.add_argument("--only", choices=["gm", "synth", "oracle"], default=None)
+    a = ap.parse_args(argv)
+    if a.only == "gm":
+        build_gm()
+    elif a.only == "synth":
+        build_synth()
+    elif a.only == "oracle":
+        build_oracle()
+    else:
+        build_all(oracle=a.oracle)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

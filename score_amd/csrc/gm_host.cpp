@@ -1,0 +1,892 @@
+/*
+ * gm_host.cpp -- C ABI of the growth machine (include/score_gm.h).
+ *
+ * The host does what SCORE's master thread does between growth steps
+ * (SCORE_Kernel.cpp:6688-6742): it advances t, re-evaluates T(t), G(T), the
+ * mobilities and the adaptive time step with the same FP64 expressions and the
+ * same glibc transcendentals, so every scalar is bit-identical to the
+ * reference's.  It also evaluates mobility weights P that the device cannot
+ * reproduce bit-exactly (glibc acos/pow/exp) when a new nucleus/deformed-grain
+ * pair is met; the device caches them.  All per-cell work runs in
+ * gm_kernels.hip.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/score_gm.h"
+#include "gm_internal.h"
+
+namespace {
+
+/* constants (SCORE_Defs_Physics.h:51-160, SCORE_Math.h:61) */
+const double kBoltzmann = 1.3806488e-23;
+const double kPi = 3.1415926535897932384626433832795;
+const double kLagbHagb = ((15.0) / (180.0) * (kPi));
+const double kMaxDev40_111 = ((10.0) / (180.0) * (kPi));
+const double kDoubleAccuracy = 5.0e-16;
+const double kInitialDeltaT = 1e-6;
+const double kInfinite = 1e36;
+const double kSmallHeatRate = 0.000277777777777777;
+const double kSmallHeat = 1.0;
+const double kRhoWellAnnealed = 1e8;
+const double kDefaultPmax = 0.0;
+const double kDefaultSmallNumber = 1e-6;
+const double kFullRecyclingAtt = 3.0;
+
+/* Algorithmic HBM bytes (what the work needs at minimum, not what the kernels move):
+ * sweep, per ACTIVE slot: activity 1 + infector 1 + P 8 + defg 4 + rxFrac read 8 + write 8;
+ * per other scanned slot: activity 1.
+ * infection, per completed cell: FullRX entry 4, parent record (cell, rxFrac, P, defg, rxg) 28,
+ * 26 cell-grid reads 104, retirement (grid 4, activity 1, rxFrac 8, two grain counts 8) 21;
+ * per new infection: record 30, grid write 4, claim atomic 4 and reset 4. */
+const uint64_t kSweepBytesActive = 30, kSweepBytesIdle = 1;
+const uint64_t kInfectBytesParent = 4 + 28 + 104 + 21, kInfectBytesNew = 30 + 4 + 4 + 4;
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *what) {
+    g_last_error = what;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+    g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+    return SCORE_GM_ERR_HIP;
+}
+
+#define HIPCHK(x)                                              \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) return hip_fail(e_, #x);         \
+    } while (0)
+
+/* ---------------------------------------------------------------- orientation math */
+
+/* q^-1 p with the Hamilton product of multiply_quat_degraef (SCORE_Math.cpp:230-239, :564-572) */
+void misorientation(const double *p, const double *q, double *r) {
+    const double m0 = q[0], m1 = -1.0 * q[1], m2 = -1.0 * q[2], m3 = -1.0 * q[3];
+    r[0] = m0 * p[0] - m1 * p[1] - m2 * p[2] - m3 * p[3];
+    r[1] = p[0] * m1 + m0 * p[1] + (m2 * p[3] - m3 * p[2]);
+    r[2] = p[0] * m2 + m0 * p[2] + (-(m1 * p[3]) + m3 * p[1]);
+    r[3] = p[0] * m3 + m0 * p[3] + (m1 * p[2] - m2 * p[1]);
+}
+
+/* six Grimmer candidates (SCORE_Math.cpp:592-602) */
+void grimmer(const double *r, double (*c)[4]) {
+    const double a = r[0], b = r[1], cc = r[2], d = r[3];
+    const double fac = 1.0 / sqrt(2.0);
+    c[0][0] = (a + b) * fac; c[0][1] = (a - b) * fac; c[0][2] = (cc + d) * fac; c[0][3] = (cc - d) * fac;
+    c[1][0] = (a + cc) * fac; c[1][1] = (a - cc) * fac; c[1][2] = (b + d) * fac; c[1][3] = (b - d) * fac;
+    c[2][0] = (a + d) * fac; c[2][1] = (a - d) * fac; c[2][2] = (b + cc) * fac; c[2][3] = (b - cc) * fac;
+    c[3][0] = (a + b + cc + d) * 0.5; c[3][1] = (a + b - cc - d) * 0.5; c[3][2] = (a - b + cc - d) * 0.5; c[3][3] = (a - b - cc + d) * 0.5;
+    c[4][0] = (a + b + cc - d) * 0.5; c[4][1] = (a + b - cc + d) * 0.5; c[4][2] = (a - b + cc + d) * 0.5; c[4][3] = (a - b - cc - d) * 0.5;
+    c[5][0] = a; c[5][1] = b; c[5][2] = cc; c[5][3] = d;
+}
+
+/* disori_angle_oriquat_cubic_imm (SCORE_Math.cpp:558-622) */
+double disori_angle(const double *p, const double *q) {
+    double r[4], c[6][4];
+    misorientation(p, q, r);
+    grimmer(r, c);
+    double omega = 0.0;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 4; j++)
+            if (fabs(c[i][j]) > omega) omega = fabs(c[i][j]);
+    if (omega < (1.0 - kDoubleAccuracy)) return 2.0 * acos(omega);
+    return 0.0;
+}
+
+/* disori_fromoriquat_cubic_imm (SCORE_Math.cpp:1067-1131) */
+void disori_quat(const double *p, const double *q, double *out) {
+    double r[4], c[6][4];
+    misorientation(p, q, r);
+    grimmer(r, c);
+    int mi = 0;
+    double mx = 0.0;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 4; j++)
+            if (fabs(c[i][j]) > mx) { mx = fabs(c[i][j]); mi = i; }
+    double v[4] = {fabs(c[mi][0]), fabs(c[mi][1]), fabs(c[mi][2]), fabs(c[mi][3])};
+    std::sort(v, v + 4);
+    out[0] = v[3]; out[1] = v[2]; out[2] = v[1]; out[3] = v[0];
+}
+
+double rh_weight_of_theta(const score_gm_params &p, double theta) {
+    return 1.0 - (p.rh_cut * exp(-1.0 * p.rh_trans * pow((theta / kLagbHagb), p.rh_exponent)));
+}
+
+/* calc_mobilityweight_rollettholm / _sebaldgottstein (SCORE_GrainBoundaryPhysics.cpp:50-135),
+ * identical arithmetic to reg_calc_mobilityweight (SCORE_Kernel.cpp:2460-2527) */
+double mobility_weight(const score_gm_params &p, const double *qrx, const double *qdef) {
+    const double theta = disori_angle(qrx, qdef);
+    if (p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) return rh_weight_of_theta(p, theta);
+    if (theta <= kLagbHagb) return -1.0;
+    double qdis[4] = {1.0, 0.0, 0.0, 0.0};
+    disori_quat(qrx, qdef, qdis);
+    const double sq3 = 1 / sqrt(3.0);
+    const double oneNinth = 1.0 / 9.0;
+    const double m40[4] = {cos(oneNinth * kPi), sq3 * sin(oneNinth * kPi), sq3 * sin(oneNinth * kPi),
+                           sq3 * sin(oneNinth * kPi)};
+    const double dev = disori_angle(qdis, m40);
+    double w = 0.0;
+    if (dev < kMaxDev40_111) {
+        const double cc = cos(0.5 * kPi * dev / kMaxDev40_111);
+        w = cc * cc;
+    }
+    return w;
+}
+
+/* Largest omega for which the Rollett-Holm weight is exactly 1.0 with margin:
+ * the device returns P = 1.0 for omega <= this value and asks the host otherwise.
+ * Returns -1 when no safe threshold exists (shortcut disabled). */
+double rh_omega_threshold(const score_gm_params &p) {
+    if (p.mobility_model != SCORE_GM_MOB_ROLLETTHOLM) return -1.0;
+    if (!(p.rh_cut > 0.0 && p.rh_trans > 0.0 && p.rh_exponent > 0.0)) return -1.0;
+    if (rh_weight_of_theta(p, kPi) != 1.0) return -1.0;
+    double lo = 0.0, hi = kPi; /* invariant: weight(hi) == 1.0 */
+    if (rh_weight_of_theta(p, lo) == 1.0) return 1.0; /* every pair has P == 1.0 */
+    for (int it = 0; it < 200; it++) {
+        double mid = 0.5 * (lo + hi);
+        if (rh_weight_of_theta(p, mid) == 1.0) hi = mid;
+        else lo = mid;
+    }
+    /* margin of 0.2 degree: the weight's deficit falls by a large factor over it */
+    const double theta_hi = hi + 0.2 / 180.0 * kPi;
+    if (theta_hi >= 2.0 * acos(0.85)) return -1.0; /* beyond the largest cubic disorientation */
+    for (int i = 0; i <= 200000; i++) {
+        double th = theta_hi + (kPi - theta_hi) * (double)i / 200000.0;
+        if (rh_weight_of_theta(p, th) != 1.0) return -1.0;
+    }
+    double om = cos(0.5 * theta_hi);
+    /* a few ulps below, so that 2*acos(om') >= theta_hi whatever the last-bit behaviour of acos */
+    for (int i = 0; i < 4; i++) om = nextafter(om, 0.0);
+    if (!(2.0 * acos(om) >= theta_hi)) return -1.0;
+    return om;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ handle */
+
+struct score_gm_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GmDev d{};
+    GmStats *hst = nullptr; /* pinned mirror of d.st */
+    score_gm_params p{};
+    std::vector<double> sched_t, sched_T, defrag_at;
+    std::vector<double> defq, rxq;
+    double cellsize = 0.0, inv_dx = 0.0;
+    double tprocessingend = 0.0;
+    /* caHdl scalars */
+    double t = 0.0, X = 0.0, Xcells = 0.0, dt = 0.0, T = 0.0, G = 0.0, b = 0.0, Gbhalfsq = 0.0;
+    double mLAGB = 0.0, mHAGB = 0.0, mGS = 0.0, mRHHAGB = 0.0, pzmin = 0.0, rhomax = 0.0;
+    double Pmax = kDefaultPmax, regPmax = kDefaultPmax;
+    uint32_t step = 0;
+    uint32_t nI = 0, nB = 0;
+    uint32_t nuclei = 0;
+    int completed = 0;
+    int err = 0;
+    uint64_t sum_sv = 0;
+    double loop_seconds = 0.0;
+    GmPParams pp{};
+    uint32_t hcount = 0; /* entries in the P cache */
+    std::vector<score_gm_traj_row> traj;
+    /* profiling */
+    bool prof = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    score_gm_profile profc{};
+    /* staging for pending P */
+    std::vector<unsigned long long> pend_keys;
+    std::vector<double> pend_vals;
+    double *d_pend_vals = nullptr;
+    uint32_t pend_vals_cap = 0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        g_last_error = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return SCORE_GM_ERR_OOM;
+    }
+    return 0;
+}
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+void free_device(score_gm_handle *h) {
+    GmDev &d = h->d;
+    void *ptrs[] = {d.grid, d.claim, d.act, d.inf, d.cell, d.defg, d.rxg, d.frac, d.P, d.mfull, d.mrec, d.tcntF,
+                    d.tcntR, d.full, d.recI, d.recB, d.pwinI, d.pwinB, d.pcntI, d.pcntB, d.rho, d.defq, d.rxq,
+                    d.defcnt, d.rxcnt, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.scan_bsum, d.st, h->d_pend_vals};
+    for (void *q : ptrs) dfree(q);
+    memset(&h->d, 0, sizeof(h->d));
+    h->d_pend_vals = nullptr;
+    if (h->hst) (void)hipHostFree(h->hst);
+    h->hst = nullptr;
+    for (hipEvent_t &ev : h->ev)
+        if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+}
+
+int validate(const score_gm_domain *dom, const score_gm_params *prm) {
+    if (!dom || !prm) return fail(SCORE_GM_ERR_ARG, "null domain or params");
+    if (dom->nx < 3 || dom->ny < 3 || dom->nz < 3) return fail(SCORE_GM_ERR_ARG, "edge length below 3 cells");
+    if (dom->nx > 1600 || dom->ny > 1600 || dom->nz > 1600)
+        return fail(SCORE_GM_ERR_ARG, "edge length above 1600 cells (CA_DIMENSIONS_MAXIMUM)");
+    if (!dom->cellgrid || !dom->ori_quat || !dom->defg_ori || !dom->defg_rho || dom->n_defg == 0)
+        return fail(SCORE_GM_ERR_ARG, "missing domain arrays");
+    if (dom->n_rxg && (!dom->rxg_ori || !dom->rxg_site)) return fail(SCORE_GM_ERR_ARG, "missing nucleus arrays");
+    if (!(dom->cellsize > 0.0)) return fail(SCORE_GM_ERR_ARG, "cellsize must be positive");
+    if (dom->n_defg >= 4096000000u || dom->n_rxg >= 4096000000u - dom->n_defg)
+        return fail(SCORE_GM_ERR_ARG, "grain ids exceed CA_GRAINIDS_MAXIMUM");
+    if (prm->mobility_model != SCORE_GM_MOB_ROLLETTHOLM && prm->mobility_model != SCORE_GM_MOB_SEBALDGOTTSTEIN)
+        return fail(SCORE_GM_ERR_ARG, "unknown mobility model");
+    if (prm->zener_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "dispersoid drag is not implemented");
+    if (prm->recovery_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "recovery is not implemented");
+    if (prm->n_defrag != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "on-the-fly defragmentation is not implemented");
+    if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
+        return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");
+    if (!(prm->maxfill > 0.0)) return fail(SCORE_GM_ERR_ARG, "maxfill must be positive");
+    uint64_t N = (uint64_t)dom->nx * dom->ny * dom->nz;
+    for (uint32_t i = 0; i < dom->n_defg; i++)
+        if (dom->defg_ori[i] >= dom->n_ori) return fail(SCORE_GM_ERR_ARG, "defg_ori out of range");
+    for (uint32_t i = 0; i < dom->n_rxg; i++)
+        if (dom->rxg_ori[i] >= dom->n_ori || dom->rxg_site[i] >= N)
+            return fail(SCORE_GM_ERR_ARG, "nucleus orientation or site out of range");
+    for (uint64_t i = 0; i < N; i++)
+        if (dom->cellgrid[i] >= dom->n_defg) return fail(SCORE_GM_ERR_ARG, "cellgrid holds a non-deformed id");
+    return 0;
+}
+
+/* ------------------------------------------------------------- physics scalars */
+
+/* update_temperature (SCORE_Kernel.cpp:5318-5337) */
+void update_temperature(score_gm_handle *h) {
+    const size_t n = h->sched_t.size();
+    size_t i = 0;
+    while (i < n && h->sched_t[i] < h->t) i++;
+    if (i == 0) { h->T = h->sched_T[0]; return; }
+    if (i >= n) { h->T = h->sched_T[n - 1]; return; }
+    h->T = h->sched_T[i - 1] +
+           ((h->sched_T[i] - h->sched_T[i - 1]) / (h->sched_t[i] - h->sched_t[i - 1])) * (h->t - h->sched_t[i - 1]);
+}
+
+/* get_shearmodulus / get_burgersvector (SCORE_GeneralPhysics.cpp:65-90), update_atomisticproperties (SCORE_Kernel.cpp:5340-5352) */
+void update_atomistic(score_gm_handle *h) {
+    const double T = h->T;
+    h->G = h->p.G0 * (-1.38e-9 * (T * T * T) + 1.21e-6 * (T * T) - 4.58e-4 * T + 1.00);
+    h->b = h->p.b0;
+    h->Gbhalfsq = 0.5 * h->G * (h->b * h->b);
+}
+
+/* update_intrinsicmobilities (SCORE_Kernel.cpp:5355-5368) */
+void update_mobilities(score_gm_handle *h) {
+    const double kT = kBoltzmann * h->T;
+    h->mLAGB = h->p.lagb_m0 * exp(-1.0 * h->p.lagb_hact / kT);
+    h->mHAGB = h->p.hagb_m0 * exp(-1.0 * h->p.hagb_hact / kT);
+    h->mGS = h->p.gs_m0 * exp(-1.0 * h->p.gs_hact / kT);
+    h->mRHHAGB = h->p.rh_hagb_m0 * exp(-1.0 * h->p.rh_hagb_hact / kT);
+}
+
+/* get_dtmax_instantslope_temp (SCORE_Kernel.cpp:5561-5591) */
+double dt_temp(const score_gm_handle *h) {
+    const size_t n = h->sched_t.size();
+    size_t i = 0;
+    while (i < n && h->sched_t[i] < h->t) i++;
+    if (i == 0) return kInitialDeltaT;
+    if (i < n) {
+        double dTdt = (fabs(h->sched_T[i] - h->sched_T[i - 1]) / (h->sched_t[i] - h->sched_t[i - 1]));
+        if (dTdt <= kSmallHeatRate) dTdt = kSmallHeatRate;
+        double dtmax = (kSmallHeat / dTdt);
+        const double dtintvl = h->sched_t[i] - h->sched_t[i - 1];
+        if (dtmax >= dtintvl) dtmax = dtintvl;
+        return dtmax;
+    }
+    return kInfinite;
+}
+
+/* get_dtmax_instantslope_cells (SCORE_Kernel.cpp:5519-5558) */
+double dt_cells(const score_gm_handle *h) {
+    const double kT = kBoltzmann * h->T;
+    if (h->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) {
+        const double mmax = h->p.rh_hagb_m0 * exp(-1.0 * h->p.rh_hagb_hact / kT);
+        const double vmax = mmax * ((h->Gbhalfsq * h->rhomax) - h->pzmin);
+        return ((h->p.maxfill * h->cellsize) / vmax);
+    }
+    const double mgs = h->p.gs_m0 * exp(-1.0 * h->p.gs_hact / kT);
+    const double mhagb = h->p.hagb_m0 * exp(-1.0 * h->p.hagb_hact / kT);
+    double m_max = (h->Pmax * mgs) + ((1.0 - h->Pmax) * mhagb);
+    if (h->Pmax < (0.0 - kDefaultSmallNumber)) m_max = h->p.lagb_m0 * exp(-1.0 * h->p.lagb_hact / kT);
+    const double v_max = m_max * ((h->Gbhalfsq * h->rhomax) - h->pzmin);
+    return ((h->p.maxfill * h->cellsize) / v_max);
+}
+
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); recovery, drag and nucleation
+ * rules are INFINITE for the modes implemented */
+double dt_minimum(const score_gm_handle *h) {
+    double mn = kInfinite;
+    const double a = dt_temp(h);
+    if (a < mn) mn = a;
+    const double c = dt_cells(h);
+    if (c < mn) mn = c;
+    return mn;
+}
+
+/* update_myrhomax (SCORE_RecoveryModels.cpp:146-165) from the device maximum */
+double rhomax_from_bits(unsigned long long bits) {
+    double r = kRhoWellAnnealed;
+    if (bits != 0ull) {
+        double v;
+        memcpy(&v, &bits, sizeof(v));
+        if (v >= r) r = v;
+    }
+    return r;
+}
+
+/* termination test (SCORE_Kernel.cpp:6619) */
+bool done(const score_gm_handle *h) {
+    return (h->Xcells >= (double)h->d.N) || (h->X > h->p.xmax) || ((int64_t)h->step > h->p.nmax) ||
+           (h->t > h->tprocessingend) || (h->t > h->p.tmax);
+}
+
+uint32_t pow2_at_least(uint64_t v) {
+    uint64_t c = 1;
+    while (c < v) c <<= 1;
+    return (uint32_t)std::min<uint64_t>(c, 1ull << 31);
+}
+
+/* per-parent and per-tile arrays sized for the current occupancy */
+int ensure_step_capacity(score_gm_handle *h) {
+    GmDev &d = h->d;
+    const uint32_t tiles = (h->nI + GM_TILE - 1) / GM_TILE + (h->nB + GM_TILE - 1) / GM_TILE;
+    if (tiles > d.tileCap) {
+        uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
+        dfree(d.mfull); dfree(d.mrec); dfree(d.tcntF); dfree(d.tcntR);
+        d.mfull = d.mrec = nullptr; d.tcntF = d.tcntR = nullptr;
+        int rc;
+        if ((rc = dalloc(&d.mfull, nc)) || (rc = dalloc(&d.mrec, nc)) || (rc = dalloc(&d.tcntF, nc)) ||
+            (rc = dalloc(&d.tcntR, nc)))
+            return rc;
+        d.tileCap = nc;
+    }
+    const uint32_t parents = h->nI + h->nB;
+    if (parents > d.parentCap) {
+        uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
+        dfree(d.full); dfree(d.pwinI); dfree(d.pwinB); dfree(d.pcntI); dfree(d.pcntB);
+        d.full = d.pwinI = d.pwinB = d.pcntI = d.pcntB = nullptr;
+        int rc;
+        if ((rc = dalloc(&d.full, nc)) || (rc = dalloc(&d.pwinI, nc)) || (rc = dalloc(&d.pwinB, nc)) ||
+            (rc = dalloc(&d.pcntI, nc)) || (rc = dalloc(&d.pcntB, nc)))
+            return rc;
+        d.parentCap = nc;
+    }
+    return 0;
+}
+
+/* Evaluate the pending pairs with glibc and store them in the device cache. */
+int resolve_pending(score_gm_handle *h, uint32_t npend) {
+    if (npend == 0) return 0;
+    GmDev &d = h->d;
+    h->pend_keys.resize(npend);
+    h->pend_vals.resize(npend);
+    HIPCHK(hipMemcpyAsync(h->pend_keys.data(), d.pend_key, sizeof(unsigned long long) * npend, hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0; i < npend; i++) {
+        const unsigned long long key = h->pend_keys[i] - 1ull;
+        const uint32_t rxg = (uint32_t)(key / d.ndefg), g = (uint32_t)(key % d.ndefg);
+        const double P = mobility_weight(h->p, &h->rxq[4 * (size_t)rxg], &h->defq[4 * (size_t)g]);
+        h->pend_vals[i] = P;
+        if (P >= h->regPmax) h->regPmax = P;
+    }
+    if (npend > h->pend_vals_cap) {
+        dfree(h->d_pend_vals);
+        h->d_pend_vals = nullptr;
+        uint32_t nc = std::max<uint32_t>(npend, 2 * h->pend_vals_cap);
+        int rc = dalloc(&h->d_pend_vals, nc);
+        if (rc) return rc;
+        h->pend_vals_cap = nc;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_pend_vals, h->pend_vals.data(), sizeof(double) * npend, hipMemcpyHostToDevice,
+                          h->stream));
+    HIPCHK(gm_launch_resolve_pending(d, h->d_pend_vals, npend, h->stream));
+    h->hcount += npend;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+/* Double the cache once it is half full: markers in the front are resolved
+ * first, then every entry is re-inserted into the larger table. */
+int grow_pcache(score_gm_handle *h) {
+    GmDev &d = h->d;
+    if ((uint64_t)h->hcount * 2 <= d.hcap) return 0;
+    HIPCHK(gm_launch_resolve_front(d, h->nI, h->nB, h->stream));
+    std::vector<unsigned long long> keys(d.hcap);
+    std::vector<double> vals(d.hcap);
+    HIPCHK(hipMemcpyAsync(keys.data(), d.hkeys, sizeof(unsigned long long) * d.hcap, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(vals.data(), d.hvals, sizeof(double) * d.hcap, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<unsigned long long> k2;
+    std::vector<double> v2;
+    for (uint32_t i = 0; i < d.hcap; i++)
+        if (keys[i]) { k2.push_back(keys[i]); v2.push_back(vals[i]); }
+    const uint32_t ncap = d.hcap * 2u;
+    dfree(d.hkeys); dfree(d.hvals); dfree(d.pend_slot); dfree(d.pend_key);
+    d.hkeys = nullptr; d.hvals = nullptr; d.pend_slot = nullptr; d.pend_key = nullptr;
+    int rc;
+    if ((rc = dalloc(&d.hkeys, ncap)) || (rc = dalloc(&d.hvals, ncap)) || (rc = dalloc(&d.pend_slot, ncap)) ||
+        (rc = dalloc(&d.pend_key, ncap)))
+        return rc;
+    d.hcap = ncap;
+    HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, ncap, h->stream));
+    HIPCHK(gm_launch_fill_f64_bits(d.hvals, GM_PENDING_TAG, ncap, h->stream));
+    unsigned long long *dk = nullptr;
+    double *dv = nullptr;
+    if ((rc = dalloc(&dk, k2.size())) || (rc = dalloc(&dv, v2.size()))) { dfree(dk); return rc; }
+    HIPCHK(hipMemcpyAsync(dk, k2.data(), sizeof(unsigned long long) * k2.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dv, v2.data(), sizeof(double) * v2.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(gm_launch_hash_insert(d, dk, dv, (uint32_t)k2.size(), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(dk);
+    dfree(dv);
+    return 0;
+}
+
+/* one iteration of SCORE's main loop (SCORE_Kernel.cpp:6615-6747) */
+int step_once(score_gm_handle *h) {
+    if (h->err) return h->err;
+    if (h->completed) return 1;
+    if (done(h)) { h->completed = 1; return 1; }
+    GmDev &d = h->d;
+    int rc = ensure_step_capacity(h);
+    if (rc) return h->err = rc;
+    GmSweepParams sp;
+    sp.model = h->p.mobility_model;
+    sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
+    sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
+    const bool prof = h->prof;
+    const uint32_t slots0 = h->nI + h->nB;
+    auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(h->ev[i], h->stream) : hipSuccess; };
+    const auto th0 = std::chrono::steady_clock::now();
+    hipError_t e;
+    if ((e = gm_launch_begin_step(d, h->stream)) != hipSuccess || (e = mark(0)) != hipSuccess ||
+        (e = gm_launch_sweep(d, sp, h->nI, h->nB, h->stream)) != hipSuccess || (e = mark(1)) != hipSuccess ||
+        (e = gm_launch_compact(d, h->nI, h->nB, h->stream)) != hipSuccess || (e = mark(2)) != hipSuccess ||
+        (e = gm_launch_infect(d, h->pp, h->stream)) != hipSuccess || (e = mark(3)) != hipSuccess ||
+        (e = gm_launch_rhomax(d, h->stream)) != hipSuccess || (e = mark(4)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return h->err = hip_fail(e, "growth step");
+    const GmStats st = *h->hst;
+    if (prof) {
+        float ms[4];
+        for (int i = 0; i < 4; i++)
+            if ((e = hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1])) != hipSuccess)
+                return h->err = hip_fail(e, "hipEventElapsedTime");
+        score_gm_profile &pf = h->profc;
+        pf.steps++;
+        pf.ms_sweep += ms[0];
+        pf.ms_compact += ms[1];
+        pf.ms_infect += ms[2];
+        pf.ms_tail += ms[3];
+        pf.sweep_slots += slots0;
+        pf.sweep_active += st.sv;
+        pf.sweep_bytes += kSweepBytesActive * (uint64_t)st.sv + kSweepBytesIdle * (uint64_t)(slots0 - st.sv);
+        pf.infect_parents += st.nfull;
+        pf.infect_new += (uint64_t)st.winI + st.winB;
+        pf.infect_bytes += kInfectBytesParent * (uint64_t)st.nfull + kInfectBytesNew * ((uint64_t)st.winI + st.winB);
+        pf.pending_resolved += st.npend;
+    }
+    if (st.err) {
+        if (st.err & GM_ERRF_BORDER_FULL)
+            return h->err = fail(SCORE_GM_ERR_CAPACITY, "Border front list full (SCORE would write out of bounds)");
+        if (st.err & GM_ERRF_INSIDE_FULL) return h->err = fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
+        return h->err = fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
+    }
+    if ((rc = resolve_pending(h, st.npend))) return h->err = rc;
+    if (st.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
+    h->nI = st.nI;
+    h->nB = st.nB;
+    if ((rc = grow_pcache(h))) return h->err = rc;
+    /* master section (SCORE_Kernel.cpp:6688-6742) */
+    score_gm_traj_row r;
+    memset(&r, 0, sizeof(r));
+    r.step = h->step;
+    r.t = h->t;
+    r.dt = h->dt;
+    r.temperature = h->T;
+    double mymobmax = kDefaultPmax;
+    if (h->regPmax >= mymobmax) mymobmax = h->regPmax;
+    h->Pmax = mymobmax;
+    h->Xcells = h->Xcells + ((double)st.nfull);
+    h->X = (h->Xcells / ((double)d.N));
+    h->sum_sv += st.sv;
+    h->rhomax = rhomax_from_bits(st.rhomax_bits);
+    r.sv = st.sv;
+    r.nfull_inside = st.nfullI;
+    r.nfull_border = st.nfull - st.nfullI;
+    r.nslots_inside = st.nI;
+    r.nslots_border = st.nB;
+    r.ninfected = st.winI + st.winB;
+    r.npending = st.npend;
+    r.X = h->X;
+    r.xcells = h->Xcells;
+    r.rhomax = h->rhomax;
+    h->t += h->dt;
+    update_temperature(h);
+    update_atomistic(h);
+    h->pzmin = 0.0;
+    update_mobilities(h);
+    h->dt = dt_minimum(h);
+    h->step++;
+    h->traj.push_back(r);
+    if (prof) h->profc.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
+    return 0;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------- ABI */
+
+extern "C" {
+
+int score_gm_abi_version(void) { return SCORE_GM_ABI_VERSION; }
+
+const char *score_gm_strerror(int status) {
+    switch (status) {
+        case SCORE_GM_OK: return "ok";
+        case SCORE_GM_ERR_ARG: return g_last_error.empty() ? "invalid argument" : g_last_error.c_str();
+        case SCORE_GM_ERR_OOM: return g_last_error.empty() ? "out of memory" : g_last_error.c_str();
+        case SCORE_GM_ERR_HIP: return g_last_error.empty() ? "HIP error" : g_last_error.c_str();
+        case SCORE_GM_ERR_CAPACITY: return g_last_error.empty() ? "front list capacity exceeded" : g_last_error.c_str();
+        case SCORE_GM_ERR_UNSUPPORTED: return g_last_error.empty() ? "unsupported option" : g_last_error.c_str();
+        case SCORE_GM_ERR_STATE: return g_last_error.empty() ? "invalid state" : g_last_error.c_str();
+        case SCORE_GM_ERR_NODEVICE: return "no usable gfx950 device";
+        default: return "unknown status";
+    }
+}
+
+int score_gm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    int usable = 0;
+    for (int i = 0; i < n; i++) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, i) == hipSuccess && strncmp(prop.gcnArchName, "gfx950", 6) == 0) usable++;
+    }
+    return usable;
+}
+
+int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int device, score_gm_handle **out) {
+    if (!out) return fail(SCORE_GM_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = validate(dom, prm);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return fail(SCORE_GM_ERR_NODEVICE, "no HIP device");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SCORE_GM_ERR_NODEVICE, "device is not gfx950");
+    HIPCHK(hipSetDevice(device));
+
+    score_gm_handle *h = new (std::nothrow) score_gm_handle();
+    if (!h) return fail(SCORE_GM_ERR_OOM, "handle");
+    h->device = device;
+    h->p = *prm;
+    h->sched_t.assign(prm->sched_time, prm->sched_time + prm->n_sched);
+    h->sched_T.assign(prm->sched_temp, prm->sched_temp + prm->n_sched);
+    h->p.sched_time = h->sched_t.data();
+    h->p.sched_temp = h->sched_T.data();
+    h->p.defrag_at = nullptr;
+    h->cellsize = dom->cellsize;
+    h->inv_dx = (1.0 / dom->cellsize);
+    /* init_processing (SCORE_Kernel.cpp:5888-5901) */
+    h->tprocessingend = 0.0;
+    for (double tt : h->sched_t)
+        if (tt >= h->tprocessingend) h->tprocessingend = tt;
+
+    GmDev &d = h->d;
+    d.nx = dom->nx; d.ny = dom->ny; d.nz = dom->nz; d.nxy = dom->nx * dom->ny;
+    d.N = (uint64_t)dom->nx * dom->ny * dom->nz;
+    d.ndefg = dom->n_defg; d.nrxg = dom->n_rxg;
+    /* list capacities (ompshar_init_myRXFrontInside/Border, SCORE_Kernel.cpp:3587-3710):
+     * the Inside list may grow up to N here; the Border list and both recycling lists are fixed */
+    d.capI = (uint32_t)d.N;
+    d.capB = 2 * ((d.nx * d.ny) + (d.nx * d.nz) + (d.ny * d.nz));
+    d.recCapI = (uint32_t)(prm->rel_cell_caching * (double)d.N * prm->maxfill * kFullRecyclingAtt);
+    d.recCapB = (uint32_t)((double)d.capB * prm->maxfill * kFullRecyclingAtt);
+
+    h->defq.resize(4 * (size_t)d.ndefg);
+    h->rxq.resize(4 * (size_t)d.nrxg + 4);
+    for (uint32_t i = 0; i < d.ndefg; i++)
+        memcpy(&h->defq[4 * (size_t)i], &dom->ori_quat[4 * (size_t)dom->defg_ori[i]], 4 * sizeof(double));
+    for (uint32_t i = 0; i < d.nrxg; i++)
+        memcpy(&h->rxq[4 * (size_t)i], &dom->ori_quat[4 * (size_t)dom->rxg_ori[i]], 4 * sizeof(double));
+
+    /* initial grain counts after solve_REPLACE_CA_STRUCTURE (SCORE_Kernel.cpp:6390-6420) */
+    std::vector<uint32_t> defcnt(d.ndefg, 0u);
+    for (uint64_t i = 0; i < d.N; i++) defcnt[dom->cellgrid[i]]++;
+
+    /* nucleus weights with glibc (calc_mobilityweight, SCORE_GrowthMachine.cpp:152) and the
+     * placed set: a site already taken by an earlier nucleus is rejected (:130-138) */
+    std::vector<double> nucP(d.nrxg + 1);
+    {
+        std::vector<uint64_t> seen((d.N + 63) / 64, 0ull);
+        for (uint32_t k = 0; k < d.nrxg; k++) {
+            const uint32_t w = dom->rxg_site[k];
+            const uint32_t g = dom->cellgrid[w];
+            nucP[k] = mobility_weight(h->p, &h->rxq[4 * (size_t)k], &h->defq[4 * (size_t)g]);
+            if (!(seen[w >> 6] >> (w & 63) & 1ull)) {
+                seen[w >> 6] |= 1ull << (w & 63);
+                h->nuclei++;
+                if (nucP[k] >= h->regPmax) h->regPmax = nucP[k];
+            }
+        }
+    }
+
+    h->pp.model = prm->mobility_model;
+    h->pp.omega_hi = rh_omega_threshold(h->p);
+    h->pp.fac = 1.0 / sqrt(2.0);
+
+    const size_t capTot = (size_t)d.capI + d.capB;
+    d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
+    d.tileCap = (uint32_t)std::max<uint64_t>(1024, (capTot / 64) / 16);
+    d.parentCap = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(65536, d.nrxg), capTot / 16);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete h;
+        return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
+    }
+    if ((rc = dalloc(&d.grid, d.N)) || (rc = dalloc(&d.claim, d.N)) || (rc = dalloc(&d.act, capTot)) ||
+        (rc = dalloc(&d.inf, capTot)) || (rc = dalloc(&d.cell, capTot)) || (rc = dalloc(&d.defg, capTot)) ||
+        (rc = dalloc(&d.rxg, capTot)) || (rc = dalloc(&d.frac, capTot)) || (rc = dalloc(&d.P, capTot)) ||
+        (rc = dalloc(&d.mfull, d.tileCap)) || (rc = dalloc(&d.mrec, d.tileCap)) ||
+        (rc = dalloc(&d.tcntF, d.tileCap)) || (rc = dalloc(&d.tcntR, d.tileCap)) ||
+        (rc = dalloc(&d.full, d.parentCap)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
+        (rc = dalloc(&d.pwinI, d.parentCap)) || (rc = dalloc(&d.pwinB, d.parentCap)) ||
+        (rc = dalloc(&d.pcntI, d.parentCap)) || (rc = dalloc(&d.pcntB, d.parentCap)) ||
+        (rc = dalloc(&d.rho, d.ndefg)) || (rc = dalloc(&d.defq, 4 * (size_t)d.ndefg)) ||
+        (rc = dalloc(&d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(&d.defcnt, d.ndefg)) ||
+        (rc = dalloc(&d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.hkeys, d.hcap)) ||
+        (rc = dalloc(&d.hvals, d.hcap)) || (rc = dalloc(&d.pend_slot, d.hcap)) || (rc = dalloc(&d.pend_key, d.hcap)) ||
+        (rc = dalloc(&d.scan_bsum, 2 * GM_SCAN_BLOCKS)) || (rc = dalloc(&d.st, 1))) {
+        free_device(h);
+        delete h;
+        return rc;
+    }
+    if (hipHostMalloc((void **)&h->hst, sizeof(GmStats), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        free_device(h);
+        delete h;
+        return fail(SCORE_GM_ERR_OOM, "pinned stats");
+    }
+    uint32_t *d_site = nullptr;
+    double *d_nucP = nullptr;
+    auto setup = [&]() -> int {
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
+        HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
+        HIPCHK(hipMemsetAsync(d.act, 0, capTot, s));
+        HIPCHK(hipMemsetAsync(d.inf, 26, capTot, s));
+        HIPCHK(hipMemsetAsync(d.frac, 0, sizeof(double) * capTot, s));
+        HIPCHK(hipMemcpyAsync(d.rho, dom->defg_rho, sizeof(double) * d.ndefg, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.defq, h->defq.data(), sizeof(double) * 4 * d.ndefg, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.rxq, h->rxq.data(), sizeof(double) * h->rxq.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.defcnt, defcnt.data(), sizeof(uint32_t) * d.ndefg, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(d.rxcnt, 0, sizeof(uint32_t) * (d.nrxg + 1), s));
+        HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, d.hcap, s));
+        HIPCHK(gm_launch_fill_f64_bits(d.hvals, GM_PENDING_TAG, d.hcap, s));
+        HIPCHK(hipMemsetAsync(d.st, 0, sizeof(GmStats), s));
+        int r2;
+        if ((r2 = dalloc(&d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(&d_nucP, (size_t)d.nrxg + 1))) return r2;
+        if (d.nrxg) {
+            HIPCHK(hipMemcpyAsync(d_site, dom->rxg_site, sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d_nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(gm_launch_nucleate(d, d_site, d_nucP, d.nrxg, s));
+        HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    };
+    rc = setup();
+    dfree(d_site);
+    dfree(d_nucP);
+    if (rc == 0 && h->hst->err) rc = fail(SCORE_GM_ERR_CAPACITY, "Border front list full while placing nuclei");
+    if (rc) {
+        free_device(h);
+        delete h;
+        return rc;
+    }
+    h->nI = h->hst->nI;
+    h->nB = h->hst->nB;
+
+    /* solve_RXGROWTH prologue (SCORE_Kernel.cpp:6501-6530): scalars and the first dt */
+    {
+        bool found = false;
+        double best = 0.0;
+        for (uint32_t g = 0; g < d.ndefg; g++)
+            if (defcnt[g] > 0 && (!found || dom->defg_rho[g] > best)) { best = dom->defg_rho[g]; found = true; }
+        h->rhomax = kRhoWellAnnealed;
+        if (found && best >= h->rhomax) h->rhomax = best;
+    }
+    h->t = 0.0; h->X = 0.0; h->Xcells = 0.0; h->step = 0;
+    update_temperature(h);
+    update_atomistic(h);
+    update_mobilities(h);
+    h->Pmax = kDefaultPmax;
+    h->dt = dt_minimum(h);
+    /* myMobilityWeightMax after nucleation (SCORE_Kernel.cpp:6598-6605) */
+    if (h->regPmax >= h->Pmax) h->Pmax = h->regPmax;
+    *out = h;
+    return SCORE_GM_OK;
+}
+
+int score_gm_step(score_gm_handle *h, uint32_t max_steps, int32_t *completed) {
+    if (!h) return fail(SCORE_GM_ERR_ARG, "null handle");
+    if (hipSetDevice(h->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = 0;
+    for (uint32_t k = 0; k < max_steps; k++) {
+        rc = step_once(h);
+        if (rc != 0) break;
+    }
+    h->loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (completed) *completed = h->completed;
+    return rc < 0 ? rc : SCORE_GM_OK;
+}
+
+int score_gm_get_result(const score_gm_handle *h, score_gm_result *res) {
+    if (!h || !res) return fail(SCORE_GM_ERR_ARG, "null argument");
+    memset(res, 0, sizeof(*res));
+    res->t = h->t;
+    res->X = h->X;
+    res->xcells = h->Xcells;
+    res->step = h->step;
+    res->completed = h->completed;
+    res->sum_sv = h->sum_sv;
+    res->nuclei_placed = h->nuclei;
+    res->pmax = h->Pmax;
+    res->loop_seconds = h->loop_seconds;
+    return h->err;
+}
+
+int score_gm_run(score_gm_handle *h, score_gm_result *res) {
+    if (!h) return fail(SCORE_GM_ERR_ARG, "null handle");
+    int32_t completed = 0;
+    int rc = 0;
+    while (!completed) {
+        rc = score_gm_step(h, 1u << 20, &completed);
+        if (rc) break;
+    }
+    if (res) score_gm_get_result(h, res);
+    return rc;
+}
+
+int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding) {
+    if (!h || !out) return fail(SCORE_GM_ERR_ARG, "null argument");
+    if (encoding != SCORE_GM_MAP_CELLGRID && encoding != SCORE_GM_MAP_RAW2GID)
+        return fail(SCORE_GM_ERR_ARG, "unknown encoding");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(out, h->d.grid, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (encoding == SCORE_GM_MAP_RAW2GID)
+        for (uint64_t i = 0; i < h->d.N; i++) out[i] = out[i] == GM_INFECTED ? 1u : 2u + out[i];
+    return 0;
+}
+
+int score_gm_get_counts(score_gm_handle *h, uint32_t *defg_counts, uint32_t *rxg_counts) {
+    if (!h) return fail(SCORE_GM_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    if (defg_counts)
+        HIPCHK(hipMemcpyAsync(defg_counts, h->d.defcnt, sizeof(uint32_t) * h->d.ndefg, hipMemcpyDeviceToHost, h->stream));
+    if (rxg_counts && h->d.nrxg)
+        HIPCHK(hipMemcpyAsync(rxg_counts, h->d.rxcnt, sizeof(uint32_t) * h->d.nrxg, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int score_gm_get_trajectory(const score_gm_handle *h, score_gm_traj_row *rows, size_t cap, size_t *n) {
+    if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
+    *n = h->traj.size();
+    size_t m = std::min(cap, h->traj.size());
+    if (rows && m) memcpy(rows, h->traj.data(), m * sizeof(*rows));
+    return 0;
+}
+
+int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t *n, uint8_t *activity,
+                       uint8_t *infector, uint32_t *cell, double *rxfrac, double *P, uint32_t *defg, uint32_t *rxg) {
+    if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
+    if (list != SCORE_GM_LIST_INSIDE && list != SCORE_GM_LIST_BORDER) return fail(SCORE_GM_ERR_ARG, "unknown list");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(gm_launch_resolve_front(h->d, h->nI, h->nB, h->stream));
+    const uint32_t cnt = list == SCORE_GM_LIST_INSIDE ? h->nI : h->nB;
+    const size_t base = list == SCORE_GM_LIST_INSIDE ? 0 : h->d.capI;
+    *n = cnt;
+    const uint32_t m = std::min(cap, cnt);
+    hipStream_t s = h->stream;
+    if (m) {
+        if (activity) HIPCHK(hipMemcpyAsync(activity, h->d.act + base, m, hipMemcpyDeviceToHost, s));
+        if (infector) HIPCHK(hipMemcpyAsync(infector, h->d.inf + base, m, hipMemcpyDeviceToHost, s));
+        if (cell) HIPCHK(hipMemcpyAsync(cell, h->d.cell + base, 4ull * m, hipMemcpyDeviceToHost, s));
+        if (rxfrac) HIPCHK(hipMemcpyAsync(rxfrac, h->d.frac + base, 8ull * m, hipMemcpyDeviceToHost, s));
+        if (P) HIPCHK(hipMemcpyAsync(P, h->d.P + base, 8ull * m, hipMemcpyDeviceToHost, s));
+        if (defg) HIPCHK(hipMemcpyAsync(defg, h->d.defg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+        if (rxg) HIPCHK(hipMemcpyAsync(rxg, h->d.rxg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int score_gm_set_profiling(score_gm_handle *h, int32_t enable) {
+    if (!h) return fail(SCORE_GM_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    if (enable) {
+        for (hipEvent_t &ev : h->ev)
+            if (!ev) HIPCHK(hipEventCreate(&ev));
+        memset(&h->profc, 0, sizeof(h->profc));
+    }
+    h->prof = enable != 0;
+    return 0;
+}
+
+int score_gm_get_profile(const score_gm_handle *h, score_gm_profile *out) {
+    if (!h || !out) return fail(SCORE_GM_ERR_ARG, "null argument");
+    *out = h->profc;
+    return 0;
+}
+
+void score_gm_destroy(score_gm_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    free_device(h);
+    delete h;
+}
+
+}  // extern "C"

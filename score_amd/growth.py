@@ -1,0 +1,145 @@
+"""Host-side mirror of SCORE's growth-machine call on one domain.
+
+`GrowthMachine` is the Python face of caHdl::solve_RXGROWTH
+(SCORE_Kernel.cpp:6497-6792): construct it from the domain state after
+solve_REPLACE_CA_STRUCTURE, call `solve_rxgrowth()` (or `step()`), then read
+the grain map, grain counts and trajectory.  Every call goes through the C ABI
+of libscore_gm.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .synth import DomainArrays, ParamsSpec
+
+TRAJ_DTYPE = np.dtype([
+    ("step", np.uint32), ("sv", np.uint32), ("nfull_inside", np.uint32), ("nfull_border", np.uint32),
+    ("nslots_inside", np.uint32), ("nslots_border", np.uint32), ("ninfected", np.uint32), ("npending", np.uint32),
+    ("t", np.float64), ("dt", np.float64), ("X", np.float64), ("xcells", np.float64),
+    ("temperature", np.float64), ("rhomax", np.float64),
+], align=True)
+assert TRAJ_DTYPE.itemsize == C.sizeof(abi.TrajRow)
+
+
+def result_dict(r: abi.Result) -> dict:
+    return {k: getattr(r, k) for k, _ in abi.Result._fields_ if not k.startswith("reserved")}
+
+
+class GrowthMachine:
+    """One domain on one GPU; see include/score_gm.h for the contract."""
+
+    def __init__(self, dom: DomainArrays, prm: ParamsSpec | None = None, device: int = 0):
+        self._lib = abi.load()
+        prm = prm or ParamsSpec()
+        self.dom = dom
+        self.prm = prm
+        cd, self._dkeep = dom.to_c()
+        cp, self._pkeep = prm.to_c()
+        h = C.c_void_p()
+        abi.check(self._lib, self._lib.score_gm_create(C.byref(cd), C.byref(cp), device, C.byref(h)))
+        self._h = h
+
+    def _handle(self):
+        if self._h is None:
+            raise abi.ScoreGmError(abi.ERR_STATE, "handle closed")
+        return self._h
+
+    def step(self, n: int = 1) -> bool:
+        """Advance up to n growth steps; True once the termination test fired."""
+        done = C.c_int32(0)
+        abi.check(self._lib, self._lib.score_gm_step(self._handle(), n, C.byref(done)))
+        return bool(done.value)
+
+    def solve_rxgrowth(self) -> dict:
+        """Run to completion (SCORE's solve_RXGROWTH)."""
+        r = abi.Result()
+        abi.check(self._lib, self._lib.score_gm_run(self._handle(), C.byref(r)))
+        return result_dict(r)
+
+    def result(self) -> dict:
+        r = abi.Result()
+        abi.check(self._lib, self._lib.score_gm_get_result(self._handle(), C.byref(r)))
+        return result_dict(r)
+
+    def grainmap(self, encoding: int = abi.MAP_CELLGRID) -> np.ndarray:
+        out = np.empty(self.dom.N, np.uint32)
+        abi.check(self._lib, self._lib.score_gm_get_grainmap(
+            self._handle(), out.ctypes.data_as(C.POINTER(C.c_uint32)), encoding))
+        return out
+
+    def counts(self) -> tuple[np.ndarray, np.ndarray]:
+        dg = np.empty(self.dom.n_defg, np.uint32)
+        rg = np.empty(max(1, self.dom.n_rxg), np.uint32)
+        abi.check(self._lib, self._lib.score_gm_get_counts(
+            self._handle(), dg.ctypes.data_as(C.POINTER(C.c_uint32)), rg.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return dg, rg[: self.dom.n_rxg]
+
+    def trajectory(self) -> np.ndarray:
+        n = C.c_size_t(0)
+        abi.check(self._lib, self._lib.score_gm_get_trajectory(self._handle(), None, 0, C.byref(n)))
+        out = np.zeros(n.value, TRAJ_DTYPE)
+        if n.value:
+            abi.check(self._lib, self._lib.score_gm_get_trajectory(
+                self._handle(), out.ctypes.data_as(C.POINTER(abi.TrajRow)), n.value, C.byref(n)))
+        return out
+
+    def front(self, which: int) -> dict:
+        n = C.c_uint32(0)
+        nul = lambda t: C.cast(None, C.POINTER(t))  # noqa: E731
+        abi.check(self._lib, self._lib.score_gm_get_front(self._handle(), which, 0, C.byref(n), nul(C.c_uint8),
+                                                          nul(C.c_uint8), nul(C.c_uint32), nul(C.c_double),
+                                                          nul(C.c_double), nul(C.c_uint32), nul(C.c_uint32)))
+        m = n.value
+        out = dict(activity=np.zeros(m, np.uint8), infector=np.zeros(m, np.uint8), cell=np.zeros(m, np.uint32),
+                   rxfrac=np.zeros(m, np.float64), P=np.zeros(m, np.float64), defg=np.zeros(m, np.uint32),
+                   rxg=np.zeros(m, np.uint32))
+        if m:
+            p = lambda k, t: out[k].ctypes.data_as(C.POINTER(t))  # noqa: E731
+            abi.check(self._lib, self._lib.score_gm_get_front(
+                self._handle(), which, m, C.byref(n), p("activity", C.c_uint8), p("infector", C.c_uint8),
+                p("cell", C.c_uint32), p("rxfrac", C.c_double), p("P", C.c_double), p("defg", C.c_uint32),
+                p("rxg", C.c_uint32)))
+        return out
+
+    def set_profiling(self, on: bool = True) -> None:
+        abi.check(self._lib, self._lib.score_gm_set_profiling(self._handle(), 1 if on else 0))
+
+    def profile(self) -> dict:
+        p = abi.Profile()
+        abi.check(self._lib, self._lib.score_gm_get_profile(self._handle(), C.byref(p)))
+        return {k: getattr(p, k) for k, _ in abi.Profile._fields_}
+
+    def close(self) -> None:
+        if self._h is not None:
+            self._lib.score_gm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return abi.load().score_gm_device_count()
+
+
+def same_trajectory(a: np.ndarray, b: np.ndarray) -> bool:
+    """Bitwise equality of two trajectories, ignoring the host-resolution counter
+    (the serial oracle resolves weights inline and always reports 0 there)."""
+    if len(a) != len(b):
+        return False
+    a, b = a.copy(), b.copy()
+    a["npending"] = 0
+    b["npending"] = 0
+    return a.tobytes() == b.tobytes()
