@@ -323,22 +323,14 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepParams sp, u
             }
         }
     }
+    /* Sv needs no counter of its own: it is the scanned slots minus the INACTIVE ones */
     uint64_t bf = __ballot(full);
     uint64_t br = __ballot(valid && !a);
-    uint64_t ba = __ballot(a);
     if (inTile && lane == 0) {
         d.mfull[t] = bf;
         d.mrec[t] = br;
         d.tcntF[t] = (uint32_t)__popcll(bf);
         d.tcntR[t] = (uint32_t)__popcll(br);
-    }
-    __shared__ uint32_t ssv[GM_BLOCK / 64];
-    if (lane == 0) ssv[threadIdx.x >> 6] = (uint32_t)__popcll(ba);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int i = 0; i < GM_BLOCK / 64; i++) tot += ssv[i];
-        if (tot) atomicAdd(&d.st->sv, tot);
     }
 }
 
@@ -358,6 +350,7 @@ __global__ void k_compact_stats(GmDev d, uint32_t tI, uint32_t T) {
     uint32_t rI = tI < T ? d.tcntR[tI] : totR;
     st->nfull = totF;
     st->nfullI = fI;
+    st->sv = (st->nI + st->nB) - totR;
     st->totRecI = rI;
     st->totRecB = totR - rI;
     st->recLenI = rI < d.recCapI ? rI : d.recCapI;
