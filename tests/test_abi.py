@@ -1,0 +1,112 @@
+"""The C ABI library: it loads, exports every function include/score_gm.h declares,
+its ctypes mirror matches the C struct layouts, and it refuses to run without a
+gfx950 device instead of falling back to anything."""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from score_amd import abi, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "score_gm.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(score_gm_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(abi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    if shutil.which("nm"):
+        out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True, text=True).stdout
+        exported = set(re.findall(r"\bT (score_gm_\w+)", out))
+        assert set(declared_functions()) <= exported
+
+
+def test_library_has_gfx950_code_object():
+    data = open(abi.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_pure_host_entry_points():
+    lib = abi.load()
+    assert lib.score_gm_abi_version() == abi.ABI_VERSION
+    assert lib.score_gm_strerror(abi.ERR_CAPACITY)
+    assert lib.score_gm_strerror(0) == b"ok"
+
+
+LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "score_gm.h"
+#define F(S, f) printf("%s.%s %zu\n", #S, #f, offsetof(S, f));
+#define Z(S) printf("%s.sizeof %zu\n", #S, sizeof(S));
+int main(void) {
+@BODY@
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_c():
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    structs = {"score_gm_domain": abi.Domain, "score_gm_params": abi.Params, "score_gm_traj_row": abi.TrajRow,
+               "score_gm_result": abi.Result, "score_gm_profile": abi.Profile}
+    body = []
+    for cname, py in structs.items():
+        for f, _ in py._fields_:
+            body.append(f"F({cname}, {f})")
+        body.append(f"Z({cname})")
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(src, "w").write(LAYOUT_C.replace("@BODY@", "\n".join(body)))
+        subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+    for cname, py in structs.items():
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+        assert int(got[f"{cname}.sizeof"]) == C.sizeof(py), cname
+
+
+def _has_gpu():
+    return os.path.exists("/dev/kfd")
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a GPU")
+def test_create_fails_loudly_without_device():
+    from score_amd.growth import GrowthMachine
+    dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
+    with pytest.raises(abi.ScoreGmError) as e:
+        GrowthMachine(dom)
+    assert e.value.code in (abi.ERR_NODEVICE, abi.ERR_HIP)
+
+
+def test_invalid_arguments_rejected_before_device_use():
+    from score_amd.growth import GrowthMachine
+    dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
+    dom.cellgrid = dom.cellgrid.copy()
+    dom.cellgrid[5] = 7            # not a deformed-grain id
+    with pytest.raises(abi.ScoreGmError) as e:
+        GrowthMachine(dom)
+    assert e.value.code == abi.ERR_ARG
+    dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
+    with pytest.raises(abi.ScoreGmError) as e:
+        GrowthMachine(dom, synth.ParamsSpec(defrag_at=(0.5,)))
+    assert e.value.code == abi.ERR_UNSUPPORTED

@@ -1,0 +1,297 @@
+"""Pins of the oracle's growth loop (SCORE_Kernel.cpp:3587-4774, :5311-5723, :6497-6792,
+SCORE_GrowthMachine.cpp:50-189): hand-derived first steps (direction order, infector
+codes, overshoot carries, slot policy, list choice, boundary rules), the time-step
+schedule, invariants of complete runs, and the statistics of the reference's own
+shipped 200^3 run (example/jmak)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, OracleError
+from score_amd import abi, synth
+from score_amd.synth import DomainArrays, ParamsSpec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KB, EC = 1.3806488e-23, 1.602176565e-19
+
+# call order of the 26 attempts and their infector codes, as listed in
+# SCORE_Kernel.cpp:4611-4638 (offset dx,dy,dz -> code)
+ORDER = [((-1, -1, 0), 16), ((0, -1, 0), 15), ((1, -1, 0), 14), ((-1, 0, 0), 13), ((1, 0, 0), 12),
+         ((-1, 1, 0), 11), ((0, 1, 0), 10), ((1, 1, 0), 9),
+         ((-1, 1, -1), 19), ((0, 1, -1), 18), ((1, 1, -1), 17), ((-1, -1, -1), 25), ((0, -1, -1), 24),
+         ((1, -1, -1), 23), ((-1, 0, -1), 22), ((0, 0, -1), 21), ((1, 0, -1), 20),
+         ((-1, -1, 1), 8), ((0, -1, 1), 7), ((1, -1, 1), 6), ((-1, 0, 1), 5), ((0, 0, 1), 4), ((1, 0, 1), 3),
+         ((-1, 1, 1), 2), ((0, 1, 1), 1), ((1, 1, 1), 0)]
+CARRY = {1: 0.0682, 2: 0.0542, 3: 0.0484}
+
+
+def qaxis(axis, deg):
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    h = math.radians(deg) / 2
+    return np.array([math.cos(h), *(math.sin(h) * a)])
+
+
+def one_grain(n, sites, rho=1e15, nuc_deg=45.0, shape=None):
+    nx, ny, nz = shape or (n, n, n)
+    N = nx * ny * nz
+    q = [qaxis([0, 0, 1], 0.0), qaxis([0, 0, 1], nuc_deg)]
+    return DomainArrays(nx, ny, nz, 0.5e-6, np.zeros(N, np.uint32), np.array(q), np.array([0], np.uint32),
+                        np.array([rho]), np.ones(len(sites), np.uint32), np.array(sites, np.uint32))
+
+
+def lin(x, y, z, nx, ny):
+    return x + y * nx + z * nx * ny
+
+
+def test_list_capacities_follow_reference():
+    dom = one_grain(20, [lin(10, 10, 10, 20, 20)])
+    caps = Oracle(dom).capacities()
+    N = 8000
+    assert caps[0] == int(0.2 * N)                         # Inside slots
+    assert caps[1] == int(0.2 * N * 0.1 * 3.0)             # Inside recycling cap
+    assert caps[2] == 2 * (400 + 400 + 400)                # Border slots
+    assert caps[3] == int(2 * 1200 * 0.1 * 3.0)            # Border recycling cap
+
+
+def test_first_infection_order_codes_carries_and_slots():
+    n = 20
+    c = (10, 10, 10)
+    dom = one_grain(n, [lin(*c, n, n)])
+    o = Oracle(dom)
+    fi = o.front(abi.LIST_INSIDE)
+    assert len(fi["activity"]) == 1 and fi["rxfrac"][0] == 1.0 and fi["infector"][0] == 26
+    assert o.dt() == 1e-6                                  # temperature rule at t = 0
+    o.step(1)
+    fi = o.front(abi.LIST_INSIDE)
+    fb = o.front(abi.LIST_BORDER)
+    assert len(fb["activity"]) == 0
+    # the nucleus retired, its 26 neighbours appended in call order into slots 1..26
+    assert fi["activity"][0] == 0 and len(fi["activity"]) == 27
+    g = o.grainmap()
+    assert g[lin(*c, n, n)] == 1 + 0                       # ndefg + rxg id
+    tr = o.trajectory()[0]
+    frac_nucleus = 1.0 + tr["sv"] * 0 + 0.0
+    overshoot = None
+    for k, ((dx, dy, dz), code) in enumerate(ORDER, start=1):
+        assert fi["cell"][k] == lin(c[0] + dx, c[1] + dy, c[2] + dz, n, n)
+        assert fi["infector"][k] == code
+        cls = abs(dx) + abs(dy) + abs(dz)
+        if overshoot is None:
+            overshoot = fi["rxfrac"][k] / CARRY[cls]
+        assert fi["rxfrac"][k] == pytest.approx(overshoot * CARRY[cls], rel=1e-12)
+        assert fi["P"][k] == 1.0 and fi["rxg"][k] == 0 and fi["defg"][k] == 0
+        assert g[fi["cell"][k]] == abi.CURRENTLY_INFECTED
+    # overshoot of the nucleus in step 0: v * 1e-6 s / dx with v = m * G b^2/2 (rho - 1e11)
+    T = 1000.0
+    G = 6.92e10 * (-1.38e-9 * T ** 3 + 1.21e-6 * T ** 2 - 4.58e-4 * T + 1.0)
+    v = 0.1 * math.exp(-2.8 * EC / (KB * T)) * 0.5 * G * 2.48e-10 ** 2 * (1e15 - 1e11)
+    assert overshoot == pytest.approx(v * 1e-6 / 0.5e-6, rel=1e-9)
+    assert tr["sv"] == 1 and tr["nfull_inside"] == 1 and tr["ninfected"] == 26
+
+
+def test_second_dt_and_increment_closed_form():
+    """after t = 0 the step is maxfill * dx / vmax, so a face cell of the fastest grain
+    advances by maxfill * (rho - 1e11) / rho per step (SCORE_Kernel.cpp:5523-5531)"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    o = Oracle(dom)
+    o.step(1)
+    before = o.front(abi.LIST_INSIDE)["rxfrac"].copy()
+    T = 1000.0
+    G = 6.92e10 * (-1.38e-9 * T ** 3 + 1.21e-6 * T ** 2 - 4.58e-4 * T + 1.0)
+    vmax = 0.1 * math.exp(-2.8 * EC / (KB * T)) * 0.5 * G * 2.48e-10 ** 2 * 1e15
+    assert o.dt() == pytest.approx(0.1 * 0.5e-6 / vmax, rel=1e-12)
+    o.step(1)
+    after = o.front(abi.LIST_INSIDE)["rxfrac"]
+    inc = after[1:27] - before[1:27]
+    shape = {1: 1.0, 2: 0.7957, 3: 0.6494}
+    for k, ((dx, dy, dz), code) in enumerate(ORDER):
+        cls = abs(dx) + abs(dy) + abs(dz)
+        assert inc[k] == pytest.approx(0.1 * shape[cls] * (1e15 - 1e11) / 1e15, rel=1e-9)
+
+
+def test_recycled_slot_reused_first():
+    """slot 0 (retired nucleus) is the first recycled slot: the next infection takes it"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    o = Oracle(dom)
+    while True:
+        o.step(1)
+        tr = o.trajectory()
+        if tr["ninfected"][-1] > 0 and len(tr) > 1:
+            break
+    fi = o.front(abi.LIST_INSIDE)
+    # the six face neighbours complete together; in slot order the first is (10,9,10)
+    # (slot 2, offset (0,-1,0)); its first free target (-1,-1,0) -> (9,8,10) takes
+    # the recycled slot 0, the following ones append after slot 26 in call order
+    assert tr["nfull_inside"][-1] == 6
+    assert fi["activity"][0] == 1 and fi["cell"][0] == lin(9, 8, 10, n, n) and fi["infector"][0] == 16
+    assert [int(c) for c in fi["cell"][27:30]] == [lin(10, 8, 10, n, n), lin(11, 8, 10, n, n), lin(9, 8, 9, n, n)]
+    assert list(fi["infector"][27:30]) == [15, 14, 25]
+
+
+def test_border_nucleus_wraps_xy_and_drops_z():
+    n = 12
+    corner = lin(0, 0, 0, n, n)
+    dom = one_grain(n, [corner])
+    o = Oracle(dom)
+    assert len(o.front(abi.LIST_BORDER)["activity"]) == 1
+    o.step(1)
+    g = o.grainmap()
+    inf = np.nonzero(g == abi.CURRENTLY_INFECTED)[0]
+    # z = -1 targets are dropped: 17 of 26 remain; x and y wrap periodically
+    assert len(inf) == 17
+    xs, ys, zs = inf % n, (inf // n) % n, inf // (n * n)
+    assert set(xs) == {0, 1, n - 1} and set(ys) == {0, 1, n - 1} and set(zs) == {0, 1}
+    fi, fb = o.front(abi.LIST_INSIDE), o.front(abi.LIST_BORDER)
+    # the only strictly interior target (1,1,1) goes to the Inside list
+    assert list(fi["cell"][fi["activity"] == 1]) == [lin(1, 1, 1, n, n)]
+    assert (fb["activity"] == 1).sum() == 16
+
+
+def test_inside_parent_near_face_feeds_border_list():
+    n = 12
+    dom = one_grain(n, [lin(1, 6, 6, n, n)])   # interior, but within two cells of the x face
+    o = Oracle(dom)
+    assert len(o.front(abi.LIST_INSIDE)["activity"]) == 1
+    o.step(1)
+    fi, fb = o.front(abi.LIST_INSIDE), o.front(abi.LIST_BORDER)
+    assert (fi["activity"] == 1).sum() == 0 and (fb["activity"] == 1).sum() == 26
+    # in call order, appended to the Border list
+    assert list(fb["infector"][:26]) == [c for _, c in ORDER]
+
+
+def test_duplicate_site_rejected_in_pool_order():
+    n = 12
+    s = lin(6, 6, 6, n, n)
+    dom = one_grain(n, [s, s, lin(3, 3, 3, n, n)])
+    o = Oracle(dom)
+    assert o.result()["nuclei_placed"] == 2
+    fi = o.front(abi.LIST_INSIDE)
+    assert list(fi["rxg"]) == [0, 2]
+
+
+def _check_complete_run(dom, prm):
+    o = Oracle(dom, prm)
+    r = o.solve_rxgrowth()
+    tr = o.trajectory()
+    g = o.grainmap()
+    dc, rc = o.counts()
+    nd = dom.n_defg
+    assert r["completed"] == 1
+    # cell conservation: counts + infected = N; the grid agrees with the counts
+    ninf = int((g == abi.CURRENTLY_INFECTED).sum())
+    assert int(dc.sum()) + int(rc.sum()) == dom.N
+    assert np.array_equal(np.bincount(g[g < nd], minlength=nd) + 0, dc - np.bincount(
+        dom.cellgrid[g == abi.CURRENTLY_INFECTED], minlength=nd))
+    rx = g[(g >= nd) & (g != abi.CURRENTLY_INFECTED)] - nd
+    assert np.array_equal(np.bincount(rx, minlength=dom.n_rxg), rc)
+    # X is an exact cell count, monotone
+    assert np.all(np.diff(tr["xcells"]) >= 0) and tr["xcells"][-1] == r["xcells"] == rc.sum()
+    assert r["X"] == r["xcells"] / dom.N
+    # the time axis adds the recorded steps
+    assert np.all(tr["t"][1:] == tr["t"][:-1] + tr["dt"][:-1])
+    return o, g, ninf
+
+
+def _components_26(mask, nx, ny, nz):
+    """26-connected components with periodic x and y (numpy flood fill)"""
+    lab = -np.ones(mask.shape, np.int64)
+    idx = np.argwhere(mask)
+    cur = 0
+    offs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1) if (a, b, c) != (0, 0, 0)]
+    for z, y, x in idx:
+        if lab[z, y, x] >= 0:
+            continue
+        stack = [(z, y, x)]
+        lab[z, y, x] = cur
+        while stack:
+            cz, cy, cx = stack.pop()
+            for dz, dy, dx in offs:
+                zz, yy, xx = cz + dz, (cy + dy) % ny, (cx + dx) % nx
+                if 0 <= zz < nz and mask[zz, yy, xx] and lab[zz, yy, xx] < 0:
+                    lab[zz, yy, xx] = cur
+                    stack.append((zz, yy, xx))
+        cur += 1
+    return cur
+
+
+def test_complete_run_invariants_and_connectivity():
+    dom = synth.make_voronoi_domain(24, 24, 24, n_grains=5, n_nuclei=25, seed=21)
+    o, g, _ = _check_complete_run(dom, ParamsSpec(xmax=1.0))
+    assert o.result()["xcells"] == dom.N
+    gg = g.reshape(24, 24, 24)
+    for r in range(dom.n_rxg):
+        m = gg == dom.n_defg + r
+        if m.any():
+            assert _components_26(m, 24, 24, 24) == 1
+            assert m.ravel()[dom.rxg_site[r]]
+
+
+def test_voronoi100_invariants_and_determinism():
+    dom = synth.workload("voronoi100", seed=1, scale=0.6)
+    o1, g1, _ = _check_complete_run(dom, ParamsSpec())
+    o2 = Oracle(dom)
+    o2.solve_rxgrowth()
+    assert np.array_equal(g1, o2.grainmap())
+    assert o1.trajectory().tobytes() == o2.trajectory().tobytes()
+    assert o1.result()["X"] > 0.99
+
+
+def test_avrami_exponent_site_saturated():
+    """homogeneous site-saturated growth at constant rate: JMAK exponent n = 3"""
+    n = 80
+    rng = np.random.default_rng(3)
+    sites = rng.choice(n ** 3, 120, replace=False)
+    o = Oracle(one_grain(n, sites), ParamsSpec(xmax=0.999))
+    o.solve_rxgrowth()
+    tr = o.trajectory()
+    t, X = tr["t"] + tr["dt"], tr["X"]
+    sel = (X > 0.05) & (X < 0.7)
+    slope = np.polyfit(np.log(t[sel]), np.log(-np.log(1 - X[sel])), 1)[0]
+    assert 2.6 < slope < 3.4
+
+
+def test_jmak_reference_statistics():
+    """Statistics of the reference's shipped 200^3 run: 1 grain (rho 1e15), 947 nuclei,
+    Rollett-Holm m0=300, 2.45 eV, 500 C, maxfill 0.1 (example/jmak)."""
+    ref = json.load(open(os.path.join(HERE, "golden", "jmak_stats.json")))
+    s = ref["setup"]
+    n, N = s["nx"], s["nx"] ** 3
+    sites = synth.distinct_sites(synth.substream(1, 3), s["n_nuclei"], N)
+    dom = one_grain(n, sites, rho=s["rho0"])
+    dom.cellsize = s["cellsize_um"] * 1e-6
+    T = s["T_C"] + 273.15
+    prm = ParamsSpec(rh_hagb_m0=s["RHModelHAGBm0"], rh_hagb_hact_ev=s["RHModelHAGBHact_eV"], G0=s["G0"], b0=s["b0"],
+                     sched_time=(0.0, 3600.0, 36000.0), sched_temp=(T, T, T), xmax=1.0, tmax=36000.0)
+    o = Oracle(dom, prm)
+    o.solve_rxgrowth()
+    tr = o.trajectory()
+    _, rc = o.counts()
+    # total front work (sum of Sv) within 1% of the reference's 124,189,713
+    assert int(tr["sv"].sum()) == pytest.approx(sum(ref["Sv"]), rel=0.01)
+    # the kinetics X(step) agree step by step
+    Xr = np.array(ref["X"])
+    m = min(len(Xr), len(tr))
+    assert np.abs(tr["X"][:m] - Xr[:m]).max() < 0.05
+    # steps to completion within 20% (the tail depends on the largest void)
+    assert 0.8 * ref["iterations"] <= len(tr) <= 1.2 * ref["iterations"]
+    # first step: 947 nuclei fill and infect up to 26 neighbours each
+    assert tr["sv"][0] == ref["Sv"][0] and abs(int(tr["sv"][1]) - ref["Sv"][1]) < 0.02 * ref["Sv"][1]
+    # final grain-size distribution
+    assert rc.sum() == N and rc.mean() == pytest.approx(ref["gsd_summary"]["mean"], rel=1e-6)
+    assert np.median(rc) == pytest.approx(ref["gsd_summary"]["median"], rel=0.06)
+    assert rc.std() == pytest.approx(ref["gsd_summary"]["std"], rel=0.12)
+
+
+def test_border_overflow_is_an_error():
+    dom = one_grain(30, [(3 * i + 1) + 30 * (3 * j + 1) + 900 for j in range(10) for i in range(10)],
+                    shape=(30, 30, 3))
+    o = Oracle(dom, ParamsSpec(xmax=1.0))
+    with pytest.raises(OracleError) as e:
+        o.solve_rxgrowth()
+    assert e.value.code == abi.ERR_CAPACITY
