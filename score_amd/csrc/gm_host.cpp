@@ -43,14 +43,26 @@ const double kDefaultPmax = 0.0;
 const double kDefaultSmallNumber = 1e-6;
 const double kFullRecyclingAtt = 3.0;
 
-/* Algorithmic HBM bytes (what the work needs at minimum, not what the kernels move):
- * sweep, per ACTIVE slot: activity 1 + infector 1 + P 8 + defg 4 + rxFrac read 8 + write 8;
- * per other scanned slot: activity 1.
- * infection, per completed cell: FullRX entry 4, parent record (cell, rxFrac, P, defg, rxg) 28,
- * 26 cell-grid reads 104, retirement (grid 4, activity 1, rxFrac 8, two grain counts 8) 21;
- * per new infection: record 30, grid write 4, claim atomic 4 and reset 4. */
-const uint64_t kSweepBytesActive = 30, kSweepBytesIdle = 1;
-const uint64_t kInfectBytesParent = 4 + 28 + 104 + 21, kInfectBytesNew = 30 + 4 + 4 + 4;
+/* Algorithmic HBM bytes (what the work needs at minimum, not what the kernels move).
+ * sweep: every occupied slot, its completion step 4; a cell infected in the previous
+ * step, its list entry 4 + completion step 4 + anchor 8 + P 8 + defg 4 + infector 1
+ * read and increment 8 + completion step 4 + final rxFrac 8 written (49); every
+ * list entry written 4.  A re-anchoring sweep reads 4+8+8+4+8+4+1 and writes
+ * 8+4+8+4+8 per active slot (69).
+ * infection, per completed cell: FullRX entry 4, parent record (cell, rxFrac, P, defg,
+ * rxg) 28, 26 cell-grid reads 104, retirement (grid 4, completion step 4, two grain
+ * counts 8) 16; per new infection: record 41, grid write 4, claim atomic 4 and reset 4. */
+const uint64_t kSweepBytesSlot = 4, kSweepBytesNew = 49, kSweepBytesListEntry = 4;
+const uint64_t kSweepBytesReanchor = 69;
+const uint64_t kInfectBytesParent = 4 + 28 + 104 + 16, kInfectBytesNew = 41 + 4 + 4 + 4;
+
+bool same_bits(double a, double b) { return memcmp(&a, &b, sizeof(double)) == 0; }
+
+bool same_scalars(const GmSweepParams &a, const GmSweepParams &b) {
+    return a.model == b.model && same_bits(a.mRH, b.mRH) && same_bits(a.mLAGB, b.mLAGB) &&
+           same_bits(a.mHAGB, b.mHAGB) && same_bits(a.mGS, b.mGS) && same_bits(a.Gbhalfsq, b.Gbhalfsq) &&
+           same_bits(a.zener, b.zener) && same_bits(a.dt, b.dt) && same_bits(a.inv_dx, b.inv_dx);
+}
 
 thread_local std::string g_last_error;
 
@@ -211,6 +223,17 @@ struct score_gm_handle {
     std::vector<double> pend_vals;
     double *d_pend_vals = nullptr;
     uint32_t pend_vals_cap = 0;
+    /* scalars of the previous sweep: a change forces re-anchoring of rxFrac */
+    GmSweepParams last_sp{};
+    bool have_last_sp = false;
+    uint32_t nunknown_seen = 0;
+    uint32_t nnew = 0; /* cells infected in the previous step */
+    /* deformed grains by descending rho and whether they still own cells */
+    std::vector<double> rho_host;
+    std::vector<uint32_t> rho_order;
+    std::vector<uint8_t> alive;
+    size_t rho_top = 0;
+    std::vector<uint32_t> zbuf;
 };
 
 namespace {
@@ -235,9 +258,10 @@ void dfree(void *p) {
 
 void free_device(score_gm_handle *h) {
     GmDev &d = h->d;
-    void *ptrs[] = {d.grid, d.claim, d.act, d.inf, d.cell, d.defg, d.rxg, d.frac, d.P, d.mfull, d.mrec, d.tcntF,
-                    d.tcntR, d.full, d.recI, d.recB, d.pwinI, d.pwinB, d.pcntI, d.pcntB, d.rho, d.defq, d.rxq,
-                    d.defcnt, d.rxcnt, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.scan_bsum, d.st, h->d_pend_vals};
+    void *ptrs[] = {d.grid, d.claim, d.cstep, d.s0, d.frac0, d.inc, d.fin, d.inf, d.cell, d.defg, d.rxg, d.P, d.fullI,
+                    d.fullB, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
+                    d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
+                    d.scan_bsum, d.st, h->d_pend_vals};
     for (void *q : ptrs) dfree(q);
     memset(&h->d, 0, sizeof(h->d));
     h->d_pend_vals = nullptr;
@@ -353,15 +377,17 @@ double dt_minimum(const score_gm_handle *h) {
     return mn;
 }
 
-/* update_myrhomax (SCORE_RecoveryModels.cpp:146-165) from the device maximum */
-double rhomax_from_bits(unsigned long long bits) {
+/* update_myrhomax (SCORE_RecoveryModels.cpp:146-165): grains sorted by rho
+ * descending (update_mydgoptimize_rho, :128-135); the first one that still owns
+ * cells gives rhomax, floored at RHOMAX_WELLANNEALED */
+void update_rhomax(score_gm_handle *h) {
+    while (h->rho_top < h->rho_order.size() && !h->alive[h->rho_order[h->rho_top]]) h->rho_top++;
     double r = kRhoWellAnnealed;
-    if (bits != 0ull) {
-        double v;
-        memcpy(&v, &bits, sizeof(v));
+    if (h->rho_top < h->rho_order.size()) {
+        const double v = h->rho_host[h->rho_order[h->rho_top]];
         if (v >= r) r = v;
     }
-    return r;
+    h->rhomax = r;
 }
 
 /* termination test (SCORE_Kernel.cpp:6619) */
@@ -376,30 +402,48 @@ uint32_t pow2_at_least(uint64_t v) {
     return (uint32_t)std::min<uint64_t>(c, 1ull << 31);
 }
 
-/* per-parent and per-tile arrays sized for the current occupancy */
+/* per-step lists, tile masks and per-parent masks sized for the current occupancy */
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
-    const uint32_t tiles = (h->nI + GM_TILE - 1) / GM_TILE + (h->nB + GM_TILE - 1) / GM_TILE;
+    if (h->nI > d.fullCapI) {
+        uint32_t nc = std::max<uint32_t>(h->nI, d.fullCapI + d.fullCapI / 2);
+        nc = std::min<uint32_t>(nc, d.capI);
+        dfree(d.fullI);
+        d.fullI = nullptr;
+        int rc = dalloc(&d.fullI, nc);
+        if (rc) return rc;
+        d.fullCapI = nc;
+    }
+    const uint32_t tiles = (h->nI + 63) / 64 + (h->nB + 63) / 64;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
-        dfree(d.mfull); dfree(d.mrec); dfree(d.tcntF); dfree(d.tcntR);
-        d.mfull = d.mrec = nullptr; d.tcntF = d.tcntR = nullptr;
+        dfree(d.mF); dfree(d.mR);
+        d.mF = d.mR = nullptr;
         int rc;
-        if ((rc = dalloc(&d.mfull, nc)) || (rc = dalloc(&d.mrec, nc)) || (rc = dalloc(&d.tcntF, nc)) ||
-            (rc = dalloc(&d.tcntR, nc)))
-            return rc;
+        if ((rc = dalloc(&d.mF, nc)) || (rc = dalloc(&d.mR, nc))) return rc;
         d.tileCap = nc;
     }
     const uint32_t parents = h->nI + h->nB;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
-        dfree(d.full); dfree(d.pwinI); dfree(d.pwinB); dfree(d.pcntI); dfree(d.pcntB);
-        d.full = d.pwinI = d.pwinB = d.pcntI = d.pcntB = nullptr;
+        dfree(d.pmI); dfree(d.pmB);
+        d.pmI = d.pmB = nullptr;
         int rc;
-        if ((rc = dalloc(&d.full, nc)) || (rc = dalloc(&d.pwinI, nc)) || (rc = dalloc(&d.pwinB, nc)) ||
-            (rc = dalloc(&d.pcntI, nc)) || (rc = dalloc(&d.pcntB, nc)))
-            return rc;
+        if ((rc = dalloc(&d.pmI, nc)) || (rc = dalloc(&d.pmB, nc))) return rc;
         d.parentCap = nc;
+    }
+    /* a step infects distinct deformed cells, at most 26 per completed cell */
+    const uint32_t newmax = (uint32_t)std::min<uint64_t>(d.N, 26ull * parents);
+    if (newmax > d.newCap) {
+        uint32_t nc = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(newmax, d.newCap + d.newCap / 2));
+        dfree(d.newl);
+        d.newl = nullptr;
+        d.newCap = 0;
+        int rc = dalloc(&d.newl, nc);
+        if (rc) return rc;
+        d.newCap = nc;
+        /* the previous step's list is gone: the sweep predicts those cells itself */
+        h->nnew = 0;
     }
     return 0;
 }
@@ -441,7 +485,8 @@ int resolve_pending(score_gm_handle *h, uint32_t npend) {
 int grow_pcache(score_gm_handle *h) {
     GmDev &d = h->d;
     if ((uint64_t)h->hcount * 2 <= d.hcap) return 0;
-    HIPCHK(gm_launch_resolve_front(d, h->nI, h->nB, h->stream));
+    HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, nullptr, nullptr, h->stream));
+    HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, nullptr, nullptr, h->stream));
     std::vector<unsigned long long> keys(d.hcap);
     std::vector<double> vals(d.hcap);
     HIPCHK(hipMemcpyAsync(keys.data(), d.hkeys, sizeof(unsigned long long) * d.hcap, hipMemcpyDeviceToHost, h->stream));
@@ -481,38 +526,55 @@ int step_once(score_gm_handle *h) {
     GmDev &d = h->d;
     int rc = ensure_step_capacity(h);
     if (rc) return h->err = rc;
-    GmSweepParams sp;
+    GmSweepArgs a;
+    memset(&a, 0, sizeof(a));
+    GmSweepParams &sp = a.sp;
     sp.model = h->p.mobility_model;
     sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
     sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
+    a.k = h->step;
+    a.reanchor = (!h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
+    a.nI = h->nI;
+    a.nB = h->nB;
+    a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_CS_MAXSTEP - 1);
+    a.nnew = h->nnew;
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
     auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(h->ev[i], h->stream) : hipSuccess; };
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
-    if ((e = gm_launch_begin_step(d, h->stream)) != hipSuccess || (e = mark(0)) != hipSuccess ||
-        (e = gm_launch_sweep(d, sp, h->nI, h->nB, h->stream)) != hipSuccess || (e = mark(1)) != hipSuccess ||
-        (e = gm_launch_compact(d, h->nI, h->nB, h->stream)) != hipSuccess || (e = mark(2)) != hipSuccess ||
-        (e = gm_launch_infect(d, h->pp, h->stream)) != hipSuccess || (e = mark(3)) != hipSuccess ||
-        (e = gm_launch_rhomax(d, h->stream)) != hipSuccess || (e = mark(4)) != hipSuccess ||
+    if ((e = mark(0)) != hipSuccess || (e = gm_launch_sweep(d, a, h->stream)) != hipSuccess ||
+        (e = mark(1)) != hipSuccess ||
+        (e = gm_launch_infect(d, h->pp, h->step, slots0, h->stream)) != hipSuccess ||
+        (e = mark(2)) != hipSuccess ||
         (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
-    const GmStats st = *h->hst;
+    h->last_sp = sp;
+    h->have_last_sp = true;
+    GmStats st = *h->hst;
+    st.nfull = st.nfullI + st.nfullB;
+    st.sv = slots0 - (st.totRecI + st.totRecB);
+    /* increments computed: the listed new cells, and the cells the sweep met unpredicted */
+    const uint32_t nunknown = st.nunknown - h->nunknown_seen + (a.reanchor ? 0u : std::min(a.nnew, d.newCap));
+    h->nunknown_seen = st.nunknown;
+    h->nnew = st.winI + st.winB;
     if (prof) {
-        float ms[4];
-        for (int i = 0; i < 4; i++)
+        float ms[2];
+        for (int i = 0; i < 2; i++)
             if ((e = hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1])) != hipSuccess)
                 return h->err = hip_fail(e, "hipEventElapsedTime");
         score_gm_profile &pf = h->profc;
         pf.steps++;
         pf.ms_sweep += ms[0];
-        pf.ms_compact += ms[1];
-        pf.ms_infect += ms[2];
-        pf.ms_tail += ms[3];
+        pf.ms_infect += ms[1];
         pf.sweep_slots += slots0;
         pf.sweep_active += st.sv;
-        pf.sweep_bytes += kSweepBytesActive * (uint64_t)st.sv + kSweepBytesIdle * (uint64_t)(slots0 - st.sv);
+        const uint64_t lists = kSweepBytesListEntry * ((uint64_t)st.nfull + st.recLenI + st.recLenB);
+        if (a.reanchor)
+            pf.sweep_bytes += kSweepBytesReanchor * (uint64_t)st.sv + kSweepBytesSlot * (slots0 - st.sv) + lists;
+        else
+            pf.sweep_bytes += kSweepBytesSlot * (uint64_t)slots0 + kSweepBytesNew * (uint64_t)nunknown + lists;
         pf.infect_parents += st.nfull;
         pf.infect_new += (uint64_t)st.winI + st.winB;
         pf.infect_bytes += kInfectBytesParent * (uint64_t)st.nfull + kInfectBytesNew * ((uint64_t)st.winI + st.winB);
@@ -522,6 +584,7 @@ int step_once(score_gm_handle *h) {
         if (st.err & GM_ERRF_BORDER_FULL)
             return h->err = fail(SCORE_GM_ERR_CAPACITY, "Border front list full (SCORE would write out of bounds)");
         if (st.err & GM_ERRF_INSIDE_FULL) return h->err = fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
+        if (st.err & GM_ERRF_MISSED) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
         return h->err = fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
     }
     if ((rc = resolve_pending(h, st.npend))) return h->err = rc;
@@ -542,7 +605,19 @@ int step_once(score_gm_handle *h) {
     h->Xcells = h->Xcells + ((double)st.nfull);
     h->X = (h->Xcells / ((double)d.N));
     h->sum_sv += st.sv;
-    h->rhomax = rhomax_from_bits(st.rhomax_bits);
+    if (st.nzeroed) {
+        /* grains that lost their last cell leave the rhomax search */
+        const uint32_t *ids = st.zeroed_ids;
+        if (st.nzeroed > GM_ZEROED_INLINE) {
+            h->zbuf.resize(st.nzeroed);
+            if ((e = hipMemcpy(h->zbuf.data(), d.zlist, sizeof(uint32_t) * st.nzeroed, hipMemcpyDeviceToHost)) !=
+                hipSuccess)
+                return h->err = hip_fail(e, "exhausted grains");
+            ids = h->zbuf.data();
+        }
+        for (uint32_t i = 0; i < st.nzeroed; i++) h->alive[ids[i]] = 0;
+    }
+    update_rhomax(h);
     r.sv = st.sv;
     r.nfull_inside = st.nfullI;
     r.nfull_border = st.nfull - st.nfullI;
@@ -677,26 +752,30 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
 
     const size_t capTot = (size_t)d.capI + d.capB;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
-    d.tileCap = (uint32_t)std::max<uint64_t>(1024, (capTot / 64) / 16);
-    d.parentCap = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(65536, d.nrxg), capTot / 16);
+    d.fullCapI = (uint32_t)std::min<uint64_t>(d.capI, std::max<uint64_t>(65536, d.N / 16));
+    d.tileCap = (uint32_t)std::max<uint64_t>(1024, (d.N / 16) / 64);
+    d.parentCap = (uint32_t)std::max<uint64_t>(65536, d.N / 16);
+    d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         delete h;
         return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
     }
-    if ((rc = dalloc(&d.grid, d.N)) || (rc = dalloc(&d.claim, d.N)) || (rc = dalloc(&d.act, capTot)) ||
-        (rc = dalloc(&d.inf, capTot)) || (rc = dalloc(&d.cell, capTot)) || (rc = dalloc(&d.defg, capTot)) ||
-        (rc = dalloc(&d.rxg, capTot)) || (rc = dalloc(&d.frac, capTot)) || (rc = dalloc(&d.P, capTot)) ||
-        (rc = dalloc(&d.mfull, d.tileCap)) || (rc = dalloc(&d.mrec, d.tileCap)) ||
-        (rc = dalloc(&d.tcntF, d.tileCap)) || (rc = dalloc(&d.tcntR, d.tileCap)) ||
-        (rc = dalloc(&d.full, d.parentCap)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
-        (rc = dalloc(&d.pwinI, d.parentCap)) || (rc = dalloc(&d.pwinB, d.parentCap)) ||
-        (rc = dalloc(&d.pcntI, d.parentCap)) || (rc = dalloc(&d.pcntB, d.parentCap)) ||
+    if ((rc = dalloc(&d.grid, d.N)) || (rc = dalloc(&d.claim, d.N)) || (rc = dalloc(&d.cstep, capTot)) ||
+        (rc = dalloc(&d.s0, capTot)) || (rc = dalloc(&d.frac0, capTot)) || (rc = dalloc(&d.inc, capTot)) ||
+        (rc = dalloc(&d.fin, capTot)) || (rc = dalloc(&d.newl, d.newCap)) || (rc = dalloc(&d.inf, capTot)) || (rc = dalloc(&d.cell, capTot)) || (rc = dalloc(&d.defg, capTot)) ||
+        (rc = dalloc(&d.rxg, capTot)) || (rc = dalloc(&d.P, capTot)) || (rc = dalloc(&d.fullI, d.fullCapI)) ||
+        (rc = dalloc(&d.fullB, d.capB)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
+        (rc = dalloc(&d.mF, d.tileCap)) || (rc = dalloc(&d.mR, d.tileCap)) || (rc = dalloc(&d.pmI, d.parentCap)) ||
+        (rc = dalloc(&d.pmB, d.parentCap)) || (rc = dalloc(&d.bcnt, 5 * GM_MAX_BLOCKS)) ||
         (rc = dalloc(&d.rho, d.ndefg)) || (rc = dalloc(&d.defq, 4 * (size_t)d.ndefg)) ||
         (rc = dalloc(&d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(&d.defcnt, d.ndefg)) ||
-        (rc = dalloc(&d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.hkeys, d.hcap)) ||
+        (rc = dalloc(&d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.zlist, d.ndefg)) ||
+        (rc = dalloc(&d.hkeys, d.hcap)) ||
         (rc = dalloc(&d.hvals, d.hcap)) || (rc = dalloc(&d.pend_slot, d.hcap)) || (rc = dalloc(&d.pend_key, d.hcap)) ||
-        (rc = dalloc(&d.scan_bsum, 2 * GM_SCAN_BLOCKS)) || (rc = dalloc(&d.st, 1))) {
+        (rc = dalloc(&d.nflag, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.nI_, (size_t)d.nrxg + 1)) ||
+        (rc = dalloc(&d.nB_, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.scan_bsum, 2 * GM_SCAN_BLOCKS)) ||
+        (rc = dalloc(&d.st, 1))) {
         free_device(h);
         delete h;
         return rc;
@@ -713,17 +792,19 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
         HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
-        HIPCHK(hipMemsetAsync(d.act, 0, capTot, s));
-        HIPCHK(hipMemsetAsync(d.inf, 26, capTot, s));
-        HIPCHK(hipMemsetAsync(d.frac, 0, sizeof(double) * capTot, s));
+        HIPCHK(gm_launch_fill_u32(d.cstep, GM_CS_INACTIVE, capTot, s));
+        HIPCHK(gm_launch_fill_u32(d.s0, 0u, capTot, s));
+        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.frac0, 0ull, capTot, s));
+        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.inc, 0ull, capTot, s));
+        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.fin, 0ull, capTot, s));
         HIPCHK(hipMemcpyAsync(d.rho, dom->defg_rho, sizeof(double) * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.defq, h->defq.data(), sizeof(double) * 4 * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.rxq, h->rxq.data(), sizeof(double) * h->rxq.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.defcnt, defcnt.data(), sizeof(uint32_t) * d.ndefg, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemsetAsync(d.rxcnt, 0, sizeof(uint32_t) * (d.nrxg + 1), s));
+        HIPCHK(gm_launch_fill_u32(d.rxcnt, 0u, (size_t)d.nrxg + 1, s));
         HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, d.hcap, s));
         HIPCHK(gm_launch_fill_f64_bits(d.hvals, GM_PENDING_TAG, d.hcap, s));
-        HIPCHK(hipMemsetAsync(d.st, 0, sizeof(GmStats), s));
+        HIPCHK(gm_launch_fill_u32((uint32_t *)d.st, 0u, sizeof(GmStats) / 4, s));
         int r2;
         if ((r2 = dalloc(&d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(&d_nucP, (size_t)d.nrxg + 1))) return r2;
         if (d.nrxg) {
@@ -738,6 +819,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     rc = setup();
     dfree(d_site);
     dfree(d_nucP);
+    dfree(d.nflag); dfree(d.nI_); dfree(d.nB_);
+    d.nflag = d.nI_ = d.nB_ = nullptr;
     if (rc == 0 && h->hst->err) rc = fail(SCORE_GM_ERR_CAPACITY, "Border front list full while placing nuclei");
     if (rc) {
         free_device(h);
@@ -748,14 +831,17 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->nB = h->hst->nB;
 
     /* solve_RXGROWTH prologue (SCORE_Kernel.cpp:6501-6530): scalars and the first dt */
-    {
-        bool found = false;
-        double best = 0.0;
-        for (uint32_t g = 0; g < d.ndefg; g++)
-            if (defcnt[g] > 0 && (!found || dom->defg_rho[g] > best)) { best = dom->defg_rho[g]; found = true; }
-        h->rhomax = kRhoWellAnnealed;
-        if (found && best >= h->rhomax) h->rhomax = best;
+    h->rho_host.assign(dom->defg_rho, dom->defg_rho + d.ndefg);
+    h->rho_order.resize(d.ndefg);
+    h->alive.resize(d.ndefg);
+    for (uint32_t g = 0; g < d.ndefg; g++) {
+        h->rho_order[g] = g;
+        h->alive[g] = defcnt[g] > 0 ? 1 : 0;
     }
+    std::stable_sort(h->rho_order.begin(), h->rho_order.end(),
+                     [&](uint32_t a, uint32_t b) { return h->rho_host[a] > h->rho_host[b]; });
+    h->rho_top = 0;
+    update_rhomax(h);
     h->t = 0.0; h->X = 0.0; h->Xcells = 0.0; h->step = 0;
     update_temperature(h);
     update_atomistic(h);
@@ -845,22 +931,37 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
     if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
     if (list != SCORE_GM_LIST_INSIDE && list != SCORE_GM_LIST_BORDER) return fail(SCORE_GM_ERR_ARG, "unknown list");
     HIPCHK(hipSetDevice(h->device));
-    HIPCHK(gm_launch_resolve_front(h->d, h->nI, h->nB, h->stream));
     const uint32_t cnt = list == SCORE_GM_LIST_INSIDE ? h->nI : h->nB;
-    const size_t base = list == SCORE_GM_LIST_INSIDE ? 0 : h->d.capI;
+    const uint32_t base = list == SCORE_GM_LIST_INSIDE ? 0u : h->d.capI;
     *n = cnt;
     const uint32_t m = std::min(cap, cnt);
     hipStream_t s = h->stream;
     if (m) {
-        if (activity) HIPCHK(hipMemcpyAsync(activity, h->d.act + base, m, hipMemcpyDeviceToHost, s));
-        if (infector) HIPCHK(hipMemcpyAsync(infector, h->d.inf + base, m, hipMemcpyDeviceToHost, s));
-        if (cell) HIPCHK(hipMemcpyAsync(cell, h->d.cell + base, 4ull * m, hipMemcpyDeviceToHost, s));
-        if (rxfrac) HIPCHK(hipMemcpyAsync(rxfrac, h->d.frac + base, 8ull * m, hipMemcpyDeviceToHost, s));
-        if (P) HIPCHK(hipMemcpyAsync(P, h->d.P + base, 8ull * m, hipMemcpyDeviceToHost, s));
-        if (defg) HIPCHK(hipMemcpyAsync(defg, h->d.defg + base, 4ull * m, hipMemcpyDeviceToHost, s));
-        if (rxg) HIPCHK(hipMemcpyAsync(rxg, h->d.rxg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+        /* activity and rxFrac as SCORE holds them before the next step, P without pending markers */
+        uint8_t *dact = nullptr;
+        double *dfrac = nullptr;
+        int rc;
+        if ((rc = dalloc(&dact, m)) || (rc = dalloc(&dfrac, m))) {
+            dfree(dact);
+            return rc;
+        }
+        auto copy = [&]() -> int {
+            HIPCHK(gm_launch_materialize(h->d, h->step, base, m, dact, dfrac, s));
+            if (activity) HIPCHK(hipMemcpyAsync(activity, dact, m, hipMemcpyDeviceToHost, s));
+            if (rxfrac) HIPCHK(hipMemcpyAsync(rxfrac, dfrac, 8ull * m, hipMemcpyDeviceToHost, s));
+            if (infector) HIPCHK(hipMemcpyAsync(infector, h->d.inf + base, m, hipMemcpyDeviceToHost, s));
+            if (cell) HIPCHK(hipMemcpyAsync(cell, h->d.cell + base, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (P) HIPCHK(hipMemcpyAsync(P, h->d.P + base, 8ull * m, hipMemcpyDeviceToHost, s));
+            if (defg) HIPCHK(hipMemcpyAsync(defg, h->d.defg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (rxg) HIPCHK(hipMemcpyAsync(rxg, h->d.rxg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            return 0;
+        };
+        rc = copy();
+        dfree(dact);
+        dfree(dfrac);
+        return rc;
     }
-    HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 

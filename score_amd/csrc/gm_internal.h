@@ -5,9 +5,20 @@
  *   grid   uint32[N]   cell states: deformed id (< ndefg), ndefg+rxg (transformed) or INFECTED
  *   claim  uint32[N]   per-cell infection claim key, 0xFFFFFFFF when free
  *   front  SoA over capI+capB slots: Inside list slots [0,capI), Border list slots [capI,capI+capB)
- *          act u8, inf u8, cell u32, defg u32, rxg u32, frac f64, P f64
+ *          hot:  cstep u32 (completion step or a GM_CS_* sentinel)
+ *          warm: frac0 f64, s0 u32, inc f64 (rxFrac anchor: value before step s0, per-step increment),
+ *                fin f64 (rxFrac after the predicted completing addition)
+ *          cold: cell u32, P f64, defg u32, rxg u32, inf u8
  * The two lists keep SCORE's slot numbering (SCORE_Kernel.cpp:3587-3874) so that
  * slot-level state can be compared with a serial run.
+ *
+ * rxFrac is kept as an anchor (frac0 at the start of step s0) plus the increment
+ * inc = ((v*dt)*inv_dx) that SCORE adds every step (SCORE_Kernel.cpp:4092).  While
+ * the per-step scalars stay bit-identical, rxFrac after step k is frac0 plus
+ * (k-s0+1) rounded additions of inc, so the step in which it first reaches 1.0
+ * is known in advance (cstep) by replaying the additions.  When any scalar
+ * changes, every active cell is re-anchored with the old increment and
+ * re-predicted with the new one.
  */
 #ifndef SCORE_GM_INTERNAL_H
 #define SCORE_GM_INTERNAL_H
@@ -20,28 +31,39 @@
 #define GM_PENDING_TAG 0x7FF8000000000000ull  /* quiet NaN; low 32 bits carry the P-cache slot */
 #define GM_PENDING_MASK 0xFFF8000000000000ull
 
+/* cstep sentinels; real completion steps are below GM_CS_MAXSTEP */
+#define GM_CS_INACTIVE 0xFFFFFFFFu
+#define GM_CS_NEVER 0xFFFFFFFEu   /* active, does not reach rxFrac >= 1 within the run */
+#define GM_CS_UNKNOWN 0xFFFFFFFDu /* infected in the previous step, increment not yet known */
+#define GM_CS_MAXSTEP 0xFFFFFFF0u
+
 /* error flags raised on the device */
 #define GM_ERRF_BORDER_FULL 0x1u
 #define GM_ERRF_INSIDE_FULL 0x2u
 #define GM_ERRF_PCACHE_FULL 0x4u
+#define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
 
-#define GM_TILE 64          /* slots per compaction tile (one wave) */
 #define GM_BLOCK 256
-#define GM_SCAN_BLOCKS 1024 /* fixed grid of the device-wide scans */
+#define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
+#define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
+#define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
 
 struct GmStats {
     uint32_t nI, nB;          /* nextSlotNeverActive of both lists */
     uint32_t sv;              /* ACTIVE slots seen by the sweep */
     uint32_t nfull;           /* parents (completed cells), Inside then Border */
-    uint32_t nfullI;
+    uint32_t nfullI, nfullB;
     uint32_t totRecI, totRecB;/* INACTIVE slots seen by the sweep */
     uint32_t recLenI, recLenB;/* recycled slots available (capped) */
     uint32_t winI, winB;      /* successful infections into each list */
+    uint32_t nIold, nBold;    /* list lengths before this step's appends */
     uint32_t npend;           /* P-cache entries inserted this step, still unresolved */
     uint32_t err;             /* GM_ERRF_* */
     uint32_t pone;            /* set when a pair weight was short-circuited to 1.0 */
-    unsigned long long rhomax_bits; /* max rho over deformed grains with cells left */
     uint32_t scan_tot[4];     /* totals written by the device-wide scans */
+    uint32_t nunknown;        /* cells whose increment the sweep computed (running total) */
+    uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
+    uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
 };
 
 struct GmDev {
@@ -53,27 +75,34 @@ struct GmDev {
     uint32_t recCapI, recCapB;
     /* state */
     uint32_t *grid, *claim;
-    uint8_t *act, *inf;
+    uint32_t *cstep, *s0;
+    double *frac0, *inc;
+    double *fin;                /* rxFrac after the completing addition */
+    uint8_t *inf;
     uint32_t *cell, *defg, *rxg;
-    double *frac, *P;
-    /* per-step compaction */
-    uint64_t *mfull, *mrec;     /* per tile masks */
-    uint32_t *tcntF, *tcntR;    /* per tile counts, then exclusive offsets */
-    uint32_t *full;             /* parents as global slot numbers */
-    uint32_t *recI, *recB;      /* recycled slots (list-relative), ascending */
-    uint32_t *pwinI, *pwinB;    /* per-parent winner masks over the 26 direction ranks */
-    uint32_t *pcntI, *pcntB;    /* per-parent counts, then exclusive offsets */
+    double *P;
+    /* per-step lists */
+    uint32_t *fullI, *fullB;    /* completed cells (global slots), ascending */
+    uint32_t *recI, *recB;      /* recycled slots (list-relative), ascending, capped */
+    uint32_t fullCapI;          /* capacity of fullI (fullB holds capB) */
+    uint64_t *mF, *mR;          /* per 64-slot tile: completed / INACTIVE masks */
+    uint32_t *pmI, *pmB;        /* per parent: winning ranks into Inside / Border */
+    uint32_t *bcnt;             /* per-block counts and offsets, 5 x GM_MAX_BLOCKS */
+    uint32_t *newl;             /* global slots of the cells infected in the last step */
+    uint32_t newCap;
     uint32_t tileCap, parentCap;
     /* grains */
     double *rho, *defq, *rxq;
     uint32_t *defcnt, *rxcnt;
+    uint32_t *zlist;            /* ids of grains exhausted in the current step */
     /* mobility-weight cache: open addressing on key rxg*ndefg+defg+1 */
     unsigned long long *hkeys;
     double *hvals;
     uint32_t hcap;
     uint32_t *pend_slot;
     unsigned long long *pend_key;
-    /* scans */
+    /* nucleation scratch */
+    uint32_t *nflag, *nI_, *nB_;
     uint32_t *scan_bsum;        /* 2*GM_SCAN_BLOCKS */
     GmStats *st;
 };
@@ -91,21 +120,27 @@ struct GmSweepParams {
     double Gbhalfsq, zener, dt, inv_dx;
 };
 
+struct GmSweepArgs {
+    GmSweepParams sp;
+    uint32_t k;                 /* step index */
+    uint32_t reanchor;          /* scalars changed since the previous step */
+    uint32_t nI, nB;            /* occupied slots */
+    uint32_t cI, cB;            /* sweep chunks per list */
+    uint32_t stepcap;           /* last step a completion can fall in */
+    uint32_t nnew;              /* cells infected in the previous step (listed in newl) */
+};
+
 /* launchers (gm_kernels.hip); all asynchronous on `s` */
-hipError_t gm_launch_begin_step(const GmDev &d, hipStream_t s);
-hipError_t gm_launch_sweep(const GmDev &d, const GmSweepParams &sp, uint32_t nI, uint32_t nB, hipStream_t s);
-hipError_t gm_launch_compact(const GmDev &d, uint32_t nI, uint32_t nB, hipStream_t s);
-hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, hipStream_t s);
-hipError_t gm_launch_rhomax(const GmDev &d, hipStream_t s);
+hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
+hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
-hipError_t gm_launch_resolve_front(const GmDev &d, uint32_t nI, uint32_t nB, hipStream_t s);
+hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, uint8_t *act, double *frac,
+                                 hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_f64_bits(double *p, unsigned long long bits, size_t n, hipStream_t s);
 hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys, const double *vals, uint32_t n,
                                  hipStream_t s);
-hipError_t gm_launch_defrag(const GmDev &d, uint32_t list, uint32_t first, uint32_t n, uint32_t *tmp_u32,
-                            uint8_t *tmp_u8, double *tmp_f64, hipStream_t s);
 
 #endif

@@ -98,6 +98,30 @@ def test_small_stepwise_sg(require_gpu):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("temps,nmax,xcells", [((1100.0, 1250.0), 30, 653), ((1150.0, 1300.0), 40, 0)])
+def test_temperature_ramp_stepwise_nmax(require_gpu, temps, nmax, xcells):
+    """A heating ramp changes the mobility, so the time step and every increment
+    change each step; nmax cuts the run while cells are predicted to complete only
+    after it.  Front values (rxFrac of every slot) are compared after each step.
+    In the second case the shear-modulus polynomial of update_atomistic is negative
+    above ~1150 K: no cell advances and the time step is negative, as in SCORE."""
+    dom = synth.make_voronoi_domain(28, 28, 28, n_grains=8, n_nuclei=20, seed=13)
+    prm = ParamsSpec(sched_time=(0.0, 3000.0), sched_temp=temps, nmax=nmax)
+    o, g = run_both(dom, prm, stepwise=True)
+    assert_final_equal(o, g)
+    r = g.result()
+    assert r["step"] == nmax + 1 and r["xcells"] == xcells
+    fr = g.front(abi.LIST_INSIDE)
+    assert fr["activity"].sum() > 0
+
+
+def test_temperature_ramp_voronoi100(require_gpu):
+    dom = synth.workload("voronoi100", seed=2, scale=0.5)
+    prm = ParamsSpec(sched_time=(0.0, 3.6e5), sched_temp=(900.0, 1100.0))
+    o, g = run_both(dom, prm)
+    assert_final_equal(o, g)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_voronoi100_scaled(require_gpu, seed):
     dom = synth.workload("voronoi100", seed=seed, scale=0.5)
@@ -185,3 +209,25 @@ def test_repeatable(require_gpu):
     b.solve_rxgrowth()
     assert np.array_equal(a.grainmap(), b.grainmap())
     assert_traj_equal(a.trajectory(), b.trajectory())
+
+
+def test_voronoi512_full_size_matches_oracle_fixture(require_gpu):
+    """The 512^3 workload (8,000 grains, 100,000 nuclei) against the oracle's complete
+    run of the same input, reduced to hashes in tests/golden/voronoi512_seed1.json."""
+    import hashlib
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "voronoi512_seed1.json")))
+    dom = synth.workload("voronoi512", seed=1)
+    h = lambda a: hashlib.sha256(a.tobytes()).hexdigest()  # noqa: E731
+    assert h(dom.cellgrid) == ref["domain_sha256"] and h(dom.rxg_site) == ref["sites_sha256"]
+    g = _gm(dom)
+    r = g.solve_rxgrowth()
+    assert r["step"] == ref["steps"] and r["t"] == ref["t_end"] and r["sum_sv"] == ref["sum_sv"]
+    assert r["nuclei_placed"] == ref["nuclei_placed"]
+    tr = g.trajectory()
+    tr["npending"] = 0
+    assert h(tr) == ref["trajectory_sha256"]
+    assert h(g.grainmap()) == ref["grainmap_sha256"]
+    dc, rc = g.counts()
+    assert h(dc) == ref["defg_counts_sha256"] and h(rc) == ref["rxg_counts_sha256"]
