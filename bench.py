@@ -6,11 +6,16 @@ One bench step = one complete growth run (SCORE's solve_RXGROWTH) of the
 when the timed region starts (the handle is created, and the nuclei placed,
 before it).  The metric is front cell-updates per second: the sum over all CA
 steps of the ACTIVE front cells advanced (SCORE's Sv) divided by the wall time
-of the run.  Kernel times come from HIP events recorded by the library on its
-own stream; the CPU baseline is the serial oracle timed on a bounded prefix of
-the same workload on the host.
+of the run.
+
+The timed runs record no events.  One more, untimed run records HIP events
+between the kernels of every step on the library's own stream and the work
+counters of each step; the roofline of the dominant kernel comes from it.  The
+CPU baseline is the serial oracle timed on a bounded prefix of the same
+workload on the host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --copy-calibration      # a plain 1 GiB device copy, for PMC scaling
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -27,10 +32,47 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_voronoi512.json")
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
+    """Algorithmic HBM bytes of one kernel over the profiled run, and the per-unit
+    figures used.  Infection kernels use the reference algorithm's per-unit bytes
+    (a completed cell reads its 26 neighbours' cell-grid entries, 104 B, and its
+    record, 30 B; a valid attempt is a 4 B atomic; a new record is 30 B plus its
+    4 B cell-grid write; retirement writes 4 B of grid and 8 B of counts).  Sweep
+    kernels use what this design's sweep must touch: a 4 B completion step per
+    occupied slot, 16 B of masks per 64-slot tile, 49 B to predict a new cell, 69 B
+    to re-anchor one, 4 B per list entry."""
+    full, att, new = pf["n_full"], pf["n_attempts"], pf["n_new"]
+    scanned, tiles = pf["slots_scanned"], (pf["slots_scanned"] + 63) // 64
+    table = {
+        "k_predict_new": (49 * pf["n_predicted"], "49 B per predicted cell"),
+        "k_sweep": (4 * scanned + 16 * tiles + 69 * pf["n_reanchored"],
+                    "4 B per occupied slot + 16 B per tile + 69 B per re-anchored slot"),
+        "k_compact": (16 * tiles + 4 * (full + pf["n_recycled"]), "16 B per tile + 4 B per list entry"),
+        "k_propose": (104 * full + 4 * att, "104 B per completed cell + 4 B per valid attempt"),
+        "k_decide": (104 * full + 4 * att + 8 * full, "104 B per completed cell + 4 B per valid attempt + 8 B masks"),
+        "k_commit": (34 * new + 42 * full, "34 B per new record + 42 B per completed cell (record 30, retire 12)"),
+    }
+    return table.get(name, (0, "scan of per-block counts"))
+
+
+def copy_calibration() -> int:
+    import torch
+    n = 1 << 30
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "dispatches": 5}), flush=True)
+    return 0
 
 
 def main(argv=None) -> int:
@@ -43,14 +85,18 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="bound of the CPU-baseline sample (oracle steps on the same domain)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the profiled run (no roofline)")
+    ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (domains shard as "
-              f"independent per-GPU queues, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: a domain runs on one GPU, "
+              f"and an ensemble of domains shards as one independent queue per GPU", file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration()
 
-    from score_amd import synth
+    from score_amd import abi, synth
     from score_amd.growth import GrowthMachine, device_count, same_trajectory
 
     if device_count() < 1:
@@ -63,35 +109,33 @@ def main(argv=None) -> int:
     log(f"workload {a.workload} seed {a.seed}: {dom.nx}x{dom.ny}x{dom.nz}, {dom.n_defg} grains, "
         f"{dom.n_rxg} nuclei, generated in {time.time() - t0:.1f}s")
 
-    runs = []
-    prof_tot = None
-    digest = None
-    for rep in range(a.warmup + a.steps):
-        timed = rep >= a.warmup
+    def one_run(profile: bool):
         gm = GrowthMachine(dom, prm)
-        gm.set_profiling(True)
+        if profile:
+            gm.set_profiling(True)
         t1 = time.perf_counter()
         res = gm.solve_rxgrowth()
         t2 = time.perf_counter()
         if not res["completed"]:
             raise RuntimeError("run did not complete")
-        pf = gm.profile()
-        gmap = gm.grainmap()
-        h = hashlib.sha256(gmap.tobytes()).hexdigest()[:16]
+        pf = gm.profile() if profile else None
+        h = hashlib.sha256(gm.grainmap().tobytes()).hexdigest()[:16]
+        gm.close()
+        return t2 - t1, res, pf, h
+
+    runs = []
+    digest = None
+    for rep in range(a.warmup + a.steps):
+        timed = rep >= a.warmup
+        secs, res, _, h = one_run(False)
         if digest is None:
             digest = h
         elif digest != h:
             raise RuntimeError("grain map differs between repetitions")
-        gm.close()
         log(f"{'run' if timed else 'warmup'} {rep}: {res['step']} CA steps, X={res['X']:.6f}, "
-            f"sum Sv={res['sum_sv']}, {t2 - t1:.3f}s, sweep {pf['ms_sweep']:.1f}ms infect {pf['ms_infect']:.1f}ms")
+            f"sum Sv={res['sum_sv']}, {secs:.3f}s")
         if timed:
-            runs.append((t2 - t1, res, pf))
-            if prof_tot is None:
-                prof_tot = dict(pf)
-            else:
-                for k, v in pf.items():
-                    prof_tot[k] += v
+            runs.append((secs, res))
 
     secs = sum(r[0] for r in runs)
     updates = sum(r[1]["sum_sv"] for r in runs)
@@ -99,29 +143,50 @@ def main(argv=None) -> int:
     ca_steps = runs[0][1]["step"]
     voxel_steps = dom.N * ca_steps * len(runs) / secs
 
-    # dominant phase and its roofline (algorithmic bytes / event-timed duration)
-    phases = {"sweep": (prof_tot["ms_sweep"], prof_tot["sweep_bytes"]),
-              "infect": (prof_tot["ms_infect"], prof_tot["infect_bytes"])}
-    dom_phase = max(phases, key=lambda k: phases[k][0])
-    ms, nbytes = phases[dom_phase]
-    launches = prof_tot["steps"]
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm", "kernel": "k_sweep" if dom_phase == "sweep" else "infection (k_propose..k_retire)",
-        "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "avg_launch_us": round(ms * 1e3 / launches, 3), "alg_bytes_per_launch": int(nbytes / launches),
-        "phase_ms_per_run": {k: round(v[0] / len(runs), 3) for k, v in phases.items()} | {
-            "compact": round(prof_tot["ms_compact"] / len(runs), 3), "tail": round(prof_tot["ms_tail"] / len(runs), 3),
-            "host_step_loop": round(prof_tot["ms_host"] / len(runs), 3)},
-    }
+    roofline = None
+    if not a.no_profile:
+        psecs, pres, pf, h = one_run(True)
+        if h != digest:
+            raise RuntimeError("profiled run differs")
+        steps = pf["steps"]
+        kms = pf["ms_kernel"]
+        dom_k = max(kms, key=lambda k: kms[k])
+        nbytes, units = kernel_bytes(dom_k, pf)
+        avg_us = kms[dom_k] * 1e3 / steps
+        achieved = nbytes / steps / (avg_us * 1e-6) / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            tr = json.load(open(TRAFFIC_FILE))
+            if tr.get("kernel") == dom_k:
+                traffic = tr.get("bytes_per_launch")
+        # effective bandwidth of the reference algorithm's bytes per phase
+        sweep_ref = 30 * pf["sum_sv"] + (pf["slots_scanned"] - pf["sum_sv"]) + 4 * (pf["n_full"] + pf["n_recycled"])
+        infect_ref = (pf["n_full"] * (26 * 4 + 30) + 4 * pf["n_attempts"] + pf["n_new"] * 34 + pf["n_full"] * 12)
+        sweep_ms = sum(kms[k] for k in ("k_predict_new", "k_sweep", "k_sweep_scan", "k_compact"))
+        infect_ms = sum(kms[k] for k in ("k_propose", "k_decide", "k_decide_scan", "k_commit"))
+        roofline = {
+            "bound": "hbm", "kernel": dom_k,
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_us": round(avg_us, 3), "alg_bytes_per_launch": int(nbytes / steps), "alg_bytes_units": units,
+            "kernel_ms_per_run": {k: round(v, 3) for k, v in kms.items()},
+            "device_ms_per_run": round(pf["ms_step"], 3), "host_loop_ms_per_run": round(pf["ms_host"], 3),
+            "reference_bytes_effective_GBs": {
+                "sweep": round(sweep_ref / (sweep_ms * 1e-3) / 1e9, 1),
+                "infection": round(infect_ref / (infect_ms * 1e-3) / 1e9, 1),
+                "step": round((sweep_ref + infect_ref) / (pf["ms_step"] * 1e-3) / 1e9, 1)},
+            "counters_per_run": {k: pf[k] for k in ("steps", "sum_sv", "slots_scanned", "n_full", "n_recycled",
+                                                    "n_attempts", "n_new", "n_predicted", "n_reanchor_steps",
+                                                    "n_reanchored", "pending_resolved")},
+        }
+        log(f"profiled run: {psecs:.3f}s; dominant kernel {dom_k} {avg_us:.1f} us/launch, "
+            f"{achieved:.0f} GB/s algorithmic; per-kernel ms {json.dumps(roofline['kernel_ms_per_run'])}")
 
     cpu = None
     if not a.no_cpu:
         from oracle.oracle import Oracle
         o = Oracle(dom, prm)
         k = 0
-        sv = 0
         tc0 = time.perf_counter()
         while time.perf_counter() - tc0 < a.cpu_seconds:
             if o.step(1):
@@ -134,8 +199,7 @@ def main(argv=None) -> int:
         # the GPU trajectory must agree with the oracle's over the sampled prefix
         gm = GrowthMachine(dom, prm)
         gm.step(len(tr))
-        gt = gm.trajectory()
-        prefix_ok = same_trajectory(gt, tr)
+        prefix_ok = same_trajectory(gm.trajectory(), tr)
         gm.close()
         o.close()
         cpu = {"value": sv / tc, "unit": "cell-updates/s", "cores": 1, "kind": "port",
@@ -149,7 +213,7 @@ def main(argv=None) -> int:
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * secs / len(runs), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{a.workload}: {dom.nx}x{dom.ny}x{dom.nz} voxels, dx=0.5um, {dom.n_defg} Voronoi "
-                               f"grains, {dom.n_rxg} site-saturated nuclei, Fe/Humphreys, T=1000K, run to X=0.99",
+                               f"grains, {dom.n_rxg} site-saturated nuclei, Fe/Rollett-Holm, T=1000K, run to X=0.99",
                    "seed": a.seed, "ca_steps_per_run": ca_steps, "sum_sv_per_run": runs[0][1]["sum_sv"],
                    "grainmap_sha256_16": digest, "parallelism": "1 GPU"},
         "voxel_steps_per_s": voxel_steps,

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 1
+#define SCORE_GM_ABI_VERSION 2
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -133,23 +133,37 @@ typedef struct score_gm_result {
     double loop_seconds;          /* host wall time spent inside step/run calls */
 } score_gm_result;
 
-/* Per-phase device time and work counters, the counterpart of SCORE's per-thread
+/* Kernels of one growth step, in launch order (index into ms_kernel). */
+#define SCORE_GM_K_PREDICT 0     /* increments and completion steps of last step's new cells */
+#define SCORE_GM_K_SWEEP 1       /* calcGrowthStepInside/Border over the occupied slots */
+#define SCORE_GM_K_SWEEP_SCAN 2  /* list offsets of the sweep's blocks */
+#define SCORE_GM_K_COMPACT 3     /* FullRX and recycling lists */
+#define SCORE_GM_K_GATHER 4      /* parent data in FullRX order */
+#define SCORE_GM_K_PROPOSE 5     /* every infection attempt claims its target */
+#define SCORE_GM_K_DECIDE 6      /* winners per parent */
+#define SCORE_GM_K_DECIDE_SCAN 7 /* ranks of the winners' blocks */
+#define SCORE_GM_K_COMMIT 8      /* new front records, retirement of the parents */
+#define SCORE_GM_NKERNELS 9
+
+/* Per-kernel device time and work counters, the counterpart of SCORE's per-thread
  * phase timers (omp_log_rxfrontstats, SCORE_Kernel.cpp:5019-5063; written to
  * ThreadProfilingGrowth.csv when OutputThreadProfiling is set).  Times are HIP
- * events recorded on the handle's stream around each phase of every step. */
+ * events recorded on the handle's stream between the kernels of every step;
+ * counters are sums over the profiled steps. */
 typedef struct score_gm_profile {
     uint64_t steps;               /* steps recorded while profiling was on */
-    double ms_sweep;              /* front sweep (calcGrowthStep Inside+Border) */
-    double ms_compact;            /* FullRX / recycling list compaction */
-    double ms_infect;             /* propose, decide, rank, commit, retire (updateFullRX) */
-    double ms_tail;               /* rhomax reduction and stats */
+    double ms_kernel[SCORE_GM_NKERNELS];
+    double ms_step;               /* device time from the first to the last kernel of each step */
     double ms_host;               /* host wall time of the profiled steps */
-    uint64_t sweep_slots;         /* front slots scanned */
-    uint64_t sweep_active;        /* ACTIVE slots advanced (sum of Sv) */
-    uint64_t sweep_bytes;         /* algorithmic HBM bytes of the sweep */
-    uint64_t infect_parents;      /* completed cells (sum of FullRX lengths) */
-    uint64_t infect_new;          /* new infections */
-    uint64_t infect_bytes;        /* algorithmic HBM bytes of the infection phase */
+    uint64_t sum_sv;              /* ACTIVE slots advanced (Sv) */
+    uint64_t slots_scanned;       /* occupied slots swept (nextSlotNeverActive of both lists) */
+    uint64_t n_full;              /* completed cells, the FullRX list lengths */
+    uint64_t n_recycled;          /* recycling-list entries written */
+    uint64_t n_attempts;          /* infection attempts that found a deformed target */
+    uint64_t n_new;               /* infections committed */
+    uint64_t n_predicted;         /* increments computed and completion steps predicted */
+    uint64_t n_reanchor_steps;    /* steps whose scalars changed: every cell re-anchored */
+    uint64_t n_reanchored;        /* slots re-anchored */
     uint64_t pending_resolved;    /* mobility weights resolved on the host */
 } score_gm_profile;
 

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 ERR_ARG = -1
@@ -112,14 +112,19 @@ class Result(C.Structure):
     ]
 
 
+# kernels of one growth step, in launch order (SCORE_GM_K_* in the header)
+KERNELS = ("k_predict_new", "k_sweep", "k_sweep_scan", "k_compact", "k_gather", "k_propose", "k_decide",
+           "k_decide_scan", "k_commit")
+NKERNELS = len(KERNELS)
+
+
 class Profile(C.Structure):
     _fields_ = [
         ("steps", C.c_uint64),
-        ("ms_sweep", C.c_double), ("ms_compact", C.c_double), ("ms_infect", C.c_double),
-        ("ms_tail", C.c_double), ("ms_host", C.c_double),
-        ("sweep_slots", C.c_uint64), ("sweep_active", C.c_uint64), ("sweep_bytes", C.c_uint64),
-        ("infect_parents", C.c_uint64), ("infect_new", C.c_uint64), ("infect_bytes", C.c_uint64),
-        ("pending_resolved", C.c_uint64),
+        ("ms_kernel", C.c_double * NKERNELS), ("ms_step", C.c_double), ("ms_host", C.c_double),
+        ("sum_sv", C.c_uint64), ("slots_scanned", C.c_uint64), ("n_full", C.c_uint64), ("n_recycled", C.c_uint64),
+        ("n_attempts", C.c_uint64), ("n_new", C.c_uint64), ("n_predicted", C.c_uint64),
+        ("n_reanchor_steps", C.c_uint64), ("n_reanchored", C.c_uint64), ("pending_resolved", C.c_uint64),
     ]
 
 

@@ -43,19 +43,6 @@ const double kDefaultPmax = 0.0;
 const double kDefaultSmallNumber = 1e-6;
 const double kFullRecyclingAtt = 3.0;
 
-/* Algorithmic HBM bytes (what the work needs at minimum, not what the kernels move).
- * sweep: every occupied slot, its completion step 4; a cell infected in the previous
- * step, its list entry 4 + completion step 4 + anchor 8 + P 8 + defg 4 + infector 1
- * read and increment 8 + completion step 4 + final rxFrac 8 written (49); every
- * list entry written 4.  A re-anchoring sweep reads 4+8+8+4+8+4+1 and writes
- * 8+4+8+4+8 per active slot (69).
- * infection, per completed cell: FullRX entry 4, parent record (cell, rxFrac, P, defg,
- * rxg) 28, 26 cell-grid reads 104, retirement (grid 4, completion step 4, two grain
- * counts 8) 16; per new infection: record 41, grid write 4, claim atomic 4 and reset 4. */
-const uint64_t kSweepBytesSlot = 4, kSweepBytesNew = 49, kSweepBytesListEntry = 4;
-const uint64_t kSweepBytesReanchor = 69;
-const uint64_t kInfectBytesParent = 4 + 28 + 104 + 16, kInfectBytesNew = 41 + 4 + 4 + 4;
-
 bool same_bits(double a, double b) { return memcmp(&a, &b, sizeof(double)) == 0; }
 
 bool same_scalars(const GmSweepParams &a, const GmSweepParams &b) {
@@ -216,7 +203,7 @@ struct score_gm_handle {
     std::vector<score_gm_traj_row> traj;
     /* profiling */
     bool prof = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[GM_NK + 1] = {};
     score_gm_profile profc{};
     /* staging for pending P */
     std::vector<unsigned long long> pend_keys;
@@ -258,8 +245,8 @@ void dfree(void *p) {
 
 void free_device(score_gm_handle *h) {
     GmDev &d = h->d;
-    void *ptrs[] = {d.grid, d.claim, d.cstep, d.s0, d.frac0, d.inc, d.fin, d.inf, d.cell, d.defg, d.rxg, d.P, d.fullI,
-                    d.fullB, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
+    void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot,
+                    d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, h->d_pend_vals};
     for (void *q : ptrs) dfree(q);
@@ -405,15 +392,6 @@ uint32_t pow2_at_least(uint64_t v) {
 /* per-step lists, tile masks and per-parent masks sized for the current occupancy */
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
-    if (h->nI > d.fullCapI) {
-        uint32_t nc = std::max<uint32_t>(h->nI, d.fullCapI + d.fullCapI / 2);
-        nc = std::min<uint32_t>(nc, d.capI);
-        dfree(d.fullI);
-        d.fullI = nullptr;
-        int rc = dalloc(&d.fullI, nc);
-        if (rc) return rc;
-        d.fullCapI = nc;
-    }
     const uint32_t tiles = (h->nI + 63) / 64 + (h->nB + 63) / 64;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
@@ -426,10 +404,14 @@ int ensure_step_capacity(score_gm_handle *h) {
     const uint32_t parents = h->nI + h->nB;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
-        dfree(d.pmI); dfree(d.pmB);
-        d.pmI = d.pmB = nullptr;
+        dfree(d.pmI); dfree(d.pmB); dfree(d.pslot); dfree(d.par);
+        d.pmI = d.pmB = d.pslot = nullptr;
+        d.par = nullptr;
+        d.parentCap = 0;
         int rc;
-        if ((rc = dalloc(&d.pmI, nc)) || (rc = dalloc(&d.pmB, nc))) return rc;
+        if ((rc = dalloc(&d.pmI, nc)) || (rc = dalloc(&d.pmB, nc)) || (rc = dalloc(&d.pslot, nc)) ||
+            (rc = dalloc(&d.par, nc)))
+            return rc;
         d.parentCap = nc;
     }
     /* a step infects distinct deformed cells, at most 26 per completed cell */
@@ -485,8 +467,9 @@ int resolve_pending(score_gm_handle *h, uint32_t npend) {
 int grow_pcache(score_gm_handle *h) {
     GmDev &d = h->d;
     if ((uint64_t)h->hcount * 2 <= d.hcap) return 0;
-    HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, nullptr, nullptr, h->stream));
-    HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, nullptr, nullptr, h->stream));
+    const GmFrontOut none{};
+    HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, none, h->stream));
+    HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, none, h->stream));
     std::vector<unsigned long long> keys(d.hcap);
     std::vector<double> vals(d.hcap);
     HIPCHK(hipMemcpyAsync(keys.data(), d.hkeys, sizeof(unsigned long long) * d.hcap, hipMemcpyDeviceToHost, h->stream));
@@ -505,7 +488,7 @@ int grow_pcache(score_gm_handle *h) {
         return rc;
     d.hcap = ncap;
     HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, ncap, h->stream));
-    HIPCHK(gm_launch_fill_f64_bits(d.hvals, GM_PENDING_TAG, ncap, h->stream));
+    HIPCHK(gm_launch_fill_u64((unsigned long long *)d.hvals, GM_PENDING_TAG, ncap, h->stream));
     unsigned long long *dk = nullptr;
     double *dv = nullptr;
     if ((rc = dalloc(&dk, k2.size())) || (rc = dalloc(&dv, v2.size()))) { dfree(dk); return rc; }
@@ -540,13 +523,11 @@ int step_once(score_gm_handle *h) {
     a.nnew = h->nnew;
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
-    auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(h->ev[i], h->stream) : hipSuccess; };
+    hipEvent_t *ev = prof ? h->ev : nullptr;
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
-    if ((e = mark(0)) != hipSuccess || (e = gm_launch_sweep(d, a, h->stream)) != hipSuccess ||
-        (e = mark(1)) != hipSuccess ||
-        (e = gm_launch_infect(d, h->pp, h->step, slots0, h->stream)) != hipSuccess ||
-        (e = mark(2)) != hipSuccess ||
+    if ((e = gm_launch_sweep(d, a, h->stream, ev)) != hipSuccess ||
+        (e = gm_launch_infect(d, h->pp, h->step, slots0, h->stream, ev)) != hipSuccess ||
         (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
@@ -556,28 +537,33 @@ int step_once(score_gm_handle *h) {
     st.nfull = st.nfullI + st.nfullB;
     st.sv = slots0 - (st.totRecI + st.totRecB);
     /* increments computed: the listed new cells, and the cells the sweep met unpredicted */
-    const uint32_t nunknown = st.nunknown - h->nunknown_seen + (a.reanchor ? 0u : std::min(a.nnew, d.newCap));
+    const uint32_t npredicted = st.nunknown - h->nunknown_seen + (a.reanchor ? 0u : std::min(a.nnew, d.newCap));
     h->nunknown_seen = st.nunknown;
     h->nnew = st.winI + st.winB;
     if (prof) {
-        float ms[2];
-        for (int i = 0; i < 2; i++)
-            if ((e = hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1])) != hipSuccess)
-                return h->err = hip_fail(e, "hipEventElapsedTime");
         score_gm_profile &pf = h->profc;
+        for (int i = 0; i < GM_NK; i++) {
+            float ms;
+            if ((e = hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1])) != hipSuccess)
+                return h->err = hip_fail(e, "hipEventElapsedTime");
+            pf.ms_kernel[i] += ms;
+        }
+        float ms;
+        if ((e = hipEventElapsedTime(&ms, h->ev[0], h->ev[GM_NK])) != hipSuccess)
+            return h->err = hip_fail(e, "hipEventElapsedTime");
+        pf.ms_step += ms;
         pf.steps++;
-        pf.ms_sweep += ms[0];
-        pf.ms_infect += ms[1];
-        pf.sweep_slots += slots0;
-        pf.sweep_active += st.sv;
-        const uint64_t lists = kSweepBytesListEntry * ((uint64_t)st.nfull + st.recLenI + st.recLenB);
-        if (a.reanchor)
-            pf.sweep_bytes += kSweepBytesReanchor * (uint64_t)st.sv + kSweepBytesSlot * (slots0 - st.sv) + lists;
-        else
-            pf.sweep_bytes += kSweepBytesSlot * (uint64_t)slots0 + kSweepBytesNew * (uint64_t)nunknown + lists;
-        pf.infect_parents += st.nfull;
-        pf.infect_new += (uint64_t)st.winI + st.winB;
-        pf.infect_bytes += kInfectBytesParent * (uint64_t)st.nfull + kInfectBytesNew * ((uint64_t)st.winI + st.winB);
+        pf.sum_sv += st.sv;
+        pf.slots_scanned += slots0;
+        pf.n_full += st.nfull;
+        pf.n_recycled += (uint64_t)st.recLenI + st.recLenB;
+        pf.n_attempts += st.nattempt;
+        pf.n_new += (uint64_t)st.winI + st.winB;
+        pf.n_predicted += a.reanchor ? 0u : npredicted;
+        if (a.reanchor) {
+            pf.n_reanchor_steps++;
+            pf.n_reanchored += st.sv;
+        }
         pf.pending_resolved += st.npend;
     }
     if (st.err) {
@@ -752,7 +738,9 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
 
     const size_t capTot = (size_t)d.capI + d.capB;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
-    d.fullCapI = (uint32_t)std::min<uint64_t>(d.capI, std::max<uint64_t>(65536, d.N / 16));
+    d.dnbx = (d.nx + 15) / 16;
+    d.dnby = (d.ny + 7) / 8;
+    d.dnbz = (d.nz + 7) / 8;
     d.tileCap = (uint32_t)std::max<uint64_t>(1024, (d.N / 16) / 64);
     d.parentCap = (uint32_t)std::max<uint64_t>(65536, d.N / 16);
     d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
@@ -762,10 +750,9 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
     }
     if ((rc = dalloc(&d.grid, d.N)) || (rc = dalloc(&d.claim, d.N)) || (rc = dalloc(&d.cstep, capTot)) ||
-        (rc = dalloc(&d.s0, capTot)) || (rc = dalloc(&d.frac0, capTot)) || (rc = dalloc(&d.inc, capTot)) ||
-        (rc = dalloc(&d.fin, capTot)) || (rc = dalloc(&d.newl, d.newCap)) || (rc = dalloc(&d.inf, capTot)) || (rc = dalloc(&d.cell, capTot)) || (rc = dalloc(&d.defg, capTot)) ||
-        (rc = dalloc(&d.rxg, capTot)) || (rc = dalloc(&d.P, capTot)) || (rc = dalloc(&d.fullI, d.fullCapI)) ||
-        (rc = dalloc(&d.fullB, d.capB)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
+        (rc = dalloc(&d.rec, capTot)) || (rc = dalloc(&d.newl, d.newCap)) ||
+        (rc = dalloc(&d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(&d.pslot, d.parentCap)) ||
+        (rc = dalloc(&d.par, d.parentCap)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
         (rc = dalloc(&d.mF, d.tileCap)) || (rc = dalloc(&d.mR, d.tileCap)) || (rc = dalloc(&d.pmI, d.parentCap)) ||
         (rc = dalloc(&d.pmB, d.parentCap)) || (rc = dalloc(&d.bcnt, 5 * GM_MAX_BLOCKS)) ||
         (rc = dalloc(&d.rho, d.ndefg)) || (rc = dalloc(&d.defq, 4 * (size_t)d.ndefg)) ||
@@ -792,18 +779,16 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
         HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
+        HIPCHK(gm_launch_dmap_init(d, s));
         HIPCHK(gm_launch_fill_u32(d.cstep, GM_CS_INACTIVE, capTot, s));
-        HIPCHK(gm_launch_fill_u32(d.s0, 0u, capTot, s));
-        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.frac0, 0ull, capTot, s));
-        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.inc, 0ull, capTot, s));
-        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.fin, 0ull, capTot, s));
+        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.rec, 0ull, capTot * (sizeof(GmRec) / 8), s));
         HIPCHK(hipMemcpyAsync(d.rho, dom->defg_rho, sizeof(double) * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.defq, h->defq.data(), sizeof(double) * 4 * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.rxq, h->rxq.data(), sizeof(double) * h->rxq.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.defcnt, defcnt.data(), sizeof(uint32_t) * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(gm_launch_fill_u32(d.rxcnt, 0u, (size_t)d.nrxg + 1, s));
         HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, d.hcap, s));
-        HIPCHK(gm_launch_fill_f64_bits(d.hvals, GM_PENDING_TAG, d.hcap, s));
+        HIPCHK(gm_launch_fill_u64((unsigned long long *)d.hvals, GM_PENDING_TAG, d.hcap, s));
         HIPCHK(gm_launch_fill_u32((uint32_t *)d.st, 0u, sizeof(GmStats) / 4, s));
         int r2;
         if ((r2 = dalloc(&d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(&d_nucP, (size_t)d.nrxg + 1))) return r2;
@@ -900,8 +885,20 @@ int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding) {
     if (encoding != SCORE_GM_MAP_CELLGRID && encoding != SCORE_GM_MAP_RAW2GID)
         return fail(SCORE_GM_ERR_ARG, "unknown encoding");
     HIPCHK(hipSetDevice(h->device));
-    HIPCHK(hipMemcpyAsync(out, h->d.grid, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    /* the grid with the front's cells marked infected, assembled in a device buffer */
+    uint32_t *tmp = nullptr;
+    int rc = dalloc(&tmp, h->d.N);
+    if (rc) return rc;
+    auto copy = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(tmp, h->d.grid, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(gm_launch_overlay_infected(h->d, h->nI, h->nB, tmp, h->stream));
+        HIPCHK(hipMemcpyAsync(out, tmp, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    };
+    rc = copy();
+    dfree(tmp);
+    if (rc) return rc;
     if (encoding == SCORE_GM_MAP_RAW2GID)
         for (uint64_t i = 0; i < h->d.N; i++) out[i] = out[i] == GM_INFECTED ? 1u : 2u + out[i];
     return 0;
@@ -938,28 +935,32 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
     hipStream_t s = h->stream;
     if (m) {
         /* activity and rxFrac as SCORE holds them before the next step, P without pending markers */
-        uint8_t *dact = nullptr;
-        double *dfrac = nullptr;
-        int rc;
-        if ((rc = dalloc(&dact, m)) || (rc = dalloc(&dfrac, m))) {
-            dfree(dact);
-            return rc;
-        }
+        char *buf = nullptr;
+        const size_t bytes = (size_t)m * (1 + 1 + 4 + 4 + 4 + 8 + 8);
+        int rc = dalloc(&buf, bytes);
+        if (rc) return rc;
+        GmFrontOut o;
+        o.frac = reinterpret_cast<double *>(buf);
+        o.P = o.frac + m;
+        o.cell = reinterpret_cast<uint32_t *>(o.P + m);
+        o.defg = o.cell + m;
+        o.rxg = o.defg + m;
+        o.act = reinterpret_cast<uint8_t *>(o.rxg + m);
+        o.inf = o.act + m;
         auto copy = [&]() -> int {
-            HIPCHK(gm_launch_materialize(h->d, h->step, base, m, dact, dfrac, s));
-            if (activity) HIPCHK(hipMemcpyAsync(activity, dact, m, hipMemcpyDeviceToHost, s));
-            if (rxfrac) HIPCHK(hipMemcpyAsync(rxfrac, dfrac, 8ull * m, hipMemcpyDeviceToHost, s));
-            if (infector) HIPCHK(hipMemcpyAsync(infector, h->d.inf + base, m, hipMemcpyDeviceToHost, s));
-            if (cell) HIPCHK(hipMemcpyAsync(cell, h->d.cell + base, 4ull * m, hipMemcpyDeviceToHost, s));
-            if (P) HIPCHK(hipMemcpyAsync(P, h->d.P + base, 8ull * m, hipMemcpyDeviceToHost, s));
-            if (defg) HIPCHK(hipMemcpyAsync(defg, h->d.defg + base, 4ull * m, hipMemcpyDeviceToHost, s));
-            if (rxg) HIPCHK(hipMemcpyAsync(rxg, h->d.rxg + base, 4ull * m, hipMemcpyDeviceToHost, s));
+            HIPCHK(gm_launch_materialize(h->d, h->step, base, m, o, s));
+            if (activity) HIPCHK(hipMemcpyAsync(activity, o.act, m, hipMemcpyDeviceToHost, s));
+            if (rxfrac) HIPCHK(hipMemcpyAsync(rxfrac, o.frac, 8ull * m, hipMemcpyDeviceToHost, s));
+            if (infector) HIPCHK(hipMemcpyAsync(infector, o.inf, m, hipMemcpyDeviceToHost, s));
+            if (cell) HIPCHK(hipMemcpyAsync(cell, o.cell, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (P) HIPCHK(hipMemcpyAsync(P, o.P, 8ull * m, hipMemcpyDeviceToHost, s));
+            if (defg) HIPCHK(hipMemcpyAsync(defg, o.defg, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (rxg) HIPCHK(hipMemcpyAsync(rxg, o.rxg, 4ull * m, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             return 0;
         };
         rc = copy();
-        dfree(dact);
-        dfree(dfrac);
+        dfree(buf);
         return rc;
     }
     return 0;

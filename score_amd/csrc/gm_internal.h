@@ -2,13 +2,20 @@
  * gm_internal.h -- device layout and kernel launchers of the growth machine.
  *
  * One domain lives in HBM as:
- *   grid   uint32[N]   cell states: deformed id (< ndefg), ndefg+rxg (transformed) or INFECTED
- *   claim  uint32[N]   per-cell infection claim key, 0xFFFFFFFF when free
- *   front  SoA over capI+capB slots: Inside list slots [0,capI), Border list slots [capI,capI+capB)
- *          hot:  cstep u32 (completion step or a GM_CS_* sentinel)
- *          warm: frac0 f64, s0 u32, inc f64 (rxFrac anchor: value before step s0, per-step increment),
- *                fin f64 (rxFrac after the predicted completing addition)
- *          cold: cell u32, P f64, defg u32, rxg u32, inf u8
+ *   grid   uint32[N]   deformed-grain id (< ndefg) until the cell transforms, then
+ *                      ndefg+rxg; a cell on the front keeps its deformed id (the
+ *                      exported map marks it INFECTED from the front records)
+ *   claim  uint32[N]   per-cell infection claim key, 0xFFFFFFFF until claimed; a
+ *                      claimed cell is infected in the same step and never
+ *                      claimed again, so claims are never reset
+ *   dmap   1 bit per cell, set while the cell is deformed (infectable), blocked so
+ *          that one 128-byte line holds a 16x8x8 brick of cells: a 3x3x3
+ *          neighbourhood usually lies in one or two lines
+ *   front  capI+capB slots: Inside list slots [0,capI), Border list slots [capI,capI+capB)
+ *          cstep uint32[slots]   completion step or a GM_CS_* sentinel; the only
+ *                                array every step reads in full
+ *          rec   GmRec[slots]    64-byte record, read and written only when a cell
+ *                                is infected, predicted, re-anchored or completes
  * The two lists keep SCORE's slot numbering (SCORE_Kernel.cpp:3587-3874) so that
  * slot-level state can be compared with a serial run.
  *
@@ -16,9 +23,9 @@
  * inc = ((v*dt)*inv_dx) that SCORE adds every step (SCORE_Kernel.cpp:4092).  While
  * the per-step scalars stay bit-identical, rxFrac after step k is frac0 plus
  * (k-s0+1) rounded additions of inc, so the step in which it first reaches 1.0
- * is known in advance (cstep) by replaying the additions.  When any scalar
- * changes, every active cell is re-anchored with the old increment and
- * re-predicted with the new one.
+ * (cstep) and the value it then holds (fin) are computed in advance
+ * (gm_radd.h).  When any scalar changes, every active cell is re-anchored with
+ * the old increment and predicted again with the new one.
  */
 #ifndef SCORE_GM_INTERNAL_H
 #define SCORE_GM_INTERNAL_H
@@ -47,6 +54,33 @@
 #define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
+#define GM_NK 9                   /* kernels per step (SCORE_GM_K_* order) */
+
+/* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
+ * anchor + increment + predicted final value.  Field pairs that kernels move
+ * together share a 16-byte quarter. */
+struct __attribute__((aligned(64))) GmRec {
+    double frac0;   /* rxFrac before step s0 */
+    double P;       /* mobility weight, or a pending marker */
+    double inc;     /* per-step increment, 0.0 when SCORE does not advance the cell */
+    double fin;     /* rxFrac after the predicted completing addition */
+    uint32_t s0;    /* anchor step */
+    uint32_t cell;  /* global cell id x + y*nx + z*nx*ny */
+    uint32_t defg;  /* deformed grain being consumed (mydefgid) */
+    uint32_t rxg;   /* growing nucleus (myrxgid) */
+    uint32_t inf;   /* infector code 0..26 */
+    uint32_t pad[3];
+};
+static_assert(sizeof(GmRec) == 64, "GmRec is one 64-byte record");
+
+/* One completed cell of the current step, in FullRX list order (Inside list
+ * first): what the infection kernels need of the parent, read coalesced. */
+struct __attribute__((aligned(16))) GmPar {
+    uint32_t slot, cell, defg, rxg;
+    double fin, P;
+    uint32_t x, y, z, fromB;
+};
+static_assert(sizeof(GmPar) == 48, "GmPar is three 16-byte quarters");
 
 struct GmStats {
     uint32_t nI, nB;          /* nextSlotNeverActive of both lists */
@@ -62,6 +96,7 @@ struct GmStats {
     uint32_t pone;            /* set when a pair weight was short-circuited to 1.0 */
     uint32_t scan_tot[4];     /* totals written by the device-wide scans */
     uint32_t nunknown;        /* cells whose increment the sweep computed (running total) */
+    uint32_t nattempt;        /* attempts that found a deformed target this step */
     uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
 };
@@ -75,18 +110,17 @@ struct GmDev {
     uint32_t recCapI, recCapB;
     /* state */
     uint32_t *grid, *claim;
-    uint32_t *cstep, *s0;
-    double *frac0, *inc;
-    double *fin;                /* rxFrac after the completing addition */
-    uint8_t *inf;
-    uint32_t *cell, *defg, *rxg;
-    double *P;
+    uint32_t *dmap;
+    uint32_t dnbx, dnby, dnbz;  /* bricks of the deformed-cell bitmap per axis */
+    uint32_t *cstep;
+    GmRec *rec;
     /* per-step lists */
-    uint32_t *fullI, *fullB;    /* completed cells (global slots), ascending */
+    uint32_t *pslot;            /* completed cells (global slots), Inside then Border, ascending each */
+    GmPar *par;                 /* the same cells' parent data */
     uint32_t *recI, *recB;      /* recycled slots (list-relative), ascending, capped */
-    uint32_t fullCapI;          /* capacity of fullI (fullB holds capB) */
     uint64_t *mF, *mR;          /* per 64-slot tile: completed / INACTIVE masks */
     uint32_t *pmI, *pmB;        /* per parent: winning ranks into Inside / Border */
+    /* pslot, par, pmI and pmB hold parentCap entries */
     uint32_t *bcnt;             /* per-block counts and offsets, 5 x GM_MAX_BLOCKS */
     uint32_t *newl;             /* global slots of the cells infected in the last step */
     uint32_t newCap;
@@ -125,21 +159,30 @@ struct GmSweepArgs {
     uint32_t k;                 /* step index */
     uint32_t reanchor;          /* scalars changed since the previous step */
     uint32_t nI, nB;            /* occupied slots */
-    uint32_t cI, cB;            /* sweep chunks per list */
     uint32_t stepcap;           /* last step a completion can fall in */
     uint32_t nnew;              /* cells infected in the previous step (listed in newl) */
 };
 
-/* launchers (gm_kernels.hip); all asynchronous on `s` */
-hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
-hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s);
+/* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
+ * ev[i] is recorded before kernel i of the step and ev[GM_NK] after the last. */
+hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
+hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s,
+                            hipEvent_t *ev);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
-hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, uint8_t *act, double *frac,
+/* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */
+struct GmFrontOut {
+    uint8_t *act, *inf;
+    uint32_t *cell, *defg, *rxg;
+    double *frac, *P;
+};
+hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, const GmFrontOut &o,
                                  hipStream_t s);
+hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s);
+/* the cell grid with every cell held by an active front record marked INFECTED */
+hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
-hipError_t gm_launch_fill_f64_bits(double *p, unsigned long long bits, size_t n, hipStream_t s);
 hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys, const double *vals, uint32_t n,
                                  hipStream_t s);
 

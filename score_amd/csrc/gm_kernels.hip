@@ -5,27 +5,38 @@
  * is, in serial order: sweep the Inside list, sweep the Border list, let every
  * completed Inside cell infect its 26 neighbours in list order, then every
  * completed Border cell.  A target is won by the first attempt in that order
- * that finds it deformed.  The same outcome is computed here by three kernels:
+ * that finds it deformed.  The same outcome is computed by nine kernels:
  *
- *   k_sweep    chunk-ordered pass over the occupied slots of both lists.  A slot
- *              costs one 4-byte read (its predicted completion step); cells
- *              infected in the previous step get their increment and prediction,
- *              and after a change of the per-step scalars every cell is
- *              re-anchored.  Completed and INACTIVE slots are ballot-compacted
- *              per 64-slot tile and placed by a decoupled look-back over chunks,
- *              giving the FullRX lists (ascending slot) and the capped recycling
- *              lists in one pass.
- *   k_propose  every (parent p, direction rank r) attempt on a deformed cell does
- *              atomicMin(claim[target], p*32 + r): the key orders attempts exactly
- *              as the serial loop issues them.
- *   k_infect   chunk-ordered pass over parents: an attempt wins iff its key is
- *              the minimum; winners per parent and destination list are counted,
- *              a look-back over chunks ranks every winner among the infections
- *              into its list (rank k takes the k-th recycled slot, else appends),
- *              the new records are written and the parents retire.
+ *   k_predict_new  increments and completion steps of the cells infected in the
+ *                  previous step, from a dense list (every lane busy).
+ *   k_sweep        pass over the occupied slots of both lists, statically
+ *                  partitioned over the grid.  A slot costs one 4-byte read (its
+ *                  predicted completion step); after a change of the per-step
+ *                  scalars every cell is re-anchored and predicted again.
+ *                  Completed and INACTIVE slots are balloted into per-tile masks
+ *                  and counted per block.
+ *   k_sweep_scan   one block scans the per-block counts into offsets.
+ *   k_compact      one wave per sweep block writes the completed slots in FullRX
+ *                  order (Inside list, then Border) and the capped recycling
+ *                  lists at those offsets.
+ *   k_gather       one thread per completed cell copies what the infection needs
+ *                  of it (ids, rxFrac, P, coordinates) into a dense array.
+ *   k_propose      every (parent p, direction rank r) attempt on a deformed cell
+ *                  (one bit of a brick-blocked bitmap) does
+ *                  atomicMin(claim[target], p*32 + r): the key orders attempts
+ *                  exactly as the serial loop issues them.
+ *   k_decide       an attempt wins iff its key is the minimum; winners per parent
+ *                  and destination list as bit masks, counted per block.
+ *   k_decide_scan  one block scans those counts: rank k into a list takes the
+ *                  k-th recycled slot, else appends.
+ *   k_commit       one thread per winner writes the new record, lists it for the
+ *                  next prediction and clears the cell's bitmap bit; then the
+ *                  parents retire.
  *
- * FP64 throughout, compiled with -ffp-contract=off so every product and sum is
- * rounded exactly as in the reference's build without FMA.
+ * No kernel waits on another block: every cross-block dependency is a kernel
+ * boundary, so the passes use only relaxed device atomics.  FP64 throughout,
+ * compiled with -ffp-contract=off so every product and sum is rounded exactly as
+ * in the reference's build without FMA.
  */
 #include "gm_internal.h"
 #include "gm_radd.h"
@@ -155,9 +166,6 @@ hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_
     k_fill_u64<<<fill_blocks(n), GM_BLOCK, 0, s>>>(p, v, n);
     return hipGetLastError();
 }
-hipError_t gm_launch_fill_f64_bits(double *p, unsigned long long bits, size_t n, hipStream_t s) {
-    return gm_launch_fill_u64((unsigned long long *)p, bits, n, s);
-}
 
 static __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -169,52 +177,82 @@ static __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+/* ------------------------------------------------------------ front records */
+
+static __device__ __forceinline__ double2 rec_q0(const GmRec *r) { /* frac0, P */
+    return *reinterpret_cast<const double2 *>(&r->frac0);
+}
+static __device__ __forceinline__ uint4 rec_q2(const GmRec *r) { /* s0, cell, defg, rxg */
+    return *reinterpret_cast<const uint4 *>(&r->s0);
+}
+static __device__ __forceinline__ void rec_set_q0(GmRec *r, double frac0, double P) {
+    *reinterpret_cast<double2 *>(&r->frac0) = make_double2(frac0, P);
+}
+static __device__ __forceinline__ void rec_set_q1(GmRec *r, double inc, double fin) {
+    *reinterpret_cast<double2 *>(&r->inc) = make_double2(inc, fin);
+}
+static __device__ __forceinline__ void rec_set_q2(GmRec *r, uint32_t s0, uint32_t cell, uint32_t defg, uint32_t rxg) {
+    *reinterpret_cast<uint4 *>(&r->s0) = make_uint4(s0, cell, defg, rxg);
+}
+
+/* a mobility weight with its pending marker replaced by the cached value */
+static __device__ __forceinline__ double resolved(const GmDev &d, double P) {
+    return is_pending(P) ? d.hvals[(uint32_t)__double_as_longlong(P)] : P;
+}
+
 /* ------------------------------------------------------------------- sweep */
 
 /* rxFrac increment of one cell: SCORE's v = m*p; v *= shape; rxFrac += (v*dt)*inv_dx
  * (SCORE_Kernel.cpp:4077-4092, :5459-5470); returns false when p <= DOUBLE_ACCURACY
  * (the cell is not advanced) */
-static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSweepParams &sp, uint32_t s,
-                                                      double &inc) {
-    double P = d.P[s];
-    if (is_pending(P)) {
-        P = d.hvals[(uint32_t)__double_as_longlong(P)];
-        d.P[s] = P;
-    }
+static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSweepParams &sp, double P, uint32_t defg,
+                                                      uint32_t inf, double &inc) {
     double m;
     if (sp.model == GM_RH) m = P * sp.mRH;
     else if (P < 0.0) m = sp.mLAGB;
     else m = (P * sp.mGS) + ((1.0 - P) * sp.mHAGB);
-    double p = (sp.Gbhalfsq * (d.rho[d.defg[s]] - GM_RHO_RX)) - sp.zener;
+    double p = (sp.Gbhalfsq * (d.rho[defg] - GM_RHO_RX)) - sp.zener;
     if (!(p > GM_DOUBLE_ACCURACY)) return false;
     double v = m * p;
-    v *= c_shape[d.inf[s]];
+    v *= c_shape[inf];
     inc = (v * sp.dt) * sp.inv_dx;
     return true;
 }
 
 /* Completion step of slot s from its value f before step k: the first step
  * >= k in which repeated rounded additions of the increment reach 1.0.  Stores
- * the increment (0.0 for a cell SCORE does not advance), the completion step
- * and the value after the completing addition, whose excess over 1.0 the
- * infection carries on. */
-static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f) {
-    double inc;
+ * the increment (0.0 for a cell SCORE does not advance), the value after the
+ * completing addition, whose excess over 1.0 the infection carries on, and the
+ * completion step. */
+static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
+                                                        double P, uint32_t defg, uint32_t inf) {
+    double inc, fin = 0.0;
     uint32_t cs = GM_CS_NEVER;
-    if (cell_increment(d, A.sp, s, inc)) {
+    if (cell_increment(d, A.sp, P, defg, inf, inc)) {
         if (A.k <= A.stepcap) {
             const uint64_t j = gm_radd_until_one(f, inc, (uint64_t)(A.stepcap - A.k) + 1u);
             if (j) {
                 cs = A.k + (uint32_t)(j - 1u);
-                d.fin[s] = f;
+                fin = f;
             }
         }
     } else {
         inc = 0.0;
     }
-    d.inc[s] = inc;
+    rec_set_q1(d.rec + s, inc, fin);
     d.cstep[s] = cs;
     return cs;
+}
+
+/* a cell infected in the previous step (anchor at step k): resolve its weight and predict */
+static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmSweepArgs &A, uint32_t s) {
+    GmRec *r = d.rec + s;
+    const double2 q0 = rec_q0(r);
+    const uint4 q2 = rec_q2(r);
+    const uint32_t inf = r->inf;
+    const double P = resolved(d, q0.y);
+    if (is_pending(q0.y)) rec_set_q0(r, q0.x, P);
+    return predict_slot(d, A, s, q0.x, P, q2.z, inf);
 }
 
 /* calcGrowthStepInside/Border (SCORE_Kernel.cpp:4046-4249) for one slot whose
@@ -224,26 +262,30 @@ static __device__ __forceinline__ int sweep_slot(const GmDev &d, const GmSweepAr
     if (cs == GM_CS_INACTIVE) return 2;
     const uint32_t k = A.k;
     if (A.reanchor) {
-        double f = d.frac0[s];
-        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, d.inc[s], k - d.s0[s]);
-        d.frac0[s] = f;
-        d.s0[s] = k;
-        cs = predict_slot(d, A, s, f);
+        /* rxFrac before step k under the old increment, then a new prediction */
+        GmRec *r = d.rec + s;
+        const double2 q0 = rec_q0(r);
+        const uint4 q2 = rec_q2(r);
+        const uint32_t inf = r->inf;
+        double f = q0.x;
+        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - q2.x);
+        const double P = resolved(d, q0.y);
+        rec_set_q0(r, f, P);
+        r->s0 = k;
+        cs = predict_slot(d, A, s, f, P, q2.z, inf);
     } else if (cs == GM_CS_UNKNOWN) {
-        cs = predict_slot(d, A, s, d.frac0[s]);
+        cs = predict_new(d, A, s);
     } else if (cs < k) {
         atomicOr(&d.st->err, GM_ERRF_MISSED);
     }
     return cs == k ? 1 : 0;
 }
 
-/* cells infected in the previous step, listed by the infection pass, predicted
+/* cells infected in the previous step, listed by the commit pass, predicted
  * with every lane of a wave busy */
 __global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A, uint32_t n) {
-    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
-        const uint32_t s = d.newl[i];
-        if (d.cstep[s] == GM_CS_UNKNOWN) predict_slot(d, A, s, d.frac0[s]);
-    }
+    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK)
+        predict_new(d, A, d.newl[i]);
 }
 
 /* per-block list counts: [0] full Inside, [1] recycled Inside, [2] full Border,
@@ -269,8 +311,8 @@ static __device__ __forceinline__ uint32_t tile_slot0(const GmDev &d, uint32_t t
 }
 
 /* pass 1: advance/predict every occupied slot, ballot the completed and the
- * INACTIVE ones into per-tile masks, count them per block and list.  Each wave
- * loads the completion steps of GM_SW_UNROLL tiles before working on them. */
+ * INACTIVE ones into per-tile masks, count them per block.  Each wave loads the
+ * completion steps of GM_SW_UNROLL tiles before working on them. */
 #define GM_SW_UNROLL 4
 __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint32_t tI, uint32_t T) {
     __shared__ uint32_t red[GM_BLOCK / 64][5];
@@ -389,71 +431,56 @@ __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
     }
 }
 
-/* block-wide exclusive scan of one value per thread (GM_BLOCK threads) */
-static __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t (*ws)[GM_BLOCK / 64],
-                                                           int c) {
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane == 63) ws[c][w] = incl;
-    __syncthreads();
-    uint32_t pre = 0, t = 0;
-#pragma unroll
-    for (int i = 0; i < GM_BLOCK / 64; i++) {
-        if (i < (int)w) pre += ws[c][i];
-        t += ws[c][i];
-    }
-    *total = t;
-    return pre + incl - v;
-}
-
 /* pass 3: append_to_fullrx_*_list / append_to_recycling_*_list
  * (SCORE_Kernel.cpp:3994-4042, :4133-4180) as ordered compaction: FullRX lists
- * hold global slots in ascending order, recycling lists are capped */
-__global__ void __launch_bounds__(GM_BLOCK) k_compact(GmDev d, uint32_t tI, uint32_t T) {
-    __shared__ uint64_t sF[GM_BLOCK], sR[GM_BLOCK];
-    __shared__ uint32_t off[4][GM_BLOCK];
-    __shared__ uint32_t ws[4][GM_BLOCK / 64];
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+ * hold global slots in ascending order (Inside, then Border, in one array),
+ * recycling lists are capped.  One wave per sweep block places that block's
+ * tiles at the block's offsets. */
+__global__ void __launch_bounds__(64) k_compact(GmDev d, uint32_t tI, uint32_t T) {
+    __shared__ uint64_t sF[64], sR[64];
+    __shared__ uint32_t so[4][64];
+    const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
     part_of(T, GM_BLOCK / 64, lo, hi);
     uint32_t run[4];
+#pragma unroll
     for (int c = 0; c < 4; c++) run[c] = d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x];
-    for (uint32_t batch = lo; batch < hi; batch += GM_BLOCK) {
-        const uint32_t t = batch + threadIdx.x;
+    run[BC_FB] += d.st->nfullI; /* Border parents follow the Inside ones */
+    for (uint32_t batch = lo; batch < hi; batch += 64) {
+        const uint32_t t = batch + lane;
         const bool valid = t < hi;
         const uint64_t bf = valid ? d.mF[t] : 0ull, br = valid ? d.mR[t] : 0ull;
         const bool isB = t >= tI;
         const uint32_t cf = (uint32_t)__popcll(bf), cr = (uint32_t)__popcll(br);
         const uint32_t v[4] = {isB ? 0u : cf, isB ? 0u : cr, isB ? cf : 0u, isB ? cr : 0u};
-        uint32_t tot[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) off[c][threadIdx.x] = run[c] + block_excl_scan(v[c], &tot[c], ws, c);
-        sF[threadIdx.x] = bf;
-        sR[threadIdx.x] = br;
+        for (int c = 0; c < 4; c++) {
+            const uint32_t incl = wave_incl_scan(v[c]);
+            so[c][lane] = run[c] + incl - v[c];
+            run[c] += __shfl(incl, 63, 64);
+        }
+        sF[lane] = bf;
+        sR[lane] = br;
         __syncthreads();
-        const uint32_t nt = hi - batch < GM_BLOCK ? hi - batch : GM_BLOCK;
-        for (uint32_t j = w; j < nt; j += GM_BLOCK / 64) {
+        const uint32_t nt = hi - batch < 64u ? hi - batch : 64u;
+        for (uint32_t j = 0; j < nt; j++) {
+            const uint64_t mf = sF[j], mr = sR[j];
+            if ((mf | mr) == 0ull) continue;
             const uint32_t tile = batch + j;
             const uint32_t s = tile_slot0(d, tile, tI) + lane;
-            const uint64_t mf = sF[j], mr = sR[j];
             const bool tb = tile >= tI;
-            if ((mf >> lane) & 1ull) {
-                const uint32_t pos = lanemask_lt_count(mf);
-                if (tb) d.fullB[off[BC_FB][j] + pos] = s;
-                else d.fullI[off[BC_FI][j] + pos] = s;
-            }
+            if ((mf >> lane) & 1ull) d.pslot[so[tb ? BC_FB : BC_FI][j] + lanemask_lt_count(mf)] = s;
             if ((mr >> lane) & 1ull) {
                 const uint32_t pos = lanemask_lt_count(mr);
                 if (tb) {
-                    const uint32_t r = off[BC_RB][j] + pos;
+                    const uint32_t r = so[BC_RB][j] + pos;
                     if (r < d.recCapB) d.recB[r] = s - d.capI;
                 } else {
-                    const uint32_t r = off[BC_RI][j] + pos;
+                    const uint32_t r = so[BC_RI][j] + pos;
                     if (r < d.recCapI) d.recI[r] = s;
                 }
             }
         }
-        for (int c = 0; c < 4; c++) run[c] += tot[c];
         __syncthreads();
     }
 }
@@ -465,36 +492,97 @@ static uint32_t grid_for(uint32_t items, uint32_t per_block) {
     return b;
 }
 
-hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s) {
+static inline hipError_t mark(hipEvent_t *ev, int i, hipStream_t s) {
+    return ev ? hipEventRecord(ev[i], s) : hipSuccess;
+}
+
+hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev) {
+    hipError_t e;
+    if ((e = mark(ev, 0, s)) != hipSuccess) return e;
     if (!a.reanchor && a.nnew) {
         const uint32_t n = a.nnew < d.newCap ? a.nnew : d.newCap;
         k_predict_new<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a, n);
     }
     const uint32_t tI = (a.nI + 63) / 64, T = tI + (a.nB + 63) / 64;
     const uint32_t nb = grid_for(T, 4 * (GM_BLOCK / 64));
+    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
     k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, a, tI, T);
+    if ((e = mark(ev, 2, s)) != hipSuccess) return e;
     k_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
-    k_compact<<<nb, GM_BLOCK, 0, s>>>(d, tI, T);
+    if ((e = mark(ev, 3, s)) != hipSuccess) return e;
+    k_compact<<<nb, 64, 0, s>>>(d, tI, T);
     return hipGetLastError();
 }
 
 /* ----------------------------------------------------------------- infection */
 
-static __device__ __forceinline__ uint32_t parent_slot(const GmDev &d, uint32_t p, uint32_t nfullI) {
-    return p < nfullI ? d.fullI[p] : d.fullB[p - nfullI];
+/* word and bit of cell (x,y,z) in the deformed-cell bitmap */
+static __device__ __forceinline__ void dmap_loc(const GmDev &d, uint32_t x, uint32_t y, uint32_t z, uint32_t &word,
+                                                uint32_t &bit) {
+    const uint32_t line = (x >> 4) + d.dnbx * ((y >> 3) + d.dnby * (z >> 3));
+    const uint32_t b = (x & 15u) | ((y & 7u) << 4) | ((z & 7u) << 7);
+    word = line * 32u + (b >> 5);
+    bit = 1u << (b & 31u);
 }
 
-/* Target of attempt rank r from the parent in global slot s at cell `pos`.
- * Inside parents (omp_infect_OutOfRXFrontInside, SCORE_Kernel.cpp:4252-4344) never
- * wrap; they send every target to the Border list unless they lie more than one
- * cell away from every face (:4606-4609).  Border parents (:4347-4493) wrap x and
- * y, drop targets outside z, and send strictly interior targets to the Inside list. */
-static __device__ __forceinline__ bool attempt_target(const GmDev &d, bool fromB, uint32_t pos, int r, uint32_t &t,
-                                                      bool &toB) {
-    uint32_t z = pos / d.nxy;
-    uint32_t rem = pos - z * d.nxy;
-    uint32_t y = rem / d.nx;
-    uint32_t x = rem - y * d.nx;
+static __device__ __forceinline__ bool is_deformed(const GmDev &d, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t w, b;
+    dmap_loc(d, x, y, z, w, b);
+    return (d.dmap[w] & b) != 0u;
+}
+
+static __device__ __forceinline__ void clear_deformed(const GmDev &d, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t w, b;
+    dmap_loc(d, x, y, z, w, b);
+    atomicAnd(&d.dmap[w], ~b);
+}
+
+/* every word of the bitmap from the cell grid (cells outside the domain stay 0) */
+__global__ void k_dmap_init(GmDev d, uint32_t nwords) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        const uint32_t line = w >> 5, sub = w & 31u;
+        const uint32_t bx = line % d.dnbx, rest = line / d.dnbx, by = rest % d.dnby, bz = rest / d.dnby;
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < 32; i++) {
+            const uint32_t b = sub * 32u + i;
+            const uint32_t x = bx * 16u + (b & 15u), y = by * 8u + ((b >> 4) & 7u), z = bz * 8u + (b >> 7);
+            if (x < d.nx && y < d.ny && z < d.nz && d.grid[x + y * d.nx + z * d.nxy] < d.ndefg) v |= 1u << i;
+        }
+        d.dmap[w] = v;
+    }
+}
+
+hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s) {
+    const uint32_t nwords = d.dnbx * d.dnby * d.dnbz * 32u;
+    k_dmap_init<<<fill_blocks(nwords), GM_BLOCK, 0, s>>>(d, nwords);
+    return hipGetLastError();
+}
+
+/* parent data in FullRX order, one thread per completed cell */
+__global__ void __launch_bounds__(GM_BLOCK) k_gather(GmDev d) {
+    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
+    for (uint32_t p = blockIdx.x * GM_BLOCK + threadIdx.x; p < nfull; p += gridDim.x * GM_BLOCK) {
+        const uint32_t s = d.pslot[p];
+        const GmRec *r = d.rec + s;
+        const double2 q0 = rec_q0(r);
+        const uint4 q2 = rec_q2(r);
+        const double fin = r->fin;
+        const uint32_t cell = q2.y;
+        const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+        GmPar *o = d.par + p;
+        *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, q2.z, q2.w);
+        *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, q0.y);
+        *reinterpret_cast<uint4 *>(&o->x) = make_uint4(x, y, z, s >= d.capI ? 1u : 0u);
+    }
+}
+
+/* Target of attempt rank r from a parent at (x,y,z).  Inside parents
+ * (omp_infect_OutOfRXFrontInside, SCORE_Kernel.cpp:4252-4344) never wrap; they send
+ * every target to the Border list unless they lie more than one cell away from
+ * every face (:4606-4609).  Border parents (:4347-4493) wrap x and y, drop
+ * targets outside z, and send strictly interior targets to the Inside list. */
+static __device__ __forceinline__ bool attempt_target(const GmDev &d, bool fromB, uint32_t x, uint32_t y, uint32_t z,
+                                                      int r, uint32_t &tx_, uint32_t &ty_, uint32_t &tz_, bool &toB) {
     int tx = (int)x + c_dx[r], ty = (int)y + c_dy[r], tz = (int)z + c_dz[r];
     const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
     if (fromB) {
@@ -510,57 +598,69 @@ static __device__ __forceinline__ bool attempt_target(const GmDev &d, bool fromB
         if (tx < 0 || tx >= nx || ty < 0 || ty >= ny || tz < 0 || tz >= nz) return false;
         toB = !((int)x > 1 && (int)x < nx - 2 && (int)y > 1 && (int)y < ny - 2 && (int)z > 1 && (int)z < nz - 2);
     }
-    t = (uint32_t)tx + (uint32_t)ty * d.nx + (uint32_t)tz * d.nxy;
+    tx_ = (uint32_t)tx;
+    ty_ = (uint32_t)ty;
+    tz_ = (uint32_t)tz;
     return true;
 }
 
-/* all attempts claim their target: the smallest key p*32+r is the serial winner */
+static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, uint32_t y, uint32_t z) {
+    return x + y * d.nx + z * d.nxy;
+}
+
+/* all attempts on deformed cells claim their target: the smallest key p*32+r
+ * is the serial winner */
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
-    const uint32_t nfullI = d.st->nfullI, nfull = nfullI + d.st->nfullB;
+    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
     const uint32_t lane = threadIdx.x & 31;
     const uint32_t ngroups = gridDim.x * (GM_BLOCK / 32);
     for (uint32_t p = blockIdx.x * (GM_BLOCK / 32) + (threadIdx.x >> 5); p < nfull; p += ngroups) {
         if (lane >= 26) continue;
-        const uint32_t s = parent_slot(d, p, nfullI);
-        uint32_t t;
+        const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+        uint32_t tx, ty, tz;
         bool toB;
-        if (!attempt_target(d, s >= d.capI, d.cell[s], (int)lane, t, toB)) continue;
-        if (d.grid[t] < d.ndefg) atomicMin(&d.claim[t], p * 32u + lane);
+        if (!attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)lane, tx, ty, tz, toB)) continue;
+        if (is_deformed(d, tx, ty, tz)) atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + lane);
     }
 }
 
 /* winners per parent and destination list, as 26-bit masks over the ranks;
- * per-block totals for the ranking scan */
+ * per-block totals for the ranking scan (and the attempts on deformed cells) */
 __global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
-    __shared__ uint32_t red[GM_BLOCK / 32][2];
-    const uint32_t nfullI = d.st->nfullI, nfull = nfullI + d.st->nfullB;
+    __shared__ uint32_t red[GM_BLOCK / 32][3];
+    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
     const uint32_t lane = threadIdx.x & 31, grp = threadIdx.x >> 5, half = threadIdx.x & 32;
     uint32_t lo, hi;
     part_of(nfull, GM_BLOCK / 32, lo, hi);
-    uint32_t cI = 0, cB = 0;
+    uint32_t cI = 0, cB = 0, cA = 0;
     for (uint32_t p = lo + grp; p < hi; p += GM_BLOCK / 32) {
-        bool wI = false, wB = false;
+        bool wI = false, wB = false, att = false;
         if (lane < 26) {
-            const uint32_t s = parent_slot(d, p, nfullI);
-            uint32_t t;
+            const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+            uint32_t tx, ty, tz;
             bool toB;
-            if (attempt_target(d, s >= d.capI, d.cell[s], (int)lane, t, toB) && d.grid[t] < d.ndefg &&
-                d.claim[t] == p * 32u + lane) {
-                wB = toB;
-                wI = !toB;
+            if (attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)lane, tx, ty, tz, toB) &&
+                is_deformed(d, tx, ty, tz)) {
+                att = true;
+                if (d.claim[cell_of(d, tx, ty, tz)] == p * 32u + lane) {
+                    wB = toB;
+                    wI = !toB;
+                }
             }
         }
         const uint32_t bI = (uint32_t)(__ballot(wI) >> half), bB = (uint32_t)(__ballot(wB) >> half);
+        const uint32_t bA = (uint32_t)(__ballot(att) >> half);
         if (lane == 0) {
             d.pmI[p] = bI;
             d.pmB[p] = bB;
             cI += (uint32_t)__popc(bI);
             cB += (uint32_t)__popc(bB);
+            cA += (uint32_t)__popc(bA);
         }
     }
-    if (lane == 0) { red[grp][0] = cI; red[grp][1] = cB; }
+    if (lane == 0) { red[grp][0] = cI; red[grp][1] = cB; red[grp][2] = cA; }
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 3) {
         uint32_t t = 0;
         for (int i = 0; i < GM_BLOCK / 32; i++) t += red[i][threadIdx.x];
         d.bcnt[threadIdx.x * GM_MAX_BLOCKS + blockIdx.x] = t;
@@ -570,12 +670,13 @@ __global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
 /* one block: offsets of every block's winners in each list, the new list lengths
  * (recycled slots first, then appended) */
 __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
-    uint32_t tot[2];
-    scan_block_counts<2>(d.bcnt, nb, tot);
+    uint32_t tot[3];
+    scan_block_counts<3>(d.bcnt, nb, tot);
     if (threadIdx.x == 0) {
         GmStats *st = d.st;
         st->winI = tot[0];
         st->winB = tot[1];
+        st->nattempt = tot[2];
         /* the commit pass reads the lengths before this step's appends */
         st->nIold = st->nI;
         st->nBold = st->nB;
@@ -584,39 +685,68 @@ __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
     }
 }
 
-/* new records as omp_infect_OutOfRXFront* write them (SCORE_Kernel.cpp:4281-4310,
+/* block-wide exclusive scan of one value per thread (GM_BLOCK threads) */
+static __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *ws) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0, t = 0;
+#pragma unroll
+    for (int i = 0; i < GM_BLOCK / 64; i++) {
+        if (i < (int)w) pre += ws[i];
+        t += ws[i];
+    }
+    *total = t;
+    return pre + incl - v;
+}
+
+/* New records as omp_infect_OutOfRXFront* write them (SCORE_Kernel.cpp:4281-4310,
  * :4402-4426), slots as omp_getNextFreeSlotIn* hand them out (:3757-3874); then
- * the block's parents retire as updateFullRX* does it (:4640-4653) */
+ * the parents retire as updateFullRX* does it (:4640-4653).  Per batch of 256
+ * parents the winners are flattened into a list so that every thread commits
+ * one infection; the slots of the new cells are listed for the next step's
+ * prediction. */
 __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint32_t k) {
     __shared__ uint32_t sI[GM_BLOCK], sB[GM_BLOCK], oI[GM_BLOCK], oB[GM_BLOCK];
-    __shared__ uint32_t ws[2][GM_BLOCK / 64];
+    __shared__ uint16_t wl[26 * GM_BLOCK];
+    __shared__ uint32_t ws[GM_BLOCK / 64];
     GmStats *st = d.st;
-    const uint32_t nfullI = st->nfullI, nfull = nfullI + st->nfullB;
+    const uint32_t nfull = st->nfullI + st->nfullB;
     const uint32_t recLenI = st->recLenI, recLenB = st->recLenB;
     const uint32_t nI = st->nIold, nB = st->nBold, winI = st->winI;
-    const uint32_t lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const uint32_t tid = threadIdx.x;
     uint32_t lo, hi;
     part_of(nfull, GM_BLOCK / 32, lo, hi);
     uint32_t runI = d.bcnt[0 * GM_MAX_BLOCKS + blockIdx.x], runB = d.bcnt[1 * GM_MAX_BLOCKS + blockIdx.x];
     for (uint32_t batch = lo; batch < hi; batch += GM_BLOCK) {
-        const uint32_t pt = batch + threadIdx.x;
-        const uint32_t mI = pt < hi ? d.pmI[pt] : 0u, mB = pt < hi ? d.pmB[pt] : 0u;
-        uint32_t totI, totB;
-        oI[threadIdx.x] = runI + block_excl_scan((uint32_t)__popc(mI), &totI, ws, 0);
-        oB[threadIdx.x] = runB + block_excl_scan((uint32_t)__popc(mB), &totB, ws, 1);
-        sI[threadIdx.x] = mI;
-        sB[threadIdx.x] = mB;
+        const uint32_t pt = batch + tid;
+        const bool valid = pt < hi;
+        const uint32_t mI = valid ? d.pmI[pt] : 0u, mB = valid ? d.pmB[pt] : 0u;
+        const uint32_t cI = (uint32_t)__popc(mI), cB = (uint32_t)__popc(mB);
+        /* one scan of both counts (each batch total < 2^16) */
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan((cI << 16) | cB, &tot, ws);
+        const uint32_t totI = tot >> 16, totB = tot & 0xFFFFu;
+        oI[tid] = runI + (ex >> 16);
+        oB[tid] = runB + (ex & 0xFFFFu);
+        sI[tid] = mI;
+        sB[tid] = mB;
+        /* winners of this parent at [exI + exB, ...) in the flat list, by rank */
+        uint32_t pos = (ex >> 16) + (ex & 0xFFFFu);
+        for (uint32_t m = mI | mB; m; m &= m - 1u) wl[pos++] = (uint16_t)(tid | ((uint32_t)__ffs(m) - 1u) << 8);
         __syncthreads();
-        const uint32_t np = hi - batch < GM_BLOCK ? hi - batch : GM_BLOCK;
-        const uint32_t bit = 1u << lane;
-        for (uint32_t j = grp; j < np; j += GM_BLOCK / 32) {
-            const uint32_t wI = sI[j], wB = sB[j];
-            if (lane >= 26 || !((wI | wB) & bit)) continue;
-            const uint32_t p = batch + j;
-            const uint32_t s = parent_slot(d, p, nfullI);
-            uint32_t t;
+        const uint32_t nw = totI + totB;
+        for (uint32_t w = tid; w < nw; w += GM_BLOCK) {
+            const uint32_t e = wl[w], j = e & 0xFFu, r = e >> 8, bit = 1u << r;
+            const GmPar *pr = d.par + batch + j;
+            const uint4 pq0 = *reinterpret_cast<const uint4 *>(&pr->slot); /* slot, cell, defg, rxg */
+            const uint4 pq2 = *reinterpret_cast<const uint4 *>(&pr->x);
+            uint32_t tx, ty, tz;
             bool toB;
-            attempt_target(d, s >= d.capI, d.cell[s], (int)lane, t, toB);
+            attempt_target(d, pq2.w != 0u, pq2.x, pq2.y, pq2.z, (int)r, tx, ty, tz, toB);
+            const uint32_t t = cell_of(d, tx, ty, tz);
+            const uint32_t wI = sI[j];
             uint32_t gslot = GM_CS_INACTIVE, nidx;
             if (wI & bit) {
                 const uint32_t rank = oI[j] + (uint32_t)__popc(wI & (bit - 1u));
@@ -625,6 +755,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint
                 else gslot = rel;
                 nidx = rank;
             } else {
+                const uint32_t wB = sB[j];
                 const uint32_t rank = oB[j] + (uint32_t)__popc(wB & (bit - 1u));
                 const uint32_t rel = rank < recLenB ? d.recB[rank] : nB + (rank - recLenB);
                 if (rel >= d.capB) atomicOr(&st->err, GM_ERRF_BORDER_FULL);
@@ -633,57 +764,76 @@ __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint
             }
             if (gslot == GM_CS_INACTIVE) continue;
             if (nidx < d.newCap) d.newl[nidx] = gslot;
-            const double overshoot = d.fin[s] - 1.0;
-            const int cls = c_cls[lane];
+            const double overshoot = pr->fin - 1.0;
+            const int cls = c_cls[r];
             const double carry = cls == 1 ? overshoot * 0.0682 : (cls == 2 ? overshoot * 0.0542 : overshoot * 0.0484);
             const uint32_t g = d.grid[t];
-            const uint32_t prxg = d.rxg[s];
-            double Pn = d.P[s];
-            if (d.defg[s] != g) Pn = pair_weight(d, pp, prxg, g);
+            double Pn = pr->P;
+            if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
+            GmRec *nr = d.rec + gslot;
+            rec_set_q0(nr, carry, Pn);
+            rec_set_q2(nr, k + 1u, t, g, pq0.w);
+            nr->inf = c_code[r];
             d.cstep[gslot] = GM_CS_UNKNOWN;
-            d.frac0[gslot] = carry;
-            d.s0[gslot] = k + 1u;
-            d.inf[gslot] = c_code[lane];
-            d.cell[gslot] = t;
-            d.P[gslot] = Pn;
-            d.defg[gslot] = g;
-            d.rxg[gslot] = prxg;
-            d.grid[t] = GM_INFECTED;
-            d.claim[t] = GM_FREE_CLAIM;
+            clear_deformed(d, tx, ty, tz);
+        }
+        /* retire the batch's parents; a deformed grain losing its last cell is
+         * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165) */
+        if (valid) {
+            const uint4 q0 = *reinterpret_cast<const uint4 *>(&d.par[pt].slot);
+            const uint32_t s = q0.x, g = q0.z, r = q0.w;
+            d.grid[q0.y] = d.ndefg + r;
+            atomicAdd(&d.rxcnt[r], 1u);
+            if (atomicSub(&d.defcnt[g], 1u) == 1u) {
+                const uint32_t i = atomicAdd(&st->nzeroed, 1u);
+                if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
+                d.zlist[i] = g;
+            }
+            d.cstep[s] = GM_CS_INACTIVE;
         }
         runI += totI;
         runB += totB;
         __syncthreads();
     }
-    /* retire this block's parents; a deformed grain losing its last cell is
-     * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165) */
-    for (uint32_t p = lo + threadIdx.x; p < hi; p += GM_BLOCK) {
-        const uint32_t s = parent_slot(d, p, nfullI);
-        const uint32_t r = d.rxg[s], g = d.defg[s];
-        d.grid[d.cell[s]] = d.ndefg + r;
-        atomicAdd(&d.rxcnt[r], 1u);
-        if (atomicSub(&d.defcnt[g], 1u) == 1u) {
-            const uint32_t i = atomicAdd(&st->nzeroed, 1u);
-            if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
-            d.zlist[i] = g;
-        }
-        d.cstep[s] = GM_CS_INACTIVE;
-    }
 }
 
 static unsigned parent_blocks(uint32_t nupper) {
     uint32_t b = (nupper + (GM_BLOCK / 32) - 1) / (GM_BLOCK / 32);
-    if (b > 4096) b = 4096;
+    if (b > 8192) b = 8192;
     if (b < 1) b = 1;
     return b;
 }
 
-hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s) {
+hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s,
+                            hipEvent_t *ev) {
+    hipError_t e;
+    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
+    k_gather<<<grid_for(parents_upper, GM_BLOCK), GM_BLOCK, 0, s>>>(d);
+    if ((e = mark(ev, 5, s)) != hipSuccess) return e;
     k_propose<<<parent_blocks(parents_upper), GM_BLOCK, 0, s>>>(d);
     const uint32_t nb = grid_for(parents_upper, 8 * (GM_BLOCK / 32));
+    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
+    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
     k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
+    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
     k_commit<<<nb, GM_BLOCK, 0, s>>>(d, pp, k);
+    if ((e = mark(ev, 9, s)) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+/* exported map: cells held by active front records are INFECTED */
+__global__ void k_overlay_infected(GmDev d, uint32_t nI, uint32_t nB, uint32_t *out) {
+    const uint32_t n = nI + nB;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = i < nI ? i : d.capI + (i - nI);
+        if (d.cstep[s] != GM_CS_INACTIVE) out[d.rec[s].cell] = GM_INFECTED;
+    }
+}
+
+hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s) {
+    if (nI + nB == 0) return hipSuccess;
+    k_overlay_infected<<<fill_blocks(nI + nB), GM_BLOCK, 0, s>>>(d, nI, nB, out);
     return hipGetLastError();
 }
 
@@ -790,10 +940,15 @@ static hipError_t scan2(const GmDev &d, uint32_t *a, uint32_t *b, uint32_t n, hi
  * (SCORE_GrowthMachine.cpp:50-189): nuclei in pool order, a site already taken
  * rejects later nuclei; interior sites append to Inside, shell sites to Border. */
 
+static __device__ __forceinline__ bool site_deformed(const GmDev &d, uint32_t w) {
+    const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+    return is_deformed(d, x, y, z);
+}
+
 __global__ void k_nuc_claim(GmDev d, const uint32_t *site, uint32_t n) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         uint32_t w = site[k];
-        if (d.grid[w] < d.ndefg) atomicMin(&d.claim[w], k);
+        if (site_deformed(d, w)) atomicMin(&d.claim[w], k);
     }
 }
 
@@ -805,7 +960,7 @@ static __device__ __forceinline__ bool site_interior(const GmDev &d, uint32_t w)
 __global__ void k_nuc_flag(GmDev d, const uint32_t *site, uint32_t n) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         uint32_t w = site[k];
-        bool win = d.grid[w] < d.ndefg && d.claim[w] == k;
+        bool win = site_deformed(d, w) && d.claim[w] == k;
         bool in = site_interior(d, w);
         d.nI_[k] = (win && in) ? 1u : 0u;
         d.nB_[k] = (win && !in) ? 1u : 0u;
@@ -828,16 +983,14 @@ __global__ void k_nuc_commit(GmDev d, const uint32_t *site, const double *nucP, 
             gslot = d.capI + rel;
         }
         uint32_t g = d.grid[w];
+        GmRec *r = d.rec + gslot;
+        rec_set_q0(r, 1.0, nucP[k]);
+        rec_set_q1(r, 0.0, 0.0);
+        rec_set_q2(r, 0u, w, g, k);
+        r->inf = 26;
         d.cstep[gslot] = GM_CS_UNKNOWN;
-        d.frac0[gslot] = 1.0;
-        d.s0[gslot] = 0u;
-        d.inf[gslot] = 26;
-        d.cell[gslot] = w;
-        d.P[gslot] = nucP[k];
-        d.defg[gslot] = g;
-        d.rxg[gslot] = k;
-        d.grid[w] = GM_INFECTED;
-        d.claim[w] = GM_FREE_CLAIM;
+        const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+        clear_deformed(d, x, y, z);
     }
 }
 
@@ -877,34 +1030,40 @@ hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_
 }
 
 /* SCORE's per-slot view before step k: activity, rxFrac (replayed from the
- * anchor; INACTIVE slots hold 0.0 as the reference leaves them), and P with
- * pending markers replaced by their cached values */
-__global__ void k_materialize(GmDev d, uint32_t k, uint32_t base, uint32_t n, uint8_t *act, double *frac) {
+ * anchor; INACTIVE slots hold 0.0 as the reference leaves them), and the record
+ * fields, with pending weights replaced by their cached values in the record
+ * itself.  With o.act null only the weights are resolved. */
+__global__ void k_materialize(GmDev d, uint32_t k, uint32_t base, uint32_t n, GmFrontOut o) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t s = base + i;
         const uint32_t cs = d.cstep[s];
-        double P = d.P[s];
-        if (is_pending(P)) {
-            P = d.hvals[(uint32_t)__double_as_longlong(P)];
-            d.P[s] = P;
-        }
-        if (!act) continue;
+        GmRec *r = d.rec + s;
+        const double2 q0 = rec_q0(r);
+        const double P = resolved(d, q0.y);
+        if (is_pending(q0.y)) rec_set_q0(r, q0.x, P);
+        if (!o.act) continue;
+        const uint4 q2 = rec_q2(r);
+        o.P[i] = P;
+        o.inf[i] = (uint8_t)r->inf;
+        o.cell[i] = q2.y;
+        o.defg[i] = q2.z;
+        o.rxg[i] = q2.w;
         if (cs == GM_CS_INACTIVE) {
-            act[i] = 0;
-            frac[i] = 0.0;
+            o.act[i] = 0;
+            o.frac[i] = 0.0;
             continue;
         }
-        act[i] = 1;
-        double f = d.frac0[s];
-        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, d.inc[s], k - d.s0[s]);
-        frac[i] = f;
+        o.act[i] = 1;
+        double f = q0.x;
+        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - q2.x);
+        o.frac[i] = f;
     }
 }
 
-hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, uint8_t *act, double *frac,
+hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, const GmFrontOut &o,
                                  hipStream_t s) {
     if (n == 0) return hipSuccess;
-    k_materialize<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, k, base, n, act, frac);
+    k_materialize<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, k, base, n, o);
     return hipGetLastError();
 }
 

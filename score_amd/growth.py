@@ -110,7 +110,9 @@ class GrowthMachine:
     def profile(self) -> dict:
         p = abi.Profile()
         abi.check(self._lib, self._lib.score_gm_get_profile(self._handle(), C.byref(p)))
-        return {k: getattr(p, k) for k, _ in abi.Profile._fields_}
+        out = {k: getattr(p, k) for k, _ in abi.Profile._fields_ if k != "ms_kernel"}
+        out["ms_kernel"] = {name: p.ms_kernel[i] for i, name in enumerate(abi.KERNELS)}
+        return out
 
     def close(self) -> None:
         if self._h is not None:

@@ -231,3 +231,22 @@ def test_voronoi512_full_size_matches_oracle_fixture(require_gpu):
     assert h(g.grainmap()) == ref["grainmap_sha256"]
     dc, rc = g.counts()
     assert h(dc) == ref["defg_counts_sha256"] and h(rc) == ref["rxg_counts_sha256"]
+
+
+def test_profile_counters(require_gpu):
+    """per-kernel event times and per-step work counters of a profiled run agree
+    with the trajectory of the same run"""
+    dom = synth.workload("voronoi100", seed=5, scale=0.5)
+    g = _gm(dom)
+    g.set_profiling(True)
+    r = g.solve_rxgrowth()
+    pf = g.profile()
+    tr = g.trajectory()
+    assert pf["steps"] == r["step"] == len(tr)
+    assert pf["sum_sv"] == r["sum_sv"] == int(tr["sv"].sum())
+    assert pf["n_full"] == int(r["xcells"]) == int(tr["nfull_inside"].sum() + tr["nfull_border"].sum())
+    assert pf["n_new"] == int(tr["ninfected"].sum())
+    assert pf["n_attempts"] >= pf["n_new"]
+    assert pf["n_reanchor_steps"] >= 1 and pf["n_predicted"] > 0
+    assert all(v >= 0.0 for v in pf["ms_kernel"].values())
+    assert pf["ms_step"] >= max(pf["ms_kernel"].values())
