@@ -46,15 +46,19 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
     record, 30 B; a valid attempt is a 4 B atomic; a new record is 30 B plus its
     4 B cell-grid write; retirement writes 4 B of grid and 8 B of counts).  Sweep
     kernels use what this design's sweep must touch: a 4 B completion step per
-    occupied slot, 16 B of masks per 64-slot tile, 49 B to predict a new cell, 69 B
-    to re-anchor one, 4 B per list entry."""
+    occupied slot, 16 B of masks per 64-slot tile, 4 B per list entry, and on the
+    steps whose scalars changed 100 B per re-anchored cell (the 64 B record read,
+    its 32 B first sector and 4 B completion step written)."""
     full, att, new = pf["n_full"], pf["n_attempts"], pf["n_new"]
     scanned, tiles = pf["slots_scanned"], (pf["slots_scanned"] + 63) // 64
     table = {
-        "k_predict_new": (49 * pf["n_predicted"], "49 B per predicted cell"),
-        "k_sweep": (4 * scanned + 16 * tiles + 69 * pf["n_reanchored"],
-                    "4 B per occupied slot + 16 B per tile + 69 B per re-anchored slot"),
+        "k_reanchor": (4 * scanned + 100 * pf["n_reanchored"],
+                       "4 B per occupied slot + 100 B per re-anchored cell (record 64 read, 32 written, step 4)"),
+        "k_predict_new": (49 * pf["n_predicted"], "49 B per predicted cell (list 4, completion step 4 read and "
+                                                  "written, record 64 read and 32 written, less what is cached)"),
+        "k_sweep": (4 * scanned + 16 * tiles, "4 B per occupied slot + 16 B per tile"),
         "k_compact": (16 * tiles + 4 * (full + pf["n_recycled"]), "16 B per tile + 4 B per list entry"),
+        "k_gather": (112 * full, "112 B per completed cell (record 64 read, parent data 48 written)"),
         "k_propose": (104 * full + 4 * att, "104 B per completed cell + 4 B per valid attempt"),
         "k_decide": (104 * full + 4 * att + 8 * full, "104 B per completed cell + 4 B per valid attempt + 8 B masks"),
         "k_commit": (34 * new + 42 * full, "34 B per new record + 42 B per completed cell (record 30, retire 12)"),
@@ -152,8 +156,9 @@ def main(argv=None) -> int:
         kms = pf["ms_kernel"]
         dom_k = max(kms, key=lambda k: kms[k])
         nbytes, units = kernel_bytes(dom_k, pf)
-        avg_us = kms[dom_k] * 1e3 / steps
-        achieved = nbytes / steps / (avg_us * 1e-6) / 1e9
+        launches = pf["n_reanchor_steps"] if dom_k == "k_reanchor" else steps
+        avg_us = kms[dom_k] * 1e3 / launches
+        achieved = nbytes / launches / (avg_us * 1e-6) / 1e9
         traffic = None
         if os.path.exists(TRAFFIC_FILE):
             tr = json.load(open(TRAFFIC_FILE))
@@ -162,13 +167,13 @@ def main(argv=None) -> int:
         # effective bandwidth of the reference algorithm's bytes per phase
         sweep_ref = 30 * pf["sum_sv"] + (pf["slots_scanned"] - pf["sum_sv"]) + 4 * (pf["n_full"] + pf["n_recycled"])
         infect_ref = (pf["n_full"] * (26 * 4 + 30) + 4 * pf["n_attempts"] + pf["n_new"] * 34 + pf["n_full"] * 12)
-        sweep_ms = sum(kms[k] for k in ("k_predict_new", "k_sweep", "k_sweep_scan", "k_compact"))
+        sweep_ms = sum(kms[k] for k in ("k_reanchor", "k_predict_new", "k_sweep", "k_sweep_scan", "k_compact"))
         infect_ms = sum(kms[k] for k in ("k_propose", "k_decide", "k_decide_scan", "k_commit"))
         roofline = {
             "bound": "hbm", "kernel": dom_k,
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "avg_launch_us": round(avg_us, 3), "alg_bytes_per_launch": int(nbytes / steps), "alg_bytes_units": units,
+            "avg_launch_us": round(avg_us, 3), "alg_bytes_per_launch": int(nbytes / launches), "alg_bytes_units": units,
             "kernel_ms_per_run": {k: round(v, 3) for k, v in kms.items()},
             "device_ms_per_run": round(pf["ms_step"], 3), "host_loop_ms_per_run": round(pf["ms_host"], 3),
             "reference_bytes_effective_GBs": {

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 2
+#define SCORE_GM_ABI_VERSION 3
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -134,16 +134,17 @@ typedef struct score_gm_result {
 } score_gm_result;
 
 /* Kernels of one growth step, in launch order (index into ms_kernel). */
-#define SCORE_GM_K_PREDICT 0     /* increments and completion steps of last step's new cells */
-#define SCORE_GM_K_SWEEP 1       /* calcGrowthStepInside/Border over the occupied slots */
-#define SCORE_GM_K_SWEEP_SCAN 2  /* list offsets of the sweep's blocks */
-#define SCORE_GM_K_COMPACT 3     /* FullRX and recycling lists */
-#define SCORE_GM_K_GATHER 4      /* parent data in FullRX order */
-#define SCORE_GM_K_PROPOSE 5     /* every infection attempt claims its target */
-#define SCORE_GM_K_DECIDE 6      /* winners per parent */
-#define SCORE_GM_K_DECIDE_SCAN 7 /* ranks of the winners' blocks */
-#define SCORE_GM_K_COMMIT 8      /* new front records, retirement of the parents */
-#define SCORE_GM_NKERNELS 9
+#define SCORE_GM_K_REANCHOR 0    /* steps whose scalars changed: every cell advanced and predicted again */
+#define SCORE_GM_K_PREDICT 1     /* other steps: increments and completion steps of last step's new cells */
+#define SCORE_GM_K_SWEEP 2       /* calcGrowthStepInside/Border over the occupied slots */
+#define SCORE_GM_K_SWEEP_SCAN 3  /* list offsets of the sweep's blocks */
+#define SCORE_GM_K_COMPACT 4     /* FullRX and recycling lists */
+#define SCORE_GM_K_GATHER 5      /* parent data in FullRX order */
+#define SCORE_GM_K_PROPOSE 6     /* every infection attempt claims its target */
+#define SCORE_GM_K_DECIDE 7      /* winners per parent */
+#define SCORE_GM_K_DECIDE_SCAN 8 /* ranks of the winners' blocks */
+#define SCORE_GM_K_COMMIT 9      /* new front records, retirement of the parents */
+#define SCORE_GM_NKERNELS 10
 
 /* Per-kernel device time and work counters, the counterpart of SCORE's per-thread
  * phase timers (omp_log_rxfrontstats, SCORE_Kernel.cpp:5019-5063; written to

@@ -527,7 +527,7 @@ int step_once(score_gm_handle *h) {
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
     if ((e = gm_launch_sweep(d, a, h->stream, ev)) != hipSuccess ||
-        (e = gm_launch_infect(d, h->pp, h->step, slots0, h->stream, ev)) != hipSuccess ||
+        (e = gm_launch_infect(d, h->pp, a, slots0, h->stream, ev)) != hipSuccess ||
         (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
@@ -559,7 +559,7 @@ int step_once(score_gm_handle *h) {
         pf.n_recycled += (uint64_t)st.recLenI + st.recLenB;
         pf.n_attempts += st.nattempt;
         pf.n_new += (uint64_t)st.winI + st.winB;
-        pf.n_predicted += a.reanchor ? 0u : npredicted;
+        pf.n_predicted += npredicted;
         if (a.reanchor) {
             pf.n_reanchor_steps++;
             pf.n_reanchored += st.sv;
@@ -738,12 +738,12 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
 
     const size_t capTot = (size_t)d.capI + d.capB;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
+    d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
     d.dnbx = (d.nx + 15) / 16;
     d.dnby = (d.ny + 7) / 8;
     d.dnbz = (d.nz + 7) / 8;
     d.tileCap = (uint32_t)std::max<uint64_t>(1024, (d.N / 16) / 64);
     d.parentCap = (uint32_t)std::max<uint64_t>(65536, d.N / 16);
-    d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         delete h;

@@ -54,21 +54,21 @@
 #define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
-#define GM_NK 9                   /* kernels per step (SCORE_GM_K_* order) */
+#define GM_NK 10                  /* kernels per step (SCORE_GM_K_* order) */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
- * anchor + increment + predicted final value.  Field pairs that kernels move
- * together share a 16-byte quarter. */
+ * anchor + increment + predicted final value.  The first 32-byte sector holds
+ * what a prediction or a re-anchoring rewrites, so those write one sector. */
 struct __attribute__((aligned(64))) GmRec {
     double frac0;   /* rxFrac before step s0 */
-    double P;       /* mobility weight, or a pending marker */
+    uint32_t s0;    /* anchor step */
+    uint32_t inf;   /* infector code 0..26 */
     double inc;     /* per-step increment, 0.0 when SCORE does not advance the cell */
     double fin;     /* rxFrac after the predicted completing addition */
-    uint32_t s0;    /* anchor step */
+    double P;       /* mobility weight, or a pending marker */
     uint32_t cell;  /* global cell id x + y*nx + z*nx*ny */
     uint32_t defg;  /* deformed grain being consumed (mydefgid) */
     uint32_t rxg;   /* growing nucleus (myrxgid) */
-    uint32_t inf;   /* infector code 0..26 */
     uint32_t pad[3];
 };
 static_assert(sizeof(GmRec) == 64, "GmRec is one 64-byte record");
@@ -166,8 +166,8 @@ struct GmSweepArgs {
 /* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
  * ev[i] is recorded before kernel i of the step and ev[GM_NK] after the last. */
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
-hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s,
-                            hipEvent_t *ev);
+hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
+                            hipStream_t s, hipEvent_t *ev);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
 /* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */

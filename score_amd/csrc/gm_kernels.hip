@@ -5,16 +5,17 @@
  * is, in serial order: sweep the Inside list, sweep the Border list, let every
  * completed Inside cell infect its 26 neighbours in list order, then every
  * completed Border cell.  A target is won by the first attempt in that order
- * that finds it deformed.  The same outcome is computed by nine kernels:
+ * that finds it deformed.  The same outcome is computed by ten kernels:
  *
- *   k_predict_new  increments and completion steps of the cells infected in the
- *                  previous step, from a dense list (every lane busy).
+ *   k_reanchor     only in steps whose scalars changed: every active cell is
+ *                  advanced to the step with its old increment and predicted
+ *                  again with the new one, one thread per slot.
+ *   k_predict_new  in the other steps: increments and completion steps of the
+ *                  cells infected in the previous step, from a dense list.
  *   k_sweep        pass over the occupied slots of both lists, statically
  *                  partitioned over the grid.  A slot costs one 4-byte read (its
- *                  predicted completion step); after a change of the per-step
- *                  scalars every cell is re-anchored and predicted again.
- *                  Completed and INACTIVE slots are balloted into per-tile masks
- *                  and counted per block.
+ *                  predicted completion step).  Completed and INACTIVE slots are
+ *                  balloted into per-tile masks and counted per block.
  *   k_sweep_scan   one block scans the per-block counts into offsets.
  *   k_compact      one wave per sweep block writes the completed slots in FullRX
  *                  order (Inside list, then Border) and the capped recycling
@@ -61,6 +62,15 @@ __constant__ uint8_t c_cls[26] = {2, 1, 2, 1, 1, 2, 1, 2, 3, 2, 3, 3, 2, 3, 2, 1
 
 static __device__ __forceinline__ uint32_t lanemask_lt_count(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* lane j's value of v (j uniform); unsigned, so the 64-bit form does not
+ * sign-extend the low word */
+static __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+}
+static __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t j) {
+    return ((uint64_t)rdlane((uint32_t)(v >> 32), j) << 32) | (uint64_t)rdlane((uint32_t)v, j);
 }
 
 static __device__ __forceinline__ bool is_pending(double v) {
@@ -179,20 +189,34 @@ static __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 
 /* ------------------------------------------------------------ front records */
 
-static __device__ __forceinline__ double2 rec_q0(const GmRec *r) { /* frac0, P */
-    return *reinterpret_cast<const double2 *>(&r->frac0);
+/* the record's 16-byte quarters: A = (frac0, s0, inf), B = (inc, fin),
+ * C = (P, cell, defg), then rxg */
+struct RecA {
+    double frac0;
+    uint32_t s0, inf;
+};
+struct RecC {
+    double P;
+    uint32_t cell, defg;
+};
+static __device__ __forceinline__ RecA rec_a(const GmRec *r) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(&r->frac0);
+    return RecA{__hiloint2double((int)v.y, (int)v.x), v.z, v.w};
 }
-static __device__ __forceinline__ uint4 rec_q2(const GmRec *r) { /* s0, cell, defg, rxg */
-    return *reinterpret_cast<const uint4 *>(&r->s0);
+static __device__ __forceinline__ RecC rec_c(const GmRec *r) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(&r->P);
+    return RecC{__hiloint2double((int)v.y, (int)v.x), v.z, v.w};
 }
-static __device__ __forceinline__ void rec_set_q0(GmRec *r, double frac0, double P) {
-    *reinterpret_cast<double2 *>(&r->frac0) = make_double2(frac0, P);
+static __device__ __forceinline__ void rec_set_a(GmRec *r, double frac0, uint32_t s0, uint32_t inf) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(frac0);
+    *reinterpret_cast<uint4 *>(&r->frac0) = make_uint4((uint32_t)b, (uint32_t)(b >> 32), s0, inf);
 }
-static __device__ __forceinline__ void rec_set_q1(GmRec *r, double inc, double fin) {
+static __device__ __forceinline__ void rec_set_b(GmRec *r, double inc, double fin) {
     *reinterpret_cast<double2 *>(&r->inc) = make_double2(inc, fin);
 }
-static __device__ __forceinline__ void rec_set_q2(GmRec *r, uint32_t s0, uint32_t cell, uint32_t defg, uint32_t rxg) {
-    *reinterpret_cast<uint4 *>(&r->s0) = make_uint4(s0, cell, defg, rxg);
+static __device__ __forceinline__ void rec_set_c(GmRec *r, double P, uint32_t cell, uint32_t defg) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(P);
+    *reinterpret_cast<uint4 *>(&r->P) = make_uint4((uint32_t)b, (uint32_t)(b >> 32), cell, defg);
 }
 
 /* a mobility weight with its pending marker replaced by the cached value */
@@ -219,13 +243,15 @@ static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSw
     return true;
 }
 
-/* Completion step of slot s from its value f before step k: the first step
- * >= k in which repeated rounded additions of the increment reach 1.0.  Stores
- * the increment (0.0 for a cell SCORE does not advance), the value after the
- * completing addition, whose excess over 1.0 the infection carries on, and the
+/* Completion step of slot s from its value f before step k (anchor step s0):
+ * the first step >= k in which repeated rounded additions of the increment
+ * reach 1.0.  Rewrites the record's first sector: the anchor, the increment
+ * (0.0 for a cell SCORE does not advance) and the value after the completing
+ * addition, whose excess over 1.0 the infection carries on; then stores the
  * completion step. */
 static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
-                                                        double P, uint32_t defg, uint32_t inf) {
+                                                        uint32_t s0, double P, uint32_t defg, uint32_t inf) {
+    const double f0 = f;
     double inc, fin = 0.0;
     uint32_t cs = GM_CS_NEVER;
     if (cell_increment(d, A.sp, P, defg, inf, inc)) {
@@ -239,7 +265,8 @@ static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const Gm
     } else {
         inc = 0.0;
     }
-    rec_set_q1(d.rec + s, inc, fin);
+    rec_set_a(d.rec + s, f0, s0, inf);
+    rec_set_b(d.rec + s, inc, fin);
     d.cstep[s] = cs;
     return cs;
 }
@@ -247,38 +274,42 @@ static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const Gm
 /* a cell infected in the previous step (anchor at step k): resolve its weight and predict */
 static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmSweepArgs &A, uint32_t s) {
     GmRec *r = d.rec + s;
-    const double2 q0 = rec_q0(r);
-    const uint4 q2 = rec_q2(r);
-    const uint32_t inf = r->inf;
-    const double P = resolved(d, q0.y);
-    if (is_pending(q0.y)) rec_set_q0(r, q0.x, P);
-    return predict_slot(d, A, s, q0.x, P, q2.z, inf);
+    const RecA a = rec_a(r);
+    const RecC c = rec_c(r);
+    const double P = resolved(d, c.P);
+    if (is_pending(c.P)) r->P = P;
+    return predict_slot(d, A, s, a.frac0, a.s0, P, c.defg, a.inf);
 }
 
 /* calcGrowthStepInside/Border (SCORE_Kernel.cpp:4046-4249) for one slot whose
  * completion step is cs: returns 1 when the cell completes in step k, 2 when the
- * slot is INACTIVE, 0 otherwise */
+ * slot is INACTIVE, 0 otherwise.  Cells still without a prediction (their weight
+ * was pending when they were infected, or they are nuclei) are predicted here. */
 static __device__ __forceinline__ int sweep_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, uint32_t cs) {
     if (cs == GM_CS_INACTIVE) return 2;
-    const uint32_t k = A.k;
-    if (A.reanchor) {
-        /* rxFrac before step k under the old increment, then a new prediction */
+    if (cs == GM_CS_UNKNOWN) cs = predict_new(d, A, s);
+    else if (cs < A.k) atomicOr(&d.st->err, GM_ERRF_MISSED);
+    return cs == A.k ? 1 : 0;
+}
+
+/* The per-step scalars changed: every active cell is advanced to the start of
+ * step k with its old increment (its anchor moves to k) and predicted again
+ * with the new one.  One thread per occupied slot. */
+__global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A) {
+    const uint32_t n = A.nI + A.nB;
+    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
+        const uint32_t s = i < A.nI ? i : d.capI + (i - A.nI);
+        const uint32_t cs = d.cstep[s];
+        if (cs == GM_CS_INACTIVE) continue;
         GmRec *r = d.rec + s;
-        const double2 q0 = rec_q0(r);
-        const uint4 q2 = rec_q2(r);
-        const uint32_t inf = r->inf;
-        double f = q0.x;
-        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - q2.x);
-        const double P = resolved(d, q0.y);
-        rec_set_q0(r, f, P);
-        r->s0 = k;
-        cs = predict_slot(d, A, s, f, P, q2.z, inf);
-    } else if (cs == GM_CS_UNKNOWN) {
-        cs = predict_new(d, A, s);
-    } else if (cs < k) {
-        atomicOr(&d.st->err, GM_ERRF_MISSED);
+        const RecA a = rec_a(r);
+        const RecC c = rec_c(r);
+        double f = a.frac0;
+        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, A.k - a.s0);
+        const double P = resolved(d, c.P);
+        if (is_pending(c.P)) r->P = P;
+        predict_slot(d, A, s, f, A.k, P, c.defg, a.inf);
     }
-    return cs == k ? 1 : 0;
 }
 
 /* cells infected in the previous step, listed by the commit pass, predicted
@@ -337,7 +368,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint
             const uint32_t s = tile_slot0(d, tile, tI) + lane;
             int r = 0;
             if (in[u]) {
-                if (!A.reanchor && cs[u] == GM_CS_UNKNOWN) unknown++;
+                if (cs[u] == GM_CS_UNKNOWN) unknown++;
                 r = sweep_slot(d, A, s, cs[u]);
             }
             const uint64_t bf = __ballot(r == 1), br = __ballot(r == 2);
@@ -498,18 +529,22 @@ static inline hipError_t mark(hipEvent_t *ev, int i, hipStream_t s) {
 
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
+    GmSweepArgs A = a;
+    A.reanchor = 0;
     if ((e = mark(ev, 0, s)) != hipSuccess) return e;
+    if (a.reanchor) k_reanchor<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
+    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
     if (!a.reanchor && a.nnew) {
         const uint32_t n = a.nnew < d.newCap ? a.nnew : d.newCap;
-        k_predict_new<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a, n);
+        k_predict_new<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, A, n);
     }
     const uint32_t tI = (a.nI + 63) / 64, T = tI + (a.nB + 63) / 64;
     const uint32_t nb = grid_for(T, 4 * (GM_BLOCK / 64));
-    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
-    k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, a, tI, T);
     if ((e = mark(ev, 2, s)) != hipSuccess) return e;
-    k_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
+    k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A, tI, T);
     if ((e = mark(ev, 3, s)) != hipSuccess) return e;
+    k_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
+    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
     k_compact<<<nb, 64, 0, s>>>(d, tI, T);
     return hipGetLastError();
 }
@@ -564,14 +599,14 @@ __global__ void __launch_bounds__(GM_BLOCK) k_gather(GmDev d) {
     for (uint32_t p = blockIdx.x * GM_BLOCK + threadIdx.x; p < nfull; p += gridDim.x * GM_BLOCK) {
         const uint32_t s = d.pslot[p];
         const GmRec *r = d.rec + s;
-        const double2 q0 = rec_q0(r);
-        const uint4 q2 = rec_q2(r);
+        const RecC c = rec_c(r);
         const double fin = r->fin;
-        const uint32_t cell = q2.y;
+        const uint32_t rxg = r->rxg;
+        const uint32_t cell = c.cell;
         const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
         GmPar *o = d.par + p;
-        *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, q2.z, q2.w);
-        *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, q0.y);
+        *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
+        *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
         *reinterpret_cast<uint4 *>(&o->x) = make_uint4(x, y, z, s >= d.capI ? 1u : 0u);
     }
 }
@@ -771,9 +806,9 @@ __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint
             double Pn = pr->P;
             if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
             GmRec *nr = d.rec + gslot;
-            rec_set_q0(nr, carry, Pn);
-            rec_set_q2(nr, k + 1u, t, g, pq0.w);
-            nr->inf = c_code[r];
+            rec_set_a(nr, carry, k + 1u, c_code[r]);
+            rec_set_c(nr, Pn, t, g);
+            nr->rxg = pq0.w;
             d.cstep[gslot] = GM_CS_UNKNOWN;
             clear_deformed(d, tx, ty, tz);
         }
@@ -804,21 +839,21 @@ static unsigned parent_blocks(uint32_t nupper) {
     return b;
 }
 
-hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, uint32_t k, uint32_t parents_upper, hipStream_t s,
-                            hipEvent_t *ev) {
+hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
+                            hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
-    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
-    k_gather<<<grid_for(parents_upper, GM_BLOCK), GM_BLOCK, 0, s>>>(d);
     if ((e = mark(ev, 5, s)) != hipSuccess) return e;
+    k_gather<<<grid_for(parents_upper, GM_BLOCK), GM_BLOCK, 0, s>>>(d);
+    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_propose<<<parent_blocks(parents_upper), GM_BLOCK, 0, s>>>(d);
     const uint32_t nb = grid_for(parents_upper, 8 * (GM_BLOCK / 32));
-    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
-    k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
     if ((e = mark(ev, 7, s)) != hipSuccess) return e;
-    k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
+    k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
     if ((e = mark(ev, 8, s)) != hipSuccess) return e;
-    k_commit<<<nb, GM_BLOCK, 0, s>>>(d, pp, k);
+    k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
     if ((e = mark(ev, 9, s)) != hipSuccess) return e;
+    k_commit<<<nb, GM_BLOCK, 0, s>>>(d, pp, a.k);
+    if ((e = mark(ev, 10, s)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -984,10 +1019,10 @@ __global__ void k_nuc_commit(GmDev d, const uint32_t *site, const double *nucP, 
         }
         uint32_t g = d.grid[w];
         GmRec *r = d.rec + gslot;
-        rec_set_q0(r, 1.0, nucP[k]);
-        rec_set_q1(r, 0.0, 0.0);
-        rec_set_q2(r, 0u, w, g, k);
-        r->inf = 26;
+        rec_set_a(r, 1.0, 0u, 26u);
+        rec_set_b(r, 0.0, 0.0);
+        rec_set_c(r, nucP[k], w, g);
+        r->rxg = k;
         d.cstep[gslot] = GM_CS_UNKNOWN;
         const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
         clear_deformed(d, x, y, z);
@@ -1038,24 +1073,24 @@ __global__ void k_materialize(GmDev d, uint32_t k, uint32_t base, uint32_t n, Gm
         const uint32_t s = base + i;
         const uint32_t cs = d.cstep[s];
         GmRec *r = d.rec + s;
-        const double2 q0 = rec_q0(r);
-        const double P = resolved(d, q0.y);
-        if (is_pending(q0.y)) rec_set_q0(r, q0.x, P);
+        const RecA a = rec_a(r);
+        const RecC c = rec_c(r);
+        const double P = resolved(d, c.P);
+        if (is_pending(c.P)) r->P = P;
         if (!o.act) continue;
-        const uint4 q2 = rec_q2(r);
         o.P[i] = P;
-        o.inf[i] = (uint8_t)r->inf;
-        o.cell[i] = q2.y;
-        o.defg[i] = q2.z;
-        o.rxg[i] = q2.w;
+        o.inf[i] = (uint8_t)a.inf;
+        o.cell[i] = c.cell;
+        o.defg[i] = c.defg;
+        o.rxg[i] = r->rxg;
         if (cs == GM_CS_INACTIVE) {
             o.act[i] = 0;
             o.frac[i] = 0.0;
             continue;
         }
         o.act[i] = 1;
-        double f = q0.x;
-        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - q2.x);
+        double f = a.frac0;
+        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - a.s0);
         o.frac[i] = f;
     }
 }

@@ -11,7 +11,9 @@
  * for odd Q.  So a run of additions within a binade is one exact multiply-add;
  * the addition that leaves the binade, and the one tie step that makes f/u
  * even, are done with the hardware addition.  One round below does both, so a
- * walk from f to 1.0 takes about one round per binade crossed.
+ * walk from f to 1.0 takes about one round per binade crossed.  All counts are
+ * integer-valued doubles below 2^53, so every step is exact and no 64-bit
+ * integer arithmetic is needed.
  *
  * Shared by host and device code; checked bit for bit against the plain loop in
  * tests/test_radd.py.
@@ -40,73 +42,67 @@ GM_HD double gm_dfrom(uint64_t b) {
     return x;
 }
 
-/* Additions of inc that provably stay inside the binade of f, as a count m and
- * the exact per-addition step `step` (a multiple of the binade's ulp).  m == 0
- * means the next addition must be done by the hardware (it leaves the binade,
- * resolves a tie, or f or inc is outside the closed form's range).  Returns
- * false when every further addition rounds back to f. */
-GM_HD bool gm_radd_run(double f, double inc, uint64_t &m, double &step) {
-    m = 0;
-    step = 0.0;
-    const uint64_t be = (gm_dbits(f) >> 52) & 0x7FFu;
-    /* zero, negative, tiny or huge values and increments that are not positive */
-    if (!(f > 0.0) || be <= 53u || be >= 0x7FEu || !(inc > 0.0)) return true;
-    const double top = gm_dfrom((be + 1u) << 52);    /* 2^(e+1) */
-    if (!((top - f) > inc)) return true;             /* the sum leaves the binade */
-    const double u = gm_dfrom((be - 52u) << 52);     /* ulp of the binade of f */
-    /* inc < 2^e here, so inc/u < 2^52; scaling by the power of two 1/u is exact */
-    const double inv_u = gm_dfrom((uint64_t)(2098u - be) << 52);
-    const double qd = __builtin_floor(inc * inv_u);
-    const uint64_t Q = (uint64_t)qd;
-    const double R = inc - qd * u;
-    const uint64_t F = (uint64_t)(f * inv_u);
-    uint64_t D;
-    if (R + R < u) D = Q;
-    else if (R + R > u) D = Q + 1;
-    else {
-        if (F & 1u) return true; /* tie with odd f/u: one hardware step makes it even */
-        D = Q + (Q & 1u);
-    }
-    if (D == 0) return false; /* the increment is absorbed by rounding */
-    /* steps whose exact sum stays below 2^(e+1): F + i*D + Q + 1 <= 2^53 - 1 */
-    const uint64_t lim = (1ull << 53) - 2u;
-    if (F + Q >= lim) return true;
-    /* m = floor(num/D); num and D are exact doubles and the rounded quotient is
-     * below 2^53, so truncating it gives m or m+1 */
-    const uint64_t num = lim - F - Q;
-    uint64_t k = (uint64_t)((double)num / (double)D);
-    if (k * D > num) k--;
-    m = k;
-    step = (double)D * u;
-    return true;
-}
+/* parity of an integer-valued double below 2^53 */
+GM_HD bool gm_odd(double v) { return v - 2.0 * __builtin_floor(v * 0.5) != 0.0; }
 
-/* One round of at most `budget` additions: the in-binade run, then one hardware
- * addition.  Returns the number done, or 0 if f can never change again.  The run
- * never reaches 2^(e+1), so a value below 1.0 can only reach 1.0 in the hardware
- * addition that ends the round. */
-GM_HD uint64_t gm_radd_round(double &f, double inc, uint64_t budget) {
-    if (budget == 0) return 0;
-    uint64_t m;
-    double step;
-    if (!gm_radd_run(f, inc, m, step)) return 0;
+/* One round of at most `budget` additions: the run that provably stays inside
+ * the binade of f, as one exact multiply-add, then one hardware addition (which
+ * leaves the binade, or makes f/u even in a tie).  Counts are integer-valued
+ * doubles (exact below 2^53).  Returns the number of additions done, or 0 if f
+ * can never change again.  The run never reaches 2^(e+1), so a value below 1.0
+ * can only reach 1.0 in the hardware addition that ends the round. */
+GM_HD double gm_radd_round(double &f, double inc, double budget) {
+    const uint64_t be = (gm_dbits(f) >> 52) & 0x7FFu;
+    /* zero, negative, tiny or huge values and increments that are not positive
+     * take plain hardware steps */
+    const bool plain = !(f > 0.0) || be <= 53u || be >= 0x7FEu || !(inc > 0.0);
+    const uint64_t bs = plain ? 1023u : be; /* keeps the bit patterns below valid */
+    const double top = gm_dfrom((bs + 1u) << 52);                /* 2^(e+1) */
+    const double u = gm_dfrom((bs - 52u) << 52);                 /* ulp of the binade */
+    const double inv_u = gm_dfrom((uint64_t)(2098u - bs) << 52); /* exact power of two */
+    /* inc < 2^e when the sum stays in the binade, so inc/u < 2^52 */
+    const bool inside = !plain && (top - f) > inc;
+    const double qd = __builtin_floor(inc * inv_u);
+    const double R = inc - qd * u;
+    const double F = f * inv_u;
+    const double twoR = R + R;
+    const bool tie = twoR == u;
+    /* D*u is the exact change of one addition inside the binade */
+    double D = twoR < u ? qd : qd + 1.0;
+    if (tie) D = qd + (gm_odd(qd) ? 1.0 : 0.0);
+    const bool tie_odd = tie && gm_odd(F); /* one hardware step makes f/u even */
+    const double lim = 9007199254740990.0; /* 2^53 - 2 */
+    const double FQ = F + qd;
+    const bool run = inside && !tie_odd && FQ < lim && D > 0.0;
+    /* additions whose exact sum stays below 2^(e+1): F + i*D + Q + 1 <= 2^53 - 1.
+     * num and D are exact and the rounded quotient lies below 2^53, so its floor
+     * is m or m+1. */
+    double m = 0.0;
+    if (run) {
+        const double num = lim - FQ;
+        m = __builtin_floor(num / D);
+        if (m * D > num) m -= 1.0;
+    }
+    if (inside && !tie_odd && D == 0.0) return 0.0; /* the increment is absorbed by rounding */
+    const double step = D * u;
     if (m >= budget) {
-        f = f + (double)budget * step;
+        f = f + budget * step;
         return budget;
     }
-    if (m) f = f + (double)m * step;
+    if (m > 0.0) f = f + m * step;
     const double g = f + inc;
-    if (m == 0 && gm_dbits(g) == gm_dbits(f)) return 0;
+    if (m == 0.0 && gm_dbits(g) == gm_dbits(f)) return 0.0;
     f = g;
-    return m + 1;
+    return m + 1.0;
 }
 
 /* f after n additions of inc */
 GM_HD double gm_radd_n(double f, double inc, uint64_t n) {
-    while (n) {
-        const uint64_t k = gm_radd_round(f, inc, n);
-        if (k == 0) break;
-        n -= k;
+    double left = (double)n;
+    while (left > 0.0) {
+        const double k = gm_radd_round(f, inc, left);
+        if (k == 0.0) break;
+        left -= k;
     }
     return f;
 }
@@ -125,12 +121,13 @@ GM_HD uint64_t gm_radd_until_one(double &f, double inc, uint64_t cap) {
         return 0; /* inc < 0: the value only decreases from here */
     }
     if (!(inc > 0.0)) return 0;
-    uint64_t done = 0;
-    while (done < cap) {
-        const uint64_t k = gm_radd_round(f, inc, cap - done);
-        if (k == 0) return 0;
+    const double capd = (double)cap;
+    double done = 0.0;
+    while (done < capd) {
+        const double k = gm_radd_round(f, inc, capd - done);
+        if (k == 0.0) return 0;
         done += k;
-        if (f >= 1.0) return done;
+        if (f >= 1.0) return (uint64_t)done;
     }
     return 0;
 }
