@@ -213,7 +213,6 @@ struct score_gm_handle {
     /* scalars of the previous sweep: a change forces re-anchoring of rxFrac */
     GmSweepParams last_sp{};
     bool have_last_sp = false;
-    uint32_t nunknown_seen = 0;
     uint32_t nnew = 0; /* cells infected in the previous step */
     /* deformed grains by descending rho and whether they still own cells */
     std::vector<double> rho_host;
@@ -392,7 +391,7 @@ uint32_t pow2_at_least(uint64_t v) {
 /* per-step lists, tile masks and per-parent masks sized for the current occupancy */
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
-    const uint32_t tiles = (h->nI + 63) / 64 + (h->nB + 63) / 64;
+    const uint32_t tiles = (((h->nI + 63) / 64 + 3u) & ~3u) + (h->nB + 63) / 64 + 4u;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
         dfree(d.mF); dfree(d.mR);
@@ -418,14 +417,16 @@ int ensure_step_capacity(score_gm_handle *h) {
     const uint32_t newmax = (uint32_t)std::min<uint64_t>(d.N, 26ull * parents);
     if (newmax > d.newCap) {
         uint32_t nc = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(newmax, d.newCap + d.newCap / 2));
-        dfree(d.newl);
-        d.newl = nullptr;
-        d.newCap = 0;
-        int rc = dalloc(&d.newl, nc);
+        uint32_t *nl = nullptr;
+        int rc = dalloc(&nl, nc);
         if (rc) return rc;
+        /* the previous step's list moves along (it is predicted in this step) */
+        const uint32_t keep = std::min(h->nnew, d.newCap);
+        if (keep) HIPCHK(hipMemcpyAsync(nl, d.newl, sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dfree(d.newl);
+        d.newl = nl;
         d.newCap = nc;
-        /* the previous step's list is gone: the sweep predicts those cells itself */
-        h->nnew = 0;
     }
     return 0;
 }
@@ -523,6 +524,9 @@ int step_once(score_gm_handle *h) {
     a.nnew = h->nnew;
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
+    /* infection keys are p*32 + rank in 32 bits: at most 2^27 completed cells per step */
+    if (slots0 >= (1u << 27))
+        return h->err = fail(SCORE_GM_ERR_UNSUPPORTED, "front larger than 2^27 slots (infection keys are 32-bit)");
     hipEvent_t *ev = prof ? h->ev : nullptr;
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
@@ -536,9 +540,8 @@ int step_once(score_gm_handle *h) {
     GmStats st = *h->hst;
     st.nfull = st.nfullI + st.nfullB;
     st.sv = slots0 - (st.totRecI + st.totRecB);
-    /* increments computed: the listed new cells, and the cells the sweep met unpredicted */
-    const uint32_t npredicted = st.nunknown - h->nunknown_seen + (a.reanchor ? 0u : std::min(a.nnew, d.newCap));
-    h->nunknown_seen = st.nunknown;
+    /* increments computed for the listed new cells (a re-anchoring step counts its cells separately) */
+    const uint32_t npredicted = a.reanchor ? 0u : std::min(a.nnew, d.newCap);
     h->nnew = st.winI + st.winB;
     if (prof) {
         score_gm_profile &pf = h->profc;
@@ -698,8 +701,10 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     d.N = (uint64_t)dom->nx * dom->ny * dom->nz;
     d.ndefg = dom->n_defg; d.nrxg = dom->n_rxg;
     /* list capacities (ompshar_init_myRXFrontInside/Border, SCORE_Kernel.cpp:3587-3710):
-     * the Inside list may grow up to N here; the Border list and both recycling lists are fixed */
-    d.capI = (uint32_t)d.N;
+     * the Inside list may grow up to N here (rounded up to whole 64-slot tiles, so
+     * that the Border slots start on a tile); the Border list and both recycling
+     * lists are fixed */
+    d.capI = (uint32_t)((d.N + 63) / 64 * 64);
     d.capB = 2 * ((d.nx * d.ny) + (d.nx * d.nz) + (d.ny * d.nz));
     d.recCapI = (uint32_t)(prm->rel_cell_caching * (double)d.N * prm->maxfill * kFullRecyclingAtt);
     d.recCapB = (uint32_t)((double)d.capB * prm->maxfill * kFullRecyclingAtt);
@@ -736,7 +741,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->pp.omega_hi = rh_omega_threshold(h->p);
     h->pp.fac = 1.0 / sqrt(2.0);
 
-    const size_t capTot = (size_t)d.capI + d.capB;
+    /* slots plus a margin for the sweep's 256-slot chunks */
+    const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 256;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
     d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
     d.dnbx = (d.nx + 15) / 16;

@@ -95,7 +95,6 @@ struct GmStats {
     uint32_t err;             /* GM_ERRF_* */
     uint32_t pone;            /* set when a pair weight was short-circuited to 1.0 */
     uint32_t scan_tot[4];     /* totals written by the device-wide scans */
-    uint32_t nunknown;        /* cells whose increment the sweep computed (running total) */
     uint32_t nattempt;        /* attempts that found a deformed target this step */
     uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */

@@ -281,17 +281,6 @@ static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmS
     return predict_slot(d, A, s, a.frac0, a.s0, P, c.defg, a.inf);
 }
 
-/* calcGrowthStepInside/Border (SCORE_Kernel.cpp:4046-4249) for one slot whose
- * completion step is cs: returns 1 when the cell completes in step k, 2 when the
- * slot is INACTIVE, 0 otherwise.  Cells still without a prediction (their weight
- * was pending when they were infected, or they are nuclei) are predicted here. */
-static __device__ __forceinline__ int sweep_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, uint32_t cs) {
-    if (cs == GM_CS_INACTIVE) return 2;
-    if (cs == GM_CS_UNKNOWN) cs = predict_new(d, A, s);
-    else if (cs < A.k) atomicOr(&d.st->err, GM_ERRF_MISSED);
-    return cs == A.k ? 1 : 0;
-}
-
 /* The per-step scalars changed: every active cell is advanced to the start of
  * step k with its old increment (its anchor moves to k) and predicted again
  * with the new one.  One thread per occupied slot. */
@@ -320,7 +309,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A
 }
 
 /* per-block list counts: [0] full Inside, [1] recycled Inside, [2] full Border,
- * [3] recycled Border, [4] increments computed; each GM_MAX_BLOCKS long */
+ * [3] recycled Border, [4] cells without a valid prediction; each GM_MAX_BLOCKS long */
 #define BC_FI 0
 #define BC_RI 1
 #define BC_FB 2
@@ -341,42 +330,67 @@ static __device__ __forceinline__ uint32_t tile_slot0(const GmDev &d, uint32_t t
     return tile < tI ? tile * 64u : d.capI + (tile - tI) * 64u;
 }
 
-/* pass 1: advance/predict every occupied slot, ballot the completed and the
- * INACTIVE ones into per-tile masks, count them per block.  Each wave loads the
- * completion steps of GM_SW_UNROLL tiles before working on them. */
+/* pass 1: every occupied slot, four per lane with one 16-byte load, 256-slot
+ * chunks of four tiles per wave; completed and INACTIVE slots are balloted into
+ * per-tile masks and counted per block.  The Inside list spans tI tiles (a
+ * multiple of four), the Border list follows from tile tI. */
 #define GM_SW_UNROLL 4
 __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint32_t tI, uint32_t T) {
     __shared__ uint32_t red[GM_BLOCK / 64][5];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t lo, hi;
-    part_of(T, GM_BLOCK / 64, lo, hi);
+    part_of(T, GM_BLOCK / 64, lo, hi); /* multiples of four tiles, except an end at T */
+    /* chunks [c_lo, c_hi); a block left empty (lo == hi == T) has none */
+    const uint32_t c_lo = lo / 4, c_hi = lo < hi ? (hi + 3) / 4 : c_lo;
     uint32_t cFI = 0, cRI = 0, cFB = 0, cRB = 0, unknown = 0;
-    for (uint32_t t0 = lo + w; t0 < hi; t0 += (GM_BLOCK / 64) * GM_SW_UNROLL) {
-        uint32_t cs[GM_SW_UNROLL];
-        bool in[GM_SW_UNROLL];
+    for (uint32_t c0 = c_lo + w; c0 < c_hi; c0 += (GM_BLOCK / 64) * GM_SW_UNROLL) {
+        uint4 cs[GM_SW_UNROLL];
+        uint32_t base[GM_SW_UNROLL], lim[GM_SW_UNROLL];
 #pragma unroll
         for (int u = 0; u < GM_SW_UNROLL; u++) {
-            const uint32_t tile = t0 + u * (GM_BLOCK / 64);
-            const uint32_t s = tile_slot0(d, tile, tI) + lane;
-            in[u] = tile < hi && s < (tile < tI ? A.nI : d.capI + A.nB);
-            cs[u] = in[u] ? d.cstep[s] : GM_CS_INACTIVE;
+            const uint32_t c = c0 + u * (GM_BLOCK / 64);
+            const uint32_t tile0 = c * 4u;
+            base[u] = tile_slot0(d, tile0, tI) + 4u * lane;
+            /* slots below lim are occupied and inside this block's tiles */
+            const uint32_t lend = tile0 < tI ? A.nI : d.capI + A.nB;
+            const uint32_t tend = tile_slot0(d, tile0, tI) + 64u * (hi - tile0 < 4u ? hi - tile0 : 4u);
+            lim[u] = c < c_hi ? (lend < tend ? lend : tend) : 0u;
+            cs[u] = base[u] < lim[u] ? *reinterpret_cast<const uint4 *>(d.cstep + base[u])
+                                     : make_uint4(GM_CS_INACTIVE, GM_CS_INACTIVE, GM_CS_INACTIVE, GM_CS_INACTIVE);
         }
 #pragma unroll
         for (int u = 0; u < GM_SW_UNROLL; u++) {
-            const uint32_t tile = t0 + u * (GM_BLOCK / 64);
-            if (tile >= hi) break;
-            const uint32_t s = tile_slot0(d, tile, tI) + lane;
-            int r = 0;
-            if (in[u]) {
-                if (cs[u] == GM_CS_UNKNOWN) unknown++;
-                r = sweep_slot(d, A, s, cs[u]);
+            const uint32_t c = c0 + u * (GM_BLOCK / 64);
+            if (c >= c_hi) break;
+            const uint32_t v[4] = {cs[u].x, cs[u].y, cs[u].z, cs[u].w};
+            /* this lane's four slots: completed bits 0-3, INACTIVE bits 4-7 */
+            uint32_t pack = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (base[u] + q < lim[u]) {
+                    const uint32_t x = v[q];
+                    if (x == GM_CS_INACTIVE) pack |= 16u << q;
+                    else if (x == A.k) pack |= 1u << q;
+                    /* every cell is predicted before the sweep (k_predict_new or
+                     * k_reanchor): a past or missing prediction is an internal error */
+                    else if (x < A.k || x == GM_CS_UNKNOWN) unknown++;
+                }
             }
-            const uint64_t bf = __ballot(r == 1), br = __ballot(r == 2);
-            if (lane == 0) {
-                d.mF[tile] = bf;
-                d.mR[tile] = br;
-                if (tile < tI) { cFI += (uint32_t)__popcll(bf); cRI += (uint32_t)__popcll(br); }
-                else { cFB += (uint32_t)__popcll(bf); cRB += (uint32_t)__popcll(br); }
+            const uint32_t tile0 = c * 4u;
+            const bool isB = tile0 >= tI;
+#pragma unroll
+            for (uint32_t t = 0; t < 4; t++) {
+                if (tile0 + t >= hi) break;
+                /* slot 64t + lane of the chunk is component lane%4 of lane 16t + lane/4 */
+                const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(16u * t + (lane >> 2)), 64);
+                const uint64_t mf = __ballot((pk >> (lane & 3u)) & 1u);
+                const uint64_t mr = __ballot((pk >> (4u + (lane & 3u))) & 1u);
+                if (lane == t) {
+                    d.mF[tile0 + t] = mf;
+                    d.mR[tile0 + t] = mr;
+                }
+                if (isB) { cFB += (uint32_t)__popcll(mf); cRB += (uint32_t)__popcll(mr); }
+                else { cFI += (uint32_t)__popcll(mf); cRI += (uint32_t)__popcll(mr); }
             }
         }
     }
@@ -443,13 +457,14 @@ static __device__ __forceinline__ void scan_block_counts(uint32_t *bcnt, uint32_
 }
 
 /* pass 2: per-block offsets and the list lengths; also resets the counters the
- * infection phase accumulates */
+ * infection phase accumulates.  Cells the sweep met without a valid prediction
+ * raise GM_ERRF_MISSED. */
 __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
     uint32_t tot[5];
     scan_block_counts<5>(d.bcnt, nb, tot);
     if (threadIdx.x == 0) {
         GmStats *st = d.st;
-        st->nunknown += tot[BC_UN];
+        if (tot[BC_UN]) st->err |= GM_ERRF_MISSED;
         st->nfullI = tot[BC_FI];
         st->totRecI = tot[BC_RI];
         st->nfullB = tot[BC_FB];
@@ -538,7 +553,8 @@ hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, 
         const uint32_t n = a.nnew < d.newCap ? a.nnew : d.newCap;
         k_predict_new<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, A, n);
     }
-    const uint32_t tI = (a.nI + 63) / 64, T = tI + (a.nB + 63) / 64;
+    /* Inside tiles rounded up to whole 256-slot chunks */
+    const uint32_t tI = ((a.nI + 63) / 64 + 3u) & ~3u, T = tI + (a.nB + 63) / 64;
     const uint32_t nb = grid_for(T, 4 * (GM_BLOCK / 64));
     if ((e = mark(ev, 2, s)) != hipSuccess) return e;
     k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A, tI, T);
@@ -643,61 +659,78 @@ static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, u
     return x + y * d.nx + z * d.nxy;
 }
 
+/* Parents are handled by groups of eight lanes; lane l of a group makes the
+ * attempts of rank l, l+8, l+16 and l+24 (< 26), so the four bitmap reads of a
+ * lane are independent and a wave keeps eight parents in flight. */
+#define GM_PG 8                       /* lanes per parent */
+#define GM_PPB (GM_BLOCK / GM_PG)     /* parents per block and round */
+
 /* all attempts on deformed cells claim their target: the smallest key p*32+r
  * is the serial winner */
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
-    const uint32_t lane = threadIdx.x & 31;
-    const uint32_t ngroups = gridDim.x * (GM_BLOCK / 32);
-    for (uint32_t p = blockIdx.x * (GM_BLOCK / 32) + (threadIdx.x >> 5); p < nfull; p += ngroups) {
-        if (lane >= 26) continue;
+    const uint32_t l8 = threadIdx.x & (GM_PG - 1);
+    const uint32_t ngroups = gridDim.x * GM_PPB;
+    for (uint32_t p = blockIdx.x * GM_PPB + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
         const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
-        uint32_t tx, ty, tz;
-        bool toB;
-        if (!attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)lane, tx, ty, tz, toB)) continue;
-        if (is_deformed(d, tx, ty, tz)) atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + lane);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t r = l8 + GM_PG * i;
+            if (r >= 26) break;
+            uint32_t tx, ty, tz;
+            bool toB;
+            if (!attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)r, tx, ty, tz, toB)) continue;
+            if (is_deformed(d, tx, ty, tz)) atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + r);
+        }
     }
 }
 
 /* winners per parent and destination list, as 26-bit masks over the ranks;
  * per-block totals for the ranking scan (and the attempts on deformed cells) */
 __global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
-    __shared__ uint32_t red[GM_BLOCK / 32][3];
+    __shared__ uint32_t red[GM_PPB][3];
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
-    const uint32_t lane = threadIdx.x & 31, grp = threadIdx.x >> 5, half = threadIdx.x & 32;
+    const uint32_t l8 = threadIdx.x & (GM_PG - 1), grp = threadIdx.x / GM_PG;
+    const uint32_t gsh = (threadIdx.x & 63) & ~(GM_PG - 1); /* first lane of the group in the wave */
     uint32_t lo, hi;
-    part_of(nfull, GM_BLOCK / 32, lo, hi);
+    part_of(nfull, GM_PPB, lo, hi);
     uint32_t cI = 0, cB = 0, cA = 0;
-    for (uint32_t p = lo + grp; p < hi; p += GM_BLOCK / 32) {
-        bool wI = false, wB = false, att = false;
-        if (lane < 26) {
-            const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
-            uint32_t tx, ty, tz;
-            bool toB;
-            if (attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)lane, tx, ty, tz, toB) &&
-                is_deformed(d, tx, ty, tz)) {
-                att = true;
-                if (d.claim[cell_of(d, tx, ty, tz)] == p * 32u + lane) {
-                    wB = toB;
-                    wI = !toB;
+    for (uint32_t p = lo + grp; p < hi; p += GM_PPB) {
+        const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+        uint32_t mI = 0, mB = 0, nA = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t r = l8 + GM_PG * i;
+            bool wI = false, wB = false;
+            if (r < 26) {
+                uint32_t tx, ty, tz;
+                bool toB;
+                if (attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)r, tx, ty, tz, toB) &&
+                    is_deformed(d, tx, ty, tz)) {
+                    nA++;
+                    if (d.claim[cell_of(d, tx, ty, tz)] == p * 32u + r) {
+                        wB = toB;
+                        wI = !toB;
+                    }
                 }
             }
+            mI |= (uint32_t)((__ballot(wI) >> gsh) & 0xFFu) << (GM_PG * i);
+            mB |= (uint32_t)((__ballot(wB) >> gsh) & 0xFFu) << (GM_PG * i);
         }
-        const uint32_t bI = (uint32_t)(__ballot(wI) >> half), bB = (uint32_t)(__ballot(wB) >> half);
-        const uint32_t bA = (uint32_t)(__ballot(att) >> half);
-        if (lane == 0) {
-            d.pmI[p] = bI;
-            d.pmB[p] = bB;
-            cI += (uint32_t)__popc(bI);
-            cB += (uint32_t)__popc(bB);
-            cA += (uint32_t)__popc(bA);
+        for (int o = GM_PG / 2; o > 0; o >>= 1) nA += __shfl_down(nA, o, GM_PG);
+        if (l8 == 0) {
+            d.pmI[p] = mI;
+            d.pmB[p] = mB;
+            cI += (uint32_t)__popc(mI);
+            cB += (uint32_t)__popc(mB);
+            cA += nA;
         }
     }
-    if (lane == 0) { red[grp][0] = cI; red[grp][1] = cB; red[grp][2] = cA; }
+    if (l8 == 0) { red[grp][0] = cI; red[grp][1] = cB; red[grp][2] = cA; }
     __syncthreads();
     if (threadIdx.x < 3) {
         uint32_t t = 0;
-        for (int i = 0; i < GM_BLOCK / 32; i++) t += red[i][threadIdx.x];
+        for (int i = 0; i < GM_PPB; i++) t += red[i][threadIdx.x];
         d.bcnt[threadIdx.x * GM_MAX_BLOCKS + blockIdx.x] = t;
     }
 }
@@ -720,59 +753,53 @@ __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
     }
 }
 
-/* block-wide exclusive scan of one value per thread (GM_BLOCK threads) */
-static __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *ws) {
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane == 63) ws[w] = incl;
-    __syncthreads();
-    uint32_t pre = 0, t = 0;
-#pragma unroll
-    for (int i = 0; i < GM_BLOCK / 64; i++) {
-        if (i < (int)w) pre += ws[i];
-        t += ws[i];
-    }
-    *total = t;
-    return pre + incl - v;
+/* One front record written whole: two full 32-byte sectors.  The increment
+ * and final value are filled in by the prediction of the next step. */
+static __device__ __forceinline__ void rec_write_new(GmRec *r, double frac0, uint32_t s0, uint32_t inf, double P,
+                                                     uint32_t cell, uint32_t defg, uint32_t rxg) {
+    rec_set_a(r, frac0, s0, inf);
+    rec_set_b(r, 0.0, 0.0);
+    rec_set_c(r, P, cell, defg);
+    *reinterpret_cast<uint4 *>(&r->rxg) = make_uint4(rxg, 0u, 0u, 0u);
 }
 
 /* New records as omp_infect_OutOfRXFront* write them (SCORE_Kernel.cpp:4281-4310,
  * :4402-4426), slots as omp_getNextFreeSlotIn* hand them out (:3757-3874); then
- * the parents retire as updateFullRX* does it (:4640-4653).  Per batch of 256
- * parents the winners are flattened into a list so that every thread commits
- * one infection; the slots of the new cells are listed for the next step's
- * prediction. */
-__global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint32_t k) {
-    __shared__ uint32_t sI[GM_BLOCK], sB[GM_BLOCK], oI[GM_BLOCK], oB[GM_BLOCK];
-    __shared__ uint16_t wl[26 * GM_BLOCK];
-    __shared__ uint32_t ws[GM_BLOCK / 64];
+ * the parents retire as updateFullRX* does it (:4640-4653).  One wave per block
+ * takes the block's parents (the partition of k_decide) 64 at a time; their
+ * winners are flattened into a list so that every lane commits one infection.
+ * The slots of the new cells are listed for the next step's prediction. */
+__global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k) {
+    __shared__ uint32_t sI[64], sB[64], oI[64], oB[64];
+    __shared__ uint16_t wl[26 * 64];
     GmStats *st = d.st;
     const uint32_t nfull = st->nfullI + st->nfullB;
     const uint32_t recLenI = st->recLenI, recLenB = st->recLenB;
     const uint32_t nI = st->nIold, nB = st->nBold, winI = st->winI;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
-    part_of(nfull, GM_BLOCK / 32, lo, hi);
+    part_of(nfull, GM_PPB, lo, hi);
     uint32_t runI = d.bcnt[0 * GM_MAX_BLOCKS + blockIdx.x], runB = d.bcnt[1 * GM_MAX_BLOCKS + blockIdx.x];
-    for (uint32_t batch = lo; batch < hi; batch += GM_BLOCK) {
-        const uint32_t pt = batch + tid;
+    for (uint32_t batch = lo; batch < hi; batch += 64) {
+        const uint32_t pt = batch + lane;
         const bool valid = pt < hi;
         const uint32_t mI = valid ? d.pmI[pt] : 0u, mB = valid ? d.pmB[pt] : 0u;
         const uint32_t cI = (uint32_t)__popc(mI), cB = (uint32_t)__popc(mB);
         /* one scan of both counts (each batch total < 2^16) */
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan((cI << 16) | cB, &tot, ws);
+        const uint32_t packed = (cI << 16) | cB;
+        const uint32_t incl = wave_incl_scan(packed);
+        const uint32_t ex = incl - packed, tot = __shfl(incl, 63, 64);
         const uint32_t totI = tot >> 16, totB = tot & 0xFFFFu;
-        oI[tid] = runI + (ex >> 16);
-        oB[tid] = runB + (ex & 0xFFFFu);
-        sI[tid] = mI;
-        sB[tid] = mB;
+        oI[lane] = runI + (ex >> 16);
+        oB[lane] = runB + (ex & 0xFFFFu);
+        sI[lane] = mI;
+        sB[lane] = mB;
         /* winners of this parent at [exI + exB, ...) in the flat list, by rank */
         uint32_t pos = (ex >> 16) + (ex & 0xFFFFu);
-        for (uint32_t m = mI | mB; m; m &= m - 1u) wl[pos++] = (uint16_t)(tid | ((uint32_t)__ffs(m) - 1u) << 8);
+        for (uint32_t m = mI | mB; m; m &= m - 1u) wl[pos++] = (uint16_t)(lane | ((uint32_t)__ffs(m) - 1u) << 8);
         __syncthreads();
         const uint32_t nw = totI + totB;
-        for (uint32_t w = tid; w < nw; w += GM_BLOCK) {
+        for (uint32_t w = lane; w < nw; w += 64) {
             const uint32_t e = wl[w], j = e & 0xFFu, r = e >> 8, bit = 1u << r;
             const GmPar *pr = d.par + batch + j;
             const uint4 pq0 = *reinterpret_cast<const uint4 *>(&pr->slot); /* slot, cell, defg, rxg */
@@ -805,10 +832,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint
             const uint32_t g = d.grid[t];
             double Pn = pr->P;
             if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
-            GmRec *nr = d.rec + gslot;
-            rec_set_a(nr, carry, k + 1u, c_code[r]);
-            rec_set_c(nr, Pn, t, g);
-            nr->rxg = pq0.w;
+            rec_write_new(d.rec + gslot, carry, k + 1u, c_code[r], Pn, t, g, pq0.w);
             d.cstep[gslot] = GM_CS_UNKNOWN;
             clear_deformed(d, tx, ty, tz);
         }
@@ -833,7 +857,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_commit(GmDev d, GmPParams pp, uint
 }
 
 static unsigned parent_blocks(uint32_t nupper) {
-    uint32_t b = (nupper + (GM_BLOCK / 32) - 1) / (GM_BLOCK / 32);
+    uint32_t b = (nupper + GM_PPB - 1) / GM_PPB;
     if (b > 8192) b = 8192;
     if (b < 1) b = 1;
     return b;
@@ -846,13 +870,13 @@ hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepAr
     k_gather<<<grid_for(parents_upper, GM_BLOCK), GM_BLOCK, 0, s>>>(d);
     if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_propose<<<parent_blocks(parents_upper), GM_BLOCK, 0, s>>>(d);
-    const uint32_t nb = grid_for(parents_upper, 8 * (GM_BLOCK / 32));
+    const uint32_t nb = grid_for(parents_upper, GM_PPB);
     if ((e = mark(ev, 7, s)) != hipSuccess) return e;
     k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
     if ((e = mark(ev, 8, s)) != hipSuccess) return e;
     k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
     if ((e = mark(ev, 9, s)) != hipSuccess) return e;
-    k_commit<<<nb, GM_BLOCK, 0, s>>>(d, pp, a.k);
+    k_commit<<<nb, 64, 0, s>>>(d, pp, a.k);
     if ((e = mark(ev, 10, s)) != hipSuccess) return e;
     return hipGetLastError();
 }
