@@ -837,18 +837,38 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k
             clear_deformed(d, tx, ty, tz);
         }
         /* retire the batch's parents; a deformed grain losing its last cell is
-         * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165) */
+         * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165).
+         * Consecutive parents often share their nucleus and deformed grain (siblings),
+         * so each run of equal ids among neighbouring lanes makes one atomic. */
+        uint4 q0 = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu);
         if (valid) {
-            const uint4 q0 = *reinterpret_cast<const uint4 *>(&d.par[pt].slot);
-            const uint32_t s = q0.x, g = q0.z, r = q0.w;
-            d.grid[q0.y] = d.ndefg + r;
-            atomicAdd(&d.rxcnt[r], 1u);
-            if (atomicSub(&d.defcnt[g], 1u) == 1u) {
-                const uint32_t i = atomicAdd(&st->nzeroed, 1u);
-                if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
-                d.zlist[i] = g;
+            q0 = *reinterpret_cast<const uint4 *>(&d.par[pt].slot);
+            d.grid[q0.y] = d.ndefg + q0.w;
+            d.cstep[q0.x] = GM_CS_INACTIVE;
+        }
+        const uint32_t g = q0.z, r = q0.w;
+        const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(r, 1, 64);
+        const uint64_t gstart = __ballot(valid && (lane == 0 || g != gprev));
+        const uint64_t rstart = __ballot(valid && (lane == 0 || r != rprev));
+        const uint64_t vmask = __ballot(valid);
+        if (valid) {
+            const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1)); /* lanes after this one */
+            if ((gstart >> lane) & 1ull) {
+                /* run length: up to the next run start or the last valid lane */
+                const uint64_t nxt = (gstart | ~vmask) & above;
+                const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+                const uint32_t n = end - lane;
+                if (atomicSub(&d.defcnt[g], n) == n) {
+                    const uint32_t i = atomicAdd(&st->nzeroed, 1u);
+                    if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
+                    d.zlist[i] = g;
+                }
             }
-            d.cstep[s] = GM_CS_INACTIVE;
+            if ((rstart >> lane) & 1ull) {
+                const uint64_t nxt = (rstart | ~vmask) & above;
+                const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+                atomicAdd(&d.rxcnt[r], end - lane);
+            }
         }
         runI += totI;
         runB += totB;
