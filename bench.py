@@ -57,9 +57,10 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
         "k_predict_new": (49 * pf["n_predicted"], "49 B per predicted cell (list 4, completion step 4 read and "
                                                   "written, record 64 read and 32 written, less what is cached)"),
         "k_sweep": (4 * scanned + 16 * tiles, "4 B per occupied slot + 16 B per tile"),
-        "k_compact": (16 * tiles + 4 * (full + pf["n_recycled"]), "16 B per tile + 4 B per list entry"),
-        "k_gather": (112 * full, "112 B per completed cell (record 64 read, parent data 48 written)"),
-        "k_propose": (104 * full + 4 * att, "104 B per completed cell + 4 B per valid attempt"),
+        "k_compact": (16 * tiles + 4 * full, "16 B per tile + 4 B per FullRX entry"),
+        "k_recycle": (4 * pf["n_recycled"], "4 B per recycled slot handed out"),
+        "k_propose": (216 * full + 4 * att, "216 B per completed cell (26 cell-grid entries 104, record 64 read, "
+                                           "parent data 48 written) + 4 B per valid attempt"),
         "k_decide": (104 * full + 4 * att + 8 * full, "104 B per completed cell + 4 B per valid attempt + 8 B masks"),
         "k_commit": (34 * new + 42 * full, "34 B per new record + 42 B per completed cell (record 30, retire 12)"),
     }

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 3
+#define SCORE_GM_ABI_VERSION 4
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -138,11 +138,11 @@ typedef struct score_gm_result {
 #define SCORE_GM_K_PREDICT 1     /* other steps: increments and completion steps of last step's new cells */
 #define SCORE_GM_K_SWEEP 2       /* calcGrowthStepInside/Border over the occupied slots */
 #define SCORE_GM_K_SWEEP_SCAN 3  /* list offsets of the sweep's blocks */
-#define SCORE_GM_K_COMPACT 4     /* FullRX and recycling lists */
-#define SCORE_GM_K_GATHER 5      /* parent data in FullRX order */
-#define SCORE_GM_K_PROPOSE 6     /* every infection attempt claims its target */
-#define SCORE_GM_K_DECIDE 7      /* winners per parent */
-#define SCORE_GM_K_DECIDE_SCAN 8 /* ranks of the winners' blocks */
+#define SCORE_GM_K_COMPACT 4     /* FullRX list */
+#define SCORE_GM_K_PROPOSE 5     /* parent data; every infection attempt claims its target */
+#define SCORE_GM_K_DECIDE 6      /* winners per parent */
+#define SCORE_GM_K_DECIDE_SCAN 7 /* ranks of the winners' blocks */
+#define SCORE_GM_K_RECYCLE 8     /* the recycled slots that are handed out */
 #define SCORE_GM_K_COMMIT 9      /* new front records, retirement of the parents */
 #define SCORE_GM_NKERNELS 10
 
@@ -159,7 +159,7 @@ typedef struct score_gm_profile {
     uint64_t sum_sv;              /* ACTIVE slots advanced (Sv) */
     uint64_t slots_scanned;       /* occupied slots swept (nextSlotNeverActive of both lists) */
     uint64_t n_full;              /* completed cells, the FullRX list lengths */
-    uint64_t n_recycled;          /* recycling-list entries written */
+    uint64_t n_recycled;          /* recycled slots handed out (recycling-list entries written) */
     uint64_t n_attempts;          /* infection attempts that found a deformed target */
     uint64_t n_new;               /* infections committed */
     uint64_t n_predicted;         /* increments computed and completion steps predicted */

@@ -559,7 +559,7 @@ int step_once(score_gm_handle *h) {
         pf.sum_sv += st.sv;
         pf.slots_scanned += slots0;
         pf.n_full += st.nfull;
-        pf.n_recycled += (uint64_t)st.recLenI + st.recLenB;
+        pf.n_recycled += (uint64_t)std::min(st.winI, st.recLenI) + std::min(st.winB, st.recLenB);
         pf.n_attempts += st.nattempt;
         pf.n_new += (uint64_t)st.winI + st.winB;
         pf.n_predicted += npredicted;
@@ -760,7 +760,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         (rc = dalloc(&d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(&d.pslot, d.parentCap)) ||
         (rc = dalloc(&d.par, d.parentCap)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
         (rc = dalloc(&d.mF, d.tileCap)) || (rc = dalloc(&d.mR, d.tileCap)) || (rc = dalloc(&d.pmI, d.parentCap)) ||
-        (rc = dalloc(&d.pmB, d.parentCap)) || (rc = dalloc(&d.bcnt, 5 * GM_MAX_BLOCKS)) ||
+        (rc = dalloc(&d.pmB, d.parentCap)) || (rc = dalloc(&d.bcnt, 8 * GM_MAX_BLOCKS)) ||
         (rc = dalloc(&d.rho, d.ndefg)) || (rc = dalloc(&d.defq, 4 * (size_t)d.ndefg)) ||
         (rc = dalloc(&d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(&d.defcnt, d.ndefg)) ||
         (rc = dalloc(&d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.zlist, d.ndefg)) ||

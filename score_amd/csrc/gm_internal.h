@@ -120,7 +120,7 @@ struct GmDev {
     uint64_t *mF, *mR;          /* per 64-slot tile: completed / INACTIVE masks */
     uint32_t *pmI, *pmB;        /* per parent: winning ranks into Inside / Border */
     /* pslot, par, pmI and pmB hold parentCap entries */
-    uint32_t *bcnt;             /* per-block counts and offsets, 5 x GM_MAX_BLOCKS */
+    uint32_t *bcnt;             /* per-block counts and offsets, 8 x GM_MAX_BLOCKS */
     uint32_t *newl;             /* global slots of the cells infected in the last step */
     uint32_t newCap;
     uint32_t tileCap, parentCap;

@@ -18,11 +18,10 @@
  *                  balloted into per-tile masks and counted per block.
  *   k_sweep_scan   one block scans the per-block counts into offsets.
  *   k_compact      one wave per sweep block writes the completed slots in FullRX
- *                  order (Inside list, then Border) and the capped recycling
- *                  lists at those offsets.
- *   k_gather       one thread per completed cell copies what the infection needs
- *                  of it (ids, rxFrac, P, coordinates) into a dense array.
- *   k_propose      every (parent p, direction rank r) attempt on a deformed cell
+ *                  order (Inside list, then Border) at those offsets.
+ *   k_propose      copies what the infection needs of each completed cell (ids,
+ *                  rxFrac, P, coordinates) into a dense parent array; then every
+ *                  (parent p, direction rank r) attempt on a deformed cell
  *                  (one bit of a brick-blocked bitmap) does
  *                  atomicMin(claim[target], p*32 + r): the key orders attempts
  *                  exactly as the serial loop issues them.
@@ -30,6 +29,8 @@
  *                  and destination list as bit masks, counted per block.
  *   k_decide_scan  one block scans those counts: rank k into a list takes the
  *                  k-th recycled slot, else appends.
+ *   k_recycle      writes the recycled slots that will be handed out: the first
+ *                  min(winners, cap) INACTIVE slots of each list, ascending.
  *   k_commit       one thread per winner writes the new record, lists it for the
  *                  next prediction and clears the cell's bitmap bit; then the
  *                  parents retire.
@@ -315,6 +316,10 @@ __global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A
 #define BC_FB 2
 #define BC_RB 3
 #define BC_UN 4
+/* rows of the infection pass: winners into Inside / Border, attempts */
+#define BC_WI 5
+#define BC_WB 6
+#define BC_WA 7
 
 /* static partition of `n` items over the grid in multiples of `q` */
 static __device__ __forceinline__ void part_of(uint32_t n, uint32_t q, uint32_t &lo, uint32_t &hi) {
@@ -477,14 +482,29 @@ __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
     }
 }
 
-/* pass 3: append_to_fullrx_*_list / append_to_recycling_*_list
- * (SCORE_Kernel.cpp:3994-4042, :4133-4180) as ordered compaction: FullRX lists
- * hold global slots in ascending order (Inside, then Border, in one array),
- * recycling lists are capped.  One wave per sweep block places that block's
- * tiles at the block's offsets. */
+/* Running offsets of the tiles [batch, batch+64) of a sweep block: lane t
+ * holds tile batch+t and gets, per list count c (BC_FI..BC_RB), the offset of
+ * its tile's first entry; run[] advances past the batch. */
+static __device__ __forceinline__ void tile_offsets(const GmDev &d, uint32_t tI, uint32_t t, bool valid, uint64_t &bf,
+                                                    uint64_t &br, uint32_t *run, uint32_t *off) {
+    bf = valid ? d.mF[t] : 0ull;
+    br = valid ? d.mR[t] : 0ull;
+    const bool isB = t >= tI;
+    const uint32_t cf = (uint32_t)__popcll(bf), cr = (uint32_t)__popcll(br);
+    const uint32_t v[4] = {isB ? 0u : cf, isB ? 0u : cr, isB ? cf : 0u, isB ? cr : 0u};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint32_t incl = wave_incl_scan(v[c]);
+        off[c] = run[c] + incl - v[c];
+        run[c] += __shfl(incl, 63, 64);
+    }
+}
+
+/* pass 3: append_to_fullrx_*_list (SCORE_Kernel.cpp:3994-4042, :4133-4180) as
+ * ordered compaction: the completed slots in ascending order, Inside then
+ * Border, in one array.  One wave per sweep block, one lane per tile: the lane
+ * writes its tile's completed slots at its offset. */
 __global__ void __launch_bounds__(64) k_compact(GmDev d, uint32_t tI, uint32_t T) {
-    __shared__ uint64_t sF[64], sR[64];
-    __shared__ uint32_t so[4][64];
     const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
     part_of(T, GM_BLOCK / 64, lo, hi);
@@ -494,40 +514,43 @@ __global__ void __launch_bounds__(64) k_compact(GmDev d, uint32_t tI, uint32_t T
     run[BC_FB] += d.st->nfullI; /* Border parents follow the Inside ones */
     for (uint32_t batch = lo; batch < hi; batch += 64) {
         const uint32_t t = batch + lane;
-        const bool valid = t < hi;
-        const uint64_t bf = valid ? d.mF[t] : 0ull, br = valid ? d.mR[t] : 0ull;
-        const bool isB = t >= tI;
-        const uint32_t cf = (uint32_t)__popcll(bf), cr = (uint32_t)__popcll(br);
-        const uint32_t v[4] = {isB ? 0u : cf, isB ? 0u : cr, isB ? cf : 0u, isB ? cr : 0u};
+        uint64_t bf, br;
+        uint32_t off[4];
+        tile_offsets(d, tI, t, t < hi, bf, br, run, off);
+        const uint32_t s0 = tile_slot0(d, t, tI);
+        uint32_t o = off[t >= tI ? BC_FB : BC_FI];
+        for (uint64_t m = bf; m; m &= m - 1ull) d.pslot[o++] = s0 + (uint32_t)__ffsll((long long)m) - 1u;
+    }
+}
+
+/* append_to_recycling_*_list (SCORE_Kernel.cpp:4037-4041, :4175-4179): the
+ * recycled slots in ascending order, of which only the first min(winners, cap)
+ * are ever handed out; written after the winners are counted, so each sweep
+ * block writes only the part of its INACTIVE slots that falls below that. */
+__global__ void __launch_bounds__(64) k_recycle(GmDev d, uint32_t tI, uint32_t T) {
+    const uint32_t lane = threadIdx.x;
+    const GmStats *st = d.st;
+    const uint32_t limI = st->winI < st->recLenI ? st->winI : st->recLenI;
+    const uint32_t limB = st->winB < st->recLenB ? st->winB : st->recLenB;
+    uint32_t run[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const uint32_t incl = wave_incl_scan(v[c]);
-            so[c][lane] = run[c] + incl - v[c];
-            run[c] += __shfl(incl, 63, 64);
+    for (int c = 0; c < 4; c++) run[c] = d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x];
+    uint32_t lo, hi;
+    part_of(T, GM_BLOCK / 64, lo, hi);
+    for (uint32_t batch = lo; batch < hi; batch += 64) {
+        if (run[BC_RI] >= limI && run[BC_RB] >= limB) return; /* nothing further is handed out */
+        const uint32_t t = batch + lane;
+        uint64_t bf, br;
+        uint32_t off[4];
+        tile_offsets(d, tI, t, t < hi, bf, br, run, off);
+        const uint32_t s0 = tile_slot0(d, t, tI);
+        if (t >= tI) {
+            uint32_t o = off[BC_RB];
+            for (uint64_t m = br; m && o < limB; m &= m - 1ull) d.recB[o++] = s0 - d.capI + (uint32_t)__ffsll((long long)m) - 1u;
+        } else {
+            uint32_t o = off[BC_RI];
+            for (uint64_t m = br; m && o < limI; m &= m - 1ull) d.recI[o++] = s0 + (uint32_t)__ffsll((long long)m) - 1u;
         }
-        sF[lane] = bf;
-        sR[lane] = br;
-        __syncthreads();
-        const uint32_t nt = hi - batch < 64u ? hi - batch : 64u;
-        for (uint32_t j = 0; j < nt; j++) {
-            const uint64_t mf = sF[j], mr = sR[j];
-            if ((mf | mr) == 0ull) continue;
-            const uint32_t tile = batch + j;
-            const uint32_t s = tile_slot0(d, tile, tI) + lane;
-            const bool tb = tile >= tI;
-            if ((mf >> lane) & 1ull) d.pslot[so[tb ? BC_FB : BC_FI][j] + lanemask_lt_count(mf)] = s;
-            if ((mr >> lane) & 1ull) {
-                const uint32_t pos = lanemask_lt_count(mr);
-                if (tb) {
-                    const uint32_t r = so[BC_RB][j] + pos;
-                    if (r < d.recCapB) d.recB[r] = s - d.capI;
-                } else {
-                    const uint32_t r = so[BC_RI][j] + pos;
-                    if (r < d.recCapI) d.recI[r] = s;
-                }
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -609,24 +632,6 @@ hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s) {
     return hipGetLastError();
 }
 
-/* parent data in FullRX order, one thread per completed cell */
-__global__ void __launch_bounds__(GM_BLOCK) k_gather(GmDev d) {
-    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
-    for (uint32_t p = blockIdx.x * GM_BLOCK + threadIdx.x; p < nfull; p += gridDim.x * GM_BLOCK) {
-        const uint32_t s = d.pslot[p];
-        const GmRec *r = d.rec + s;
-        const RecC c = rec_c(r);
-        const double fin = r->fin;
-        const uint32_t rxg = r->rxg;
-        const uint32_t cell = c.cell;
-        const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
-        GmPar *o = d.par + p;
-        *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
-        *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
-        *reinterpret_cast<uint4 *>(&o->x) = make_uint4(x, y, z, s >= d.capI ? 1u : 0u);
-    }
-}
-
 /* Target of attempt rank r from a parent at (x,y,z).  Inside parents
  * (omp_infect_OutOfRXFrontInside, SCORE_Kernel.cpp:4252-4344) never wrap; they send
  * every target to the Border list unless they lie more than one cell away from
@@ -665,14 +670,34 @@ static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, u
 #define GM_PG 8                       /* lanes per parent */
 #define GM_PPB (GM_BLOCK / GM_PG)     /* parents per block and round */
 
-/* all attempts on deformed cells claim their target: the smallest key p*32+r
- * is the serial winner */
+/* The parent data in FullRX order (lane 0 of a group copies what the infection
+ * needs of the completed cell into the dense parent array, read by the later
+ * passes), then all attempts on deformed cells claim their target: the
+ * smallest key p*32+r is the serial winner. */
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
     const uint32_t l8 = threadIdx.x & (GM_PG - 1);
     const uint32_t ngroups = gridDim.x * GM_PPB;
     for (uint32_t p = blockIdx.x * GM_PPB + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
-        const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+        uint4 q2 = make_uint4(0u, 0u, 0u, 0u);
+        if (l8 == 0) {
+            const uint32_t s = d.pslot[p];
+            const GmRec *r = d.rec + s;
+            const RecC c = rec_c(r);
+            const double fin = r->fin;
+            const uint32_t rxg = r->rxg;
+            const uint32_t cell = c.cell;
+            const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+            GmPar *o = d.par + p;
+            q2 = make_uint4(x, y, z, s >= d.capI ? 1u : 0u);
+            *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
+            *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
+            *reinterpret_cast<uint4 *>(&o->x) = q2;
+        }
+        q2.x = (uint32_t)__shfl((int)q2.x, 0, GM_PG);
+        q2.y = (uint32_t)__shfl((int)q2.y, 0, GM_PG);
+        q2.z = (uint32_t)__shfl((int)q2.z, 0, GM_PG);
+        q2.w = (uint32_t)__shfl((int)q2.w, 0, GM_PG);
 #pragma unroll
         for (uint32_t i = 0; i < 4; i++) {
             const uint32_t r = l8 + GM_PG * i;
@@ -731,7 +756,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
     if (threadIdx.x < 3) {
         uint32_t t = 0;
         for (int i = 0; i < GM_PPB; i++) t += red[i][threadIdx.x];
-        d.bcnt[threadIdx.x * GM_MAX_BLOCKS + blockIdx.x] = t;
+        d.bcnt[(BC_WI + threadIdx.x) * GM_MAX_BLOCKS + blockIdx.x] = t;
     }
 }
 
@@ -739,7 +764,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
  * (recycled slots first, then appended) */
 __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
     uint32_t tot[3];
-    scan_block_counts<3>(d.bcnt, nb, tot);
+    scan_block_counts<3>(d.bcnt + BC_WI * GM_MAX_BLOCKS, nb, tot);
     if (threadIdx.x == 0) {
         GmStats *st = d.st;
         st->winI = tot[0];
@@ -779,7 +804,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k
     const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
     part_of(nfull, GM_PPB, lo, hi);
-    uint32_t runI = d.bcnt[0 * GM_MAX_BLOCKS + blockIdx.x], runB = d.bcnt[1 * GM_MAX_BLOCKS + blockIdx.x];
+    uint32_t runI = d.bcnt[BC_WI * GM_MAX_BLOCKS + blockIdx.x], runB = d.bcnt[BC_WB * GM_MAX_BLOCKS + blockIdx.x];
     for (uint32_t batch = lo; batch < hi; batch += 64) {
         const uint32_t pt = batch + lane;
         const bool valid = pt < hi;
@@ -887,14 +912,16 @@ hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepAr
                             hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
     if ((e = mark(ev, 5, s)) != hipSuccess) return e;
-    k_gather<<<grid_for(parents_upper, GM_BLOCK), GM_BLOCK, 0, s>>>(d);
-    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_propose<<<parent_blocks(parents_upper), GM_BLOCK, 0, s>>>(d);
     const uint32_t nb = grid_for(parents_upper, GM_PPB);
-    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
-    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
     k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
+    /* the sweep's tiles and partition, for the recycling lists */
+    const uint32_t tI = ((a.nI + 63) / 64 + 3u) & ~3u, T = tI + (a.nB + 63) / 64;
+    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
+    k_recycle<<<grid_for(T, 4 * (GM_BLOCK / 64)), 64, 0, s>>>(d, tI, T);
     if ((e = mark(ev, 9, s)) != hipSuccess) return e;
     k_commit<<<nb, 64, 0, s>>>(d, pp, a.k);
     if ((e = mark(ev, 10, s)) != hipSuccess) return e;
