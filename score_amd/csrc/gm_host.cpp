@@ -207,7 +207,7 @@ struct score_gm_handle {
     score_gm_profile profc{};
     /* staging for pending P */
     std::vector<unsigned long long> pend_keys;
-    std::vector<double> pend_vals;
+    double *hpend_vals = nullptr; /* pinned */
     double *d_pend_vals = nullptr;
     uint32_t pend_vals_cap = 0;
     /* scalars of the previous sweep: a change forces re-anchoring of rxFrac */
@@ -253,6 +253,8 @@ void free_device(score_gm_handle *h) {
     h->d_pend_vals = nullptr;
     if (h->hst) (void)hipHostFree(h->hst);
     h->hst = nullptr;
+    if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
+    h->hpend_vals = nullptr;
     for (hipEvent_t &ev : h->ev)
         if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -432,34 +434,47 @@ int ensure_step_capacity(score_gm_handle *h) {
 }
 
 /* Evaluate the pending pairs with glibc and store them in the device cache. */
-int resolve_pending(score_gm_handle *h, uint32_t npend) {
+int resolve_pending(score_gm_handle *h, const GmStats &st) {
+    const uint32_t npend = st.npend;
     if (npend == 0) return 0;
     GmDev &d = h->d;
-    h->pend_keys.resize(npend);
-    h->pend_vals.resize(npend);
-    HIPCHK(hipMemcpyAsync(h->pend_keys.data(), d.pend_key, sizeof(unsigned long long) * npend, hipMemcpyDeviceToHost,
-                          h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (uint32_t i = 0; i < npend; i++) {
-        const unsigned long long key = h->pend_keys[i] - 1ull;
-        const uint32_t rxg = (uint32_t)(key / d.ndefg), g = (uint32_t)(key % d.ndefg);
-        const double P = mobility_weight(h->p, &h->rxq[4 * (size_t)rxg], &h->defq[4 * (size_t)g]);
-        h->pend_vals[i] = P;
-        if (P >= h->regPmax) h->regPmax = P;
+    /* the keys: inline in the step's stats, or copied when there are more */
+    const unsigned long long *keys = st.pend_keys;
+    if (npend > GM_PEND_INLINE) {
+        h->pend_keys.resize(npend);
+        HIPCHK(hipMemcpyAsync(h->pend_keys.data(), d.pend_key, sizeof(unsigned long long) * npend,
+                              hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        keys = h->pend_keys.data();
     }
     if (npend > h->pend_vals_cap) {
+        /* both staging buffers are idle here: the stream was synchronised after the step */
+        const uint32_t nc = std::max<uint32_t>(std::max<uint32_t>(npend, 2 * h->pend_vals_cap), 1024);
         dfree(h->d_pend_vals);
         h->d_pend_vals = nullptr;
-        uint32_t nc = std::max<uint32_t>(npend, 2 * h->pend_vals_cap);
+        if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
+        h->hpend_vals = nullptr;
+        h->pend_vals_cap = 0;
         int rc = dalloc(&h->d_pend_vals, nc);
         if (rc) return rc;
+        if (hipHostMalloc((void **)&h->hpend_vals, sizeof(double) * nc, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SCORE_GM_ERR_OOM, "pinned P staging");
+        }
         h->pend_vals_cap = nc;
     }
-    HIPCHK(hipMemcpyAsync(h->d_pend_vals, h->pend_vals.data(), sizeof(double) * npend, hipMemcpyHostToDevice,
-                          h->stream));
+    for (uint32_t i = 0; i < npend; i++) {
+        const unsigned long long key = keys[i] - 1ull;
+        const uint32_t rxg = (uint32_t)(key / d.ndefg), g = (uint32_t)(key % d.ndefg);
+        const double P = mobility_weight(h->p, &h->rxq[4 * (size_t)rxg], &h->defq[4 * (size_t)g]);
+        h->hpend_vals[i] = P;
+        if (P >= h->regPmax) h->regPmax = P;
+    }
+    /* stream-ordered: the values reach the cache before the next step's kernels
+     * read it, and the pinned buffer is not rewritten before the next step's sync */
+    HIPCHK(hipMemcpyAsync(h->d_pend_vals, h->hpend_vals, sizeof(double) * npend, hipMemcpyHostToDevice, h->stream));
     HIPCHK(gm_launch_resolve_pending(d, h->d_pend_vals, npend, h->stream));
     h->hcount += npend;
-    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -576,7 +591,7 @@ int step_once(score_gm_handle *h) {
         if (st.err & GM_ERRF_MISSED) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
         return h->err = fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
     }
-    if ((rc = resolve_pending(h, st.npend))) return h->err = rc;
+    if ((rc = resolve_pending(h, st))) return h->err = rc;
     if (st.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
     h->nI = st.nI;
     h->nB = st.nB;

@@ -54,6 +54,7 @@
 #define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
+#define GM_PEND_INLINE 64         /* new P-cache keys returned with the per-step stats */
 #define GM_NK 10                  /* kernels per step (SCORE_GM_K_* order) */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
@@ -98,6 +99,7 @@ struct GmStats {
     uint32_t nattempt;        /* attempts that found a deformed target this step */
     uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
+    unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
 };
 
 struct GmDev {

@@ -137,6 +137,7 @@ static __device__ double pair_weight(const GmDev &d, const GmPParams &pp, uint32
                 uint32_t i = atomicAdd(&d.st->npend, 1u);
                 d.pend_slot[i] = h;
                 d.pend_key[i] = key;
+                if (i < GM_PEND_INLINE) d.st->pend_keys[i] = key;
                 return pending_box(h);
             }
             k = old;
