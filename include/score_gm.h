@@ -94,7 +94,10 @@ typedef struct score_gm_params {
     int64_t nmax;                 /* NMAX */
     double rel_cell_caching;      /* InitialCacheSizeRXCells */
     double rel_cell_recaching;    /* ReCacheSizeRXCells */
-    /* on-the-fly defragmentation thresholds, compared with Xcells (SCORE_Kernel.cpp:6625-6630) */
+    /* on-the-fly defragmentation (ompshar_defragmentation, SCORE_Kernel.cpp:3877-3934):
+     * thresholds in cells, compared with Xcells at the start of a step, at most one
+     * per step (SCORE_Kernel.cpp:6625-6630).  SCORE multiplies its X log points by
+     * the box volume (SCORE_OnTheFlyCharacterization.cpp:56-57); pass those products. */
     uint32_t n_defrag;
     uint32_t reserved2;
     const double *defrag_at;

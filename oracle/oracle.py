@@ -50,6 +50,8 @@ def load() -> C.CDLL:
         lib.oracle_get_dt.restype = C.c_double
         lib.oracle_get_front.argtypes = [C.c_void_p, C.c_int, C.c_uint32, u32p, u8p, u8p, u32p, f64p, f64p, u32p,
                                          u32p]
+        lib.oracle_defragment.argtypes = [C.c_void_p]
+        lib.oracle_defragment.restype = None
         lib.oracle_capacities.argtypes = [C.c_void_p, u32p]
         lib.oracle_capacities.restype = None
         lib.oracle_disori_angle.argtypes = [f64p, f64p]
@@ -130,6 +132,10 @@ class Oracle:
 
     def dt(self) -> float:
         return self._lib.oracle_get_dt(self._h)
+
+    def defragment(self) -> None:
+        """Both lists compacted as ompshar_defragmentation does, outside the step loop."""
+        self._lib.oracle_defragment(self._h)
 
     def capacities(self) -> np.ndarray:
         out = np.zeros(4, np.uint32)

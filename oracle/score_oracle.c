@@ -745,6 +745,12 @@ int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8
     return 0;
 }
 
+/* ompshar_defragmentation of both lists outside the step loop (a test hook) */
+void oracle_defragment(oracle *o) {
+    o_defrag_list(&o->in);
+    o_defrag_list(&o->bd);
+}
+
 /* list capacities the reference derives (SCORE_Kernel.cpp:3589,3615,3631,3679,3682,3697):
  * inside, inside recycle cap, border, border recycle cap */
 void oracle_capacities(const oracle *o, uint32_t *out4) {

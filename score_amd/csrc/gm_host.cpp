@@ -214,6 +214,8 @@ struct score_gm_handle {
     GmSweepParams last_sp{};
     bool have_last_sp = false;
     uint32_t nnew = 0; /* cells infected in the previous step */
+    uint32_t defrag_cnt = 0;     /* defragmentations done (reg_defrag_cnt) */
+    bool force_reanchor = false; /* records moved: every cell is predicted again */
     /* deformed grains by descending rho and whether they still own cells */
     std::vector<double> rho_host;
     std::vector<uint32_t> rho_order;
@@ -276,7 +278,7 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
         return fail(SCORE_GM_ERR_ARG, "unknown mobility model");
     if (prm->zener_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "dispersoid drag is not implemented");
     if (prm->recovery_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "recovery is not implemented");
-    if (prm->n_defrag != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "on-the-fly defragmentation is not implemented");
+    if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
         return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");
     if (!(prm->maxfill > 0.0)) return fail(SCORE_GM_ERR_ARG, "maxfill must be positive");
@@ -517,13 +519,69 @@ int grow_pcache(score_gm_handle *h) {
     return 0;
 }
 
+/* ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934) of the list whose slot
+ * 0 is global slot `base`: capacity `cap`, nextSlotNeverActive `nnever`. */
+int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &nnever) {
+    GmDev &d = h->d;
+    hipStream_t s = h->stream;
+    /* the first INACTIVE slot; the reference's search runs on past nnever while
+     * it meets ACTIVE records left behind by an earlier defragmentation */
+    HIPCHK(gm_launch_defrag_first_inactive(d, base, 0, nnever, s));
+    HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    uint32_t first = h->hst->dfirst;
+    if (first >= nnever && nnever < cap) {
+        HIPCHK(gm_launch_defrag_first_inactive(d, base, nnever, cap, s));
+        HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        first = h->hst->dfirst;
+    }
+    if (first + 1u >= nnever) {
+        nnever = first;
+        return 0;
+    }
+    const uint32_t n = nnever - first - 1u;
+    GmRec *trec = nullptr;
+    uint32_t *tcs = nullptr;
+    int rc;
+    if ((rc = dalloc(&trec, n)) || (rc = dalloc(&tcs, n))) {
+        dfree(trec);
+        return rc;
+    }
+    hipError_t e;
+    if ((e = gm_launch_defrag_move(d, base, first, nnever, h->step, trec, tcs, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+        dfree(trec);
+        dfree(tcs);
+        return hip_fail(e, "defragmentation");
+    }
+    dfree(trec);
+    dfree(tcs);
+    nnever = first + h->hst->dcount;
+    return 0;
+}
+
 /* one iteration of SCORE's main loop (SCORE_Kernel.cpp:6615-6747) */
 int step_once(score_gm_handle *h) {
     if (h->err) return h->err;
     if (h->completed) return 1;
     if (done(h)) { h->completed = 1; return 1; }
     GmDev &d = h->d;
-    int rc = ensure_step_capacity(h);
+    int rc;
+    /* on-the-fly defragmentation at the Xcells thresholds (SCORE_Kernel.cpp:6625-6630) */
+    if (h->defrag_cnt < h->defrag_at.size() && h->Xcells >= h->defrag_at[h->defrag_cnt]) {
+        if ((rc = defragment_list(h, 0u, d.capI, h->nI)) || (rc = defragment_list(h, d.capI, d.capB, h->nB)))
+            return h->err = rc;
+        /* the device keeps the list lengths in its stats */
+        hipError_t e2;
+        if ((e2 = gm_launch_fill_u32(&d.st->nI, h->nI, 1, h->stream)) != hipSuccess ||
+            (e2 = gm_launch_fill_u32(&d.st->nB, h->nB, 1, h->stream)) != hipSuccess)
+            return h->err = hip_fail(e2, "defragmentation");
+        h->defrag_cnt++;
+        h->force_reanchor = true;
+    }
+    rc = ensure_step_capacity(h);
     if (rc) return h->err = rc;
     GmSweepArgs a;
     memset(&a, 0, sizeof(a));
@@ -532,7 +590,7 @@ int step_once(score_gm_handle *h) {
     sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
     sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
     a.k = h->step;
-    a.reanchor = (!h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
+    a.reanchor = (h->force_reanchor || !h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
     a.nI = h->nI;
     a.nB = h->nB;
     a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_CS_MAXSTEP - 1);
@@ -552,6 +610,7 @@ int step_once(score_gm_handle *h) {
         return h->err = hip_fail(e, "growth step");
     h->last_sp = sp;
     h->have_last_sp = true;
+    h->force_reanchor = false;
     GmStats st = *h->hst;
     st.nfull = st.nfullI + st.nfullB;
     st.sv = slots0 - (st.totRecI + st.totRecB);
@@ -620,6 +679,14 @@ int step_once(score_gm_handle *h) {
             ids = h->zbuf.data();
         }
         for (uint32_t i = 0; i < st.nzeroed; i++) h->alive[ids[i]] = 0;
+    }
+    if (st.cwrap) {
+        /* a count went below zero and wrapped: cellcount > 0 again, as in SCORE */
+        std::vector<uint32_t> cnt(d.ndefg);
+        if ((e = hipMemcpy(cnt.data(), d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost)) != hipSuccess)
+            return h->err = hip_fail(e, "deformed grain counts");
+        for (uint32_t g = 0; g < d.ndefg; g++) h->alive[g] = cnt[g] > 0 ? 1 : 0;
+        h->rho_top = 0;
     }
     update_rhomax(h);
     r.sv = st.sv;
@@ -703,7 +770,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->sched_T.assign(prm->sched_temp, prm->sched_temp + prm->n_sched);
     h->p.sched_time = h->sched_t.data();
     h->p.sched_temp = h->sched_T.data();
-    h->p.defrag_at = nullptr;
+    if (prm->n_defrag) h->defrag_at.assign(prm->defrag_at, prm->defrag_at + prm->n_defrag);
+    h->p.defrag_at = h->defrag_at.data();
     h->cellsize = dom->cellsize;
     h->inv_dx = (1.0 / dom->cellsize);
     /* init_processing (SCORE_Kernel.cpp:5888-5901) */

@@ -98,6 +98,9 @@ struct GmStats {
     uint32_t scan_tot[4];     /* totals written by the device-wide scans */
     uint32_t nattempt;        /* attempts that found a deformed target this step */
     uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
+    uint32_t cwrap;           /* a deformed-grain count was decremented past zero this step */
+    uint32_t dfirst;          /* defragmentation: first INACTIVE slot of the list */
+    uint32_t dcount;          /* defragmentation: ACTIVE slots moved */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
 };
@@ -180,6 +183,14 @@ struct GmFrontOut {
 hipError_t gm_launch_materialize(const GmDev &d, uint32_t k, uint32_t base, uint32_t n, const GmFrontOut &o,
                                  hipStream_t s);
 hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s);
+/* ompshar_defragmentation on the list whose slot 0 is global slot `base`:
+ * first_inactive leaves in st->dfirst the first INACTIVE slot in [lo, hi), or hi;
+ * move compacts the ACTIVE slots of (first, nnever) to [first, first+count) through
+ * the scratch records trec/tcs (nnever-first-1 entries), leaves count in st->dcount
+ * and freezes the ACTIVE records left behind at their value before step k. */
+hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s);
+hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
+                                 GmRec *trec, uint32_t *tcs, hipStream_t s);
 /* the cell grid with every cell held by an active front record marked INFECTED */
 hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -480,6 +480,7 @@ __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
         st->npend = 0;
         st->pone = 0;
         st->nzeroed = 0;
+        st->cwrap = 0;
     }
 }
 
@@ -884,10 +885,15 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k
                 const uint64_t nxt = (gstart | ~vmask) & above;
                 const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
                 const uint32_t n = end - lane;
-                if (atomicSub(&d.defcnt[g], n) == n) {
+                const uint32_t old = atomicSub(&d.defcnt[g], n);
+                if (old == n) {
                     const uint32_t i = atomicAdd(&st->nzeroed, 1u);
                     if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
                     d.zlist[i] = g;
+                } else if (old < n) {
+                    /* only a cell held by two records retires twice; the count
+                     * wraps as SCORE's uint32 does and the host re-reads them all */
+                    st->cwrap = 1u;
                 }
             }
             if ((rstart >> lane) & 1ull) {
@@ -1193,5 +1199,120 @@ hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys,
                                  hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_hash_insert<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, keys, vals, n);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------- defragmentation
+ * ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934), one list at a time: the
+ * first INACTIVE slot is found (past nextSlotNeverActive if every slot below it
+ * is ACTIVE, as the reference's unbounded search does), then every ACTIVE slot
+ * after it moves down, in order, to close the gaps.  The reference copies the
+ * records and leaves the slots behind the compacted block as they were; an
+ * ACTIVE record left there is frozen here at its value before step k (anchor k,
+ * increment 0), which is what the reference holds in that slot from then on. */
+
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_first(GmDev d, uint32_t base, uint32_t lo, uint32_t hi) {
+    const uint32_t lane = threadIdx.x & 63u, stride = gridDim.x * GM_BLOCK;
+    for (uint32_t i0 = lo + blockIdx.x * GM_BLOCK + (threadIdx.x & ~63u); i0 < hi; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        const uint64_t m = __ballot(i < hi && d.cstep[base + i] == GM_CS_INACTIVE);
+        if (m) {
+            /* later chunks of this wave only hold larger slots */
+            if (lane == 0) atomicMin(&d.st->dfirst, i0 + (uint32_t)__ffsll((long long)m) - 1u);
+            return;
+        }
+        if (hi - i0 <= stride) return;
+    }
+}
+
+hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s) {
+    k_fill_u32<<<1, 64, 0, s>>>(&d.st->dfirst, hi, 1);
+    if (hi > lo) k_defrag_first<<<fill_blocks(hi - lo), GM_BLOCK, 0, s>>>(d, base, lo, hi);
+    return hipGetLastError();
+}
+
+/* ACTIVE slots among list slots first+1+i, i in [0, n), per block chunk */
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_count(GmDev d, uint32_t base, uint32_t first, uint32_t n) {
+    uint32_t lo, hi;
+    chunk_of(n, lo, hi);
+    uint32_t c = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += GM_BLOCK)
+        c += d.cstep[base + first + 1u + i] != GM_CS_INACTIVE ? 1u : 0u;
+    uint32_t tot;
+    block_incl_scan(c, &tot);
+    if (threadIdx.x == 0) d.bcnt[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_defrag_scan(GmDev d, uint32_t nb) {
+    uint32_t tot[1];
+    scan_block_counts<1>(d.bcnt, nb, tot);
+    if (threadIdx.x == 0) d.st->dcount = tot[0];
+}
+
+/* Each ACTIVE record after `first` is copied, in slot order, to the scratch
+ * array; one that the compacted block will not overwrite is then frozen. */
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t base, uint32_t first, uint32_t n,
+                                                           uint32_t k, GmRec *trec, uint32_t *tcs) {
+    uint32_t lo, hi;
+    chunk_of(n, lo, hi);
+    uint32_t run = d.bcnt[blockIdx.x];
+    const uint32_t keep = first + d.st->dcount; /* list slots below this receive moved records */
+    for (uint32_t b0 = lo; b0 < hi; b0 += GM_BLOCK) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t c = first + 1u + i;
+        const uint32_t cs = i < hi ? d.cstep[base + c] : GM_CS_INACTIVE;
+        const uint32_t act = cs != GM_CS_INACTIVE ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t incl = block_incl_scan(act, &tot);
+        if (act) {
+            const uint32_t o = run + incl - 1u;
+            GmRec *r = d.rec + base + c;
+            const uint4 *src = reinterpret_cast<const uint4 *>(r);
+            const uint4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
+            uint4 *dst = reinterpret_cast<uint4 *>(trec + o);
+            dst[0] = q0;
+            dst[1] = q1;
+            dst[2] = q2;
+            dst[3] = q3;
+            tcs[o] = cs;
+            if (c >= keep) {
+                double f = __hiloint2double((int)q0.y, (int)q0.x);
+                const double inc = __hiloint2double((int)q1.y, (int)q1.x);
+                if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, inc, k - q0.z);
+                const double P = resolved(d, __hiloint2double((int)q2.y, (int)q2.x));
+                rec_set_a(r, f, k, q0.w);
+                rec_set_b(r, 0.0, 0.0);
+                rec_set_c(r, P, q2.z, q2.w);
+                d.cstep[base + c] = GM_CS_NEVER;
+            }
+        }
+        run += tot;
+    }
+}
+
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_scatter(GmDev d, uint32_t base, uint32_t first,
+                                                            const GmRec *trec, const uint32_t *tcs) {
+    const uint32_t n = d.st->dcount;
+    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(trec + i);
+        uint4 *dst = reinterpret_cast<uint4 *>(d.rec + base + first + i);
+        const uint4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
+        dst[0] = q0;
+        dst[1] = q1;
+        dst[2] = q2;
+        dst[3] = q3;
+        d.cstep[base + first + i] = tcs[i];
+    }
+}
+
+hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
+                                 GmRec *trec, uint32_t *tcs, hipStream_t s) {
+    if (first + 1u >= nnever) return hipSuccess;
+    const uint32_t n = nnever - first - 1u;
+    const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
+    k_defrag_count<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n);
+    k_defrag_scan<<<1, 1024, 0, s>>>(d, nb);
+    k_defrag_gather<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n, k, trec, tcs);
+    k_defrag_scatter<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, base, first, trec, tcs);
     return hipGetLastError();
 }

@@ -106,7 +106,15 @@ def test_invalid_arguments_rejected_before_device_use():
     with pytest.raises(abi.ScoreGmError) as e:
         GrowthMachine(dom)
     assert e.value.code == abi.ERR_ARG
+    # thresholds announced but not given; then a drag model the library does not have
+    lib = abi.load()
     dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
-    with pytest.raises(abi.ScoreGmError) as e:
-        GrowthMachine(dom, synth.ParamsSpec(defrag_at=(0.5,)))
-    assert e.value.code == abi.ERR_UNSUPPORTED
+    cd, keep_d = dom.to_c()
+    cp, keep_p = synth.ParamsSpec(defrag_at=(0.5,)).to_c()
+    cp.defrag_at = C.cast(None, C.POINTER(C.c_double))
+    h = C.c_void_p()
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec().to_c()
+    cp.zener_mode = 1
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_UNSUPPORTED
+    assert not h.value

@@ -135,6 +135,25 @@ def test_voronoi100_full(require_gpu):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("fracs", [(0.1, 0.2, 0.2, 0.5), (0.0, 0.0, 0.0)], ids=["thresholds", "revival"])
+def test_defragmentation_stepwise(require_gpu, fracs):
+    """On-the-fly defragmentation at Xcells thresholds (SCORE_Kernel.cpp:6625-6630).
+    Thresholds of 0 defragment in the first steps, before any cell completes, so
+    the second one takes back records left behind by the first (a cell held
+    twice, retired twice, a grain count wrapping below zero)."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=6, n_nuclei=12, seed=1)
+    prm = ParamsSpec(defrag_at=tuple(f * dom.N for f in fracs))
+    o, g = run_both(dom, prm, stepwise=True)
+    assert_final_equal(o, g)
+
+
+def test_defragmentation_voronoi100(require_gpu):
+    dom = synth.workload("voronoi100", seed=1)
+    prm = ParamsSpec(defrag_at=tuple(f * dom.N for f in (0.05, 0.2, 0.4, 0.6, 0.8, 0.9, 0.95)))
+    o, g = run_both(dom, prm)
+    assert_final_equal(o, g)
+
+
 def test_single_grain_many_nuclei_xmax1(require_gpu):
     """one deformed grain, nuclei on the faces and edges, run to X = 1 (all cells)"""
     dom = synth.make_voronoi_domain(40, 40, 40, n_grains=1, n_nuclei=60, seed=7, log10_rho=(15.0, 15.0))

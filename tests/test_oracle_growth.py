@@ -295,3 +295,49 @@ def test_border_overflow_is_an_error():
     with pytest.raises(OracleError) as e:
         o.solve_rxgrowth()
     assert e.value.code == abi.ERR_CAPACITY
+
+
+def _records(f, sel):
+    return list(zip(f["cell"][sel].tolist(), f["defg"][sel].tolist(), f["rxg"][sel].tolist(),
+                    f["infector"][sel].tolist(), f["rxfrac"][sel].view(np.uint64).tolist(),
+                    f["P"][sel].view(np.uint64).tolist()))
+
+
+def test_defragmentation_is_a_stable_compaction():
+    """ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934) keeps the slots before
+    the first INACTIVE one, moves every later ACTIVE record down in slot order and
+    sets nextSlotNeverActive to the end of the compacted block."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=6, n_nuclei=12, seed=1)
+    o = Oracle(dom, ParamsSpec())
+    while o.result()["X"] < 0.3:
+        assert not o.step(1)
+    before = {lst: o.front(lst) for lst in (abi.LIST_INSIDE, abi.LIST_BORDER)}
+    o.defragment()
+    for lst, b in before.items():
+        a = o.front(lst)
+        act = b["activity"].astype(bool)
+        assert not act.all(), "the case needs INACTIVE slots to close"
+        first = int(np.argmin(act))
+        assert len(a["activity"]) == first + int(act[first + 1:].sum())
+        assert a["activity"].all()
+        assert _records(a, slice(None)) == _records(b, act)
+        assert _records(a, slice(0, first)) == _records(b, slice(0, first))
+    o.close()
+
+
+def test_defragmentation_revives_records_left_behind():
+    """The reference's search for the first INACTIVE slot is unbounded
+    (SCORE_Kernel.cpp:3888-3891) and the slots behind the compacted block keep
+    their records.  After a defragmentation in a step without completed cells the
+    next one finds no INACTIVE slot below nextSlotNeverActive, runs on into the
+    records left behind and takes them back: those cells are then held twice,
+    retire twice, and a deformed grain's uint32 count wraps below zero."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=6, n_nuclei=12, seed=1)
+    o = Oracle(dom, ParamsSpec(defrag_at=(0.0, 0.0, 0.0)))
+    r = o.solve_rxgrowth()
+    g = o.grainmap(abi.MAP_CELLGRID)
+    rx_cells = int(((g >= dom.n_defg) & (g < abi.CURRENTLY_INFECTED)).sum())
+    dc, rc = o.counts()
+    assert int(rc.sum()) == r["xcells"] > rx_cells
+    assert int(dc.max()) > dom.N
+    o.close()
