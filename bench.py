@@ -16,6 +16,7 @@ workload on the host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for PMC scaling
+    python bench.py --batch 512 --inflight 8 # ensemble of independent domains, domains/hour
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -80,6 +81,62 @@ def copy_calibration() -> int:
     return 0
 
 
+def batch_bench(a) -> int:
+    """Ensemble of independent domains (the rank's loop over its domains in
+    SCORE's SIMULATE_myCAs): edges U{64..192}, edge^3/2000 grains, nucleus density
+    log-uniform in [1e-5, 1e-3] per voxel, T uniform in [900, 1100] K, each run to
+    X = 0.99.  Timed once with one domain at a time and once with `--inflight`
+    domains in flight; both must give the same results."""
+    import numpy as np
+    from score_amd import synth
+    from score_amd.growth import batch_run, device_count
+
+    if device_count() < 1:
+        print("no gfx950 device visible", file=sys.stderr)
+        return 3
+    rng = np.random.default_rng(a.seed)
+    t0 = time.time()
+    doms, prms = [], []
+    for i in range(a.batch):
+        e = int(rng.integers(64, 193))
+        dens = 10.0 ** rng.uniform(-5.0, -3.0)
+        T = float(rng.uniform(900.0, 1100.0))
+        doms.append(synth.make_voronoi_domain(e, e, e, n_grains=max(1, e ** 3 // 2000),
+                                              n_nuclei=max(1, int(round(dens * e ** 3))), seed=a.seed * 100003 + i))
+        prms.append(synth.ParamsSpec(sched_temp=(T, T)))
+    voxels = sum(d.N for d in doms)
+    log(f"batch of {a.batch} domains, {voxels} voxels, generated in {time.time() - t0:.1f}s")
+    batch_run(doms[:2], prms[:2], max_inflight=2)  # warm-up
+    timings = {}
+    outs = {}
+    for k in (1, a.inflight):
+        t1 = time.perf_counter()
+        res = batch_run(doms, prms, max_inflight=k)
+        timings[k] = time.perf_counter() - t1
+        outs[k] = res
+        log(f"max_inflight {k}: {timings[k]:.2f}s, {a.batch / timings[k] * 3600:.0f} domains/hour")
+    same = all(r1["result"]["xcells"] == r2["result"]["xcells"] and r1["result"]["t"] == r2["result"]["t"]
+               and np.array_equal(r1["defg_counts"], r2["defg_counts"])
+               for r1, r2 in zip(outs[1], outs[a.inflight]))
+    if not same:
+        raise RuntimeError("batched results differ from one-at-a-time results")
+    secs = np.array([r["seconds"] for r in outs[a.inflight]])
+    sv = sum(r["result"]["sum_sv"] for r in outs[a.inflight])
+    out = {
+        "metric": "domains/hour", "value": a.batch / timings[a.inflight] * 3600.0, "unit": "domains/hour",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "one_at_a_time_domains_per_hour": a.batch / timings[1] * 3600.0,
+        "speedup_vs_one_at_a_time": timings[1] / timings[a.inflight],
+        "cell_updates_per_s": sv / timings[a.inflight],
+        "domain_seconds_p50": float(np.percentile(secs, 50)), "domain_seconds_p99": float(np.percentile(secs, 99)),
+        "config": {"workload": f"ensemble of {a.batch} Voronoi domains, edges U{{64..192}}, edge^3/2000 grains, "
+                               f"nuclei log-U[1e-5,1e-3]/voxel, T~U[900,1100] K, to X=0.99",
+                   "voxels": voxels, "max_inflight": a.inflight, "seed": a.seed},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +149,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the profiled run (no roofline)")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="ensemble mode: this many independent domains, domains/hour batched vs one at a time")
+    ap.add_argument("--inflight", type=int, default=8, help="domains in flight in ensemble mode")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -100,6 +160,8 @@ def main(argv=None) -> int:
         return 2
     if a.copy_calibration:
         return copy_calibration()
+    if a.batch:
+        return batch_bench(a)
 
     from score_amd import abi, synth
     from score_amd.growth import GrowthMachine, device_count, same_trajectory

@@ -221,6 +221,39 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
                        uint8_t *activity, uint8_t *infector, uint32_t *cell,
                        double *rxfrac, double *P, uint32_t *defg, uint32_t *rxg);
 
+/* Grain assignment of the cells on the front, as
+ * caHdl::ompcrit_clarify_status_infectedcells(rxthreshold) collects it
+ * (SCORE_Io_Characterization.cpp:925-997): for every ACTIVE record, Inside list
+ * then Border list in slot order, the linear cell index and the grain id the
+ * cell shows (n_defg + nucleus id once rxFrac >= rxthreshold, else its deformed
+ * grain id).  SCORE calls it with RXFRACTION_THRESHOLD = 0.5.  Sets *n to the
+ * number of entries and copies min(cap, *n) of them; either array may be NULL. */
+int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n,
+                        uint32_t *ixyz, uint32_t *gid);
+
+/* One domain of a batch: what ensembleHdl::SIMULATE_myCAs (SCORE_Kernel.cpp:1693-1757)
+ * hands to solve_RXGROWTH for each caHdl of the rank, and what it reads back. */
+typedef struct score_gm_batch_item {
+    const score_gm_domain *domain; /* in */
+    const score_gm_params *params; /* in */
+    uint32_t *defg_counts;         /* out: n_defg entries, or NULL */
+    uint32_t *rxg_counts;          /* out: n_rxg entries, or NULL */
+    uint32_t *grainmap;            /* out: nx*ny*nz entries, or NULL */
+    int32_t map_encoding;          /* SCORE_GM_MAP_* of grainmap */
+    int32_t status;                /* out: SCORE_GM_OK or this domain's error */
+    score_gm_result result;        /* out */
+    double seconds;                /* out: wall time of this domain, create to destroy */
+} score_gm_batch_item;
+
+/* Runs every domain of `items` to completion on one device, as the rank's loop
+ * over its domains does (SCORE_Kernel.cpp:1693-1757), with up to max_inflight
+ * domains (0 selects 8) in flight at once, each on its own handle, stream and
+ * host thread, taken in index order.  A domain whose device memory does not fit
+ * next to the ones in flight waits for one of them to finish.  Returns
+ * SCORE_GM_OK when every item succeeded, else the status of the lowest failed
+ * item; every item's own status is set either way. */
+int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, uint32_t max_inflight);
+
 /* Switch per-phase event timing on (1) or off (0); switching on clears the
  * counters.  Replaces SCORE's OutputThreadProfiling switch (SCORE_Kernel.cpp:6677-6683). */
 int score_gm_set_profiling(score_gm_handle *h, int32_t enable);

@@ -50,6 +50,8 @@ def load() -> C.CDLL:
         lib.oracle_get_dt.restype = C.c_double
         lib.oracle_get_front.argtypes = [C.c_void_p, C.c_int, C.c_uint32, u32p, u8p, u8p, u32p, f64p, f64p, u32p,
                                          u32p]
+        lib.oracle_get_status.argtypes = [C.c_void_p, C.c_double, C.c_uint32, u32p, u32p]
+        lib.oracle_get_status.restype = C.c_uint32
         lib.oracle_defragment.argtypes = [C.c_void_p]
         lib.oracle_defragment.restype = None
         lib.oracle_capacities.argtypes = [C.c_void_p, u32p]
@@ -132,6 +134,13 @@ class Oracle:
 
     def dt(self) -> float:
         return self._lib.oracle_get_dt(self._h)
+
+    def status(self, rxthreshold: float = 0.5):
+        n = self._lib.oracle_get_status(self._h, rxthreshold, 0, None, None)
+        ixyz, gid = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+        self._lib.oracle_get_status(self._h, rxthreshold, n, p(ixyz), p(gid))
+        return ixyz, gid
 
     def defragment(self) -> None:
         """Both lists compacted as ompshar_defragmentation does, outside the step loop."""

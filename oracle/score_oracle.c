@@ -745,6 +745,26 @@ int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8
     return 0;
 }
 
+/* ompcrit_clarify_status_infectedcells (SCORE_Io_Characterization.cpp:925-997):
+ * ACTIVE records of the Inside then the Border list, in slot order; returns the count */
+uint32_t oracle_get_status(const oracle *o, double thr, uint32_t cap, uint32_t *ixyz, uint32_t *gid) {
+    uint32_t n = 0;
+    const o_list *lists[2] = {&o->in, &o->bd};
+    for (int l = 0; l < 2; l++) {
+        const o_list *L = lists[l];
+        for (uint32_t c = 0; c < L->nnever; c++) {
+            const o_cell *e = &L->c[c];
+            if (!e->act) continue;
+            if (n < cap) {
+                ixyz[n] = (uint32_t)((int64_t)e->x + (int64_t)e->y * o->nx + (int64_t)e->z * o->nxy);
+                gid[n] = e->frac < thr ? e->defg : o->n_defg + e->rxg;
+            }
+            n++;
+        }
+    }
+    return n;
+}
+
 /* ompshar_defragmentation of both lists outside the step loop (a test hook) */
 void oracle_defragment(oracle *o) {
     o_defrag_list(&o->in);

@@ -46,6 +46,8 @@ EXPORTED = (
     "score_gm_get_counts",
     "score_gm_get_trajectory",
     "score_gm_get_front",
+    "score_gm_get_status",
+    "score_gm_batch_run",
     "score_gm_set_profiling",
     "score_gm_get_profile",
     "score_gm_destroy",
@@ -112,6 +114,16 @@ class Result(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("domain", C.POINTER(Domain)), ("params", C.POINTER(Params)),
+        ("defg_counts", C.POINTER(C.c_uint32)), ("rxg_counts", C.POINTER(C.c_uint32)),
+        ("grainmap", C.POINTER(C.c_uint32)),
+        ("map_encoding", C.c_int32), ("status", C.c_int32),
+        ("result", Result), ("seconds", C.c_double),
+    ]
+
+
 # kernels of one growth step, in launch order (SCORE_GM_K_* in the header)
 KERNELS = ("k_reanchor", "k_predict_new", "k_sweep", "k_sweep_scan", "k_compact", "k_propose", "k_decide",
            "k_decide_scan", "k_recycle", "k_commit")
@@ -167,6 +179,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.score_gm_get_front.restype = C.c_int
     lib.score_gm_get_front.argtypes = [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), _u32p, _f64p, _f64p, _u32p, _u32p]
+    lib.score_gm_get_status.restype = C.c_int
+    lib.score_gm_get_status.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), _u32p, _u32p]
+    lib.score_gm_batch_run.restype = C.c_int
+    lib.score_gm_batch_run.argtypes = [C.POINTER(BatchItem), C.c_uint32, C.c_int32, C.c_uint32]
     lib.score_gm_set_profiling.restype = C.c_int
     lib.score_gm_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     lib.score_gm_get_profile.restype = C.c_int

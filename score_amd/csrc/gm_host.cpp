@@ -18,9 +18,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/score_gm.h"
@@ -1053,6 +1057,116 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
         return rc;
     }
     return 0;
+}
+
+int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n, uint32_t *ixyz,
+                        uint32_t *gid) {
+    if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
+    if (h->err) return h->err;
+    HIPCHK(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const uint32_t m = h->nI + h->nB;
+    uint32_t *buf = nullptr;
+    int rc = dalloc(&buf, 2ull * std::max<uint32_t>(m, 1u));
+    if (rc) return rc;
+    auto run = [&]() -> int {
+        HIPCHK(gm_launch_status(h->d, h->nI, h->nB, h->step, rxthreshold, buf, buf + m, s));
+        HIPCHK(hipMemcpyAsync(h->hst, h->d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const uint32_t cnt = h->hst->dcount;
+        *n = cnt;
+        const uint32_t k = std::min(cap, cnt);
+        if (k && ixyz) HIPCHK(hipMemcpyAsync(ixyz, buf, 4ull * k, hipMemcpyDeviceToHost, s));
+        if (k && gid) HIPCHK(hipMemcpyAsync(gid, buf + m, 4ull * k, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    };
+    rc = run();
+    dfree(buf);
+    return rc;
+}
+
+namespace {
+
+/* one domain of a batch, create to destroy; a domain that does not fit into
+ * device memory retries each time another one finishes */
+struct BatchGate {
+    std::mutex m;
+    std::condition_variable cv;
+    uint32_t inflight = 0; /* domains taken, including those waiting for memory */
+    uint32_t waiting = 0;
+    uint64_t finished = 0;
+};
+
+int batch_one(score_gm_batch_item &it, int32_t device, BatchGate &gate) {
+    score_gm_handle *h = nullptr;
+    int rc;
+    for (;;) {
+        uint64_t seen;
+        {
+            std::lock_guard<std::mutex> lk(gate.m);
+            seen = gate.finished;
+        }
+        rc = score_gm_create(it.domain, it.params, device, &h);
+        if (rc != SCORE_GM_ERR_OOM) break;
+        /* wait for a running domain to give its memory back; with none running
+         * besides the waiting ones, it does not fit */
+        std::unique_lock<std::mutex> lk(gate.m);
+        if (gate.finished == seen && gate.inflight - gate.waiting <= 1) break;
+        gate.waiting++;
+        gate.cv.wait(lk, [&] { return gate.finished != seen; });
+        gate.waiting--;
+    }
+    if (rc) return rc;
+    rc = score_gm_run(h, &it.result);
+    if (!rc && (it.defg_counts || it.rxg_counts)) rc = score_gm_get_counts(h, it.defg_counts, it.rxg_counts);
+    if (!rc && it.grainmap) rc = score_gm_get_grainmap(h, it.grainmap, it.map_encoding);
+    score_gm_destroy(h);
+    return rc;
+}
+
+}  // namespace
+
+int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, uint32_t max_inflight) {
+    if (n && !items) return fail(SCORE_GM_ERR_ARG, "null items");
+    for (uint32_t i = 0; i < n; i++) {
+        items[i].status = SCORE_GM_ERR_STATE;
+        memset(&items[i].result, 0, sizeof(items[i].result));
+        items[i].seconds = 0.0;
+    }
+    const uint32_t nt = std::min<uint32_t>(n, max_inflight ? max_inflight : 8u);
+    std::atomic<uint32_t> next{0};
+    BatchGate gate;
+    auto worker = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) return;
+            {
+                std::lock_guard<std::mutex> lk(gate.m);
+                gate.inflight++;
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            items[i].status = batch_one(items[i], device, gate);
+            items[i].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            {
+                std::lock_guard<std::mutex> lk(gate.m);
+                gate.inflight--;
+                gate.finished++;
+            }
+            gate.cv.notify_all();
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (uint32_t t = 1; t < nt; t++) pool.emplace_back(worker);
+    } catch (...) {
+        /* fewer threads than asked: the ones running take every item */
+    }
+    worker();
+    for (std::thread &t : pool) t.join();
+    for (uint32_t i = 0; i < n; i++)
+        if (items[i].status != SCORE_GM_OK) return items[i].status;
+    return SCORE_GM_OK;
 }
 
 int score_gm_set_profiling(score_gm_handle *h, int32_t enable) {

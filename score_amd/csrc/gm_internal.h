@@ -100,7 +100,7 @@ struct GmStats {
     uint32_t nzeroed;         /* deformed grains that lost their last cell this step */
     uint32_t cwrap;           /* a deformed-grain count was decremented past zero this step */
     uint32_t dfirst;          /* defragmentation: first INACTIVE slot of the list */
-    uint32_t dcount;          /* defragmentation: ACTIVE slots moved */
+    uint32_t dcount;          /* defragmentation: ACTIVE slots moved; status vector: entries */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
 };
@@ -191,6 +191,11 @@ hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s);
 hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s);
 hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
                                  GmRec *trec, uint32_t *tcs, hipStream_t s);
+/* ompcrit_clarify_status_infectedcells: (cell, grain shown) of every ACTIVE record,
+ * Inside then Border, in slot order, before step k; the count goes to st->dcount.
+ * ixyz and gid hold nI+nB entries. */
+hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k, double thr, uint32_t *ixyz,
+                            uint32_t *gid, hipStream_t s);
 /* the cell grid with every cell held by an active front record marked INFECTED */
 hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

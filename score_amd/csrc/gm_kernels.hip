@@ -1243,7 +1243,8 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_count(GmDev d, uint32_t bas
     if (threadIdx.x == 0) d.bcnt[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(1024) k_defrag_scan(GmDev d, uint32_t nb) {
+/* exclusive scan of one row of per-block counts; the total goes to st->dcount */
+__global__ void __launch_bounds__(1024) k_scan_row(GmDev d, uint32_t nb) {
     uint32_t tot[1];
     scan_block_counts<1>(d.bcnt, nb, tot);
     if (threadIdx.x == 0) d.st->dcount = tot[0];
@@ -1311,8 +1312,70 @@ hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, 
     const uint32_t n = nnever - first - 1u;
     const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
     k_defrag_count<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n);
-    k_defrag_scan<<<1, 1024, 0, s>>>(d, nb);
+    k_scan_row<<<1, 1024, 0, s>>>(d, nb);
     k_defrag_gather<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n, k, trec, tcs);
     k_defrag_scatter<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, base, first, trec, tcs);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------ status vector
+ * ompcrit_clarify_status_infectedcells (SCORE_Io_Characterization.cpp:925-997):
+ * every ACTIVE record, Inside list then Border list in slot order, as its
+ * linear cell index and the grain the cell shows before step k: ndefg+rxg once
+ * rxFrac has reached the threshold, else its deformed grain.  Item i of
+ * [0, nI+nB) is Inside slot i or Border slot i-nI. */
+
+static __device__ __forceinline__ uint32_t item_slot(const GmDev &d, uint32_t nI, uint32_t i) {
+    return i < nI ? i : d.capI + (i - nI);
+}
+
+__global__ void __launch_bounds__(GM_BLOCK) k_status_count(GmDev d, uint32_t nI, uint32_t n) {
+    uint32_t lo, hi;
+    chunk_of(n, lo, hi);
+    uint32_t c = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += GM_BLOCK)
+        c += d.cstep[item_slot(d, nI, i)] != GM_CS_INACTIVE ? 1u : 0u;
+    uint32_t tot;
+    block_incl_scan(c, &tot);
+    if (threadIdx.x == 0) d.bcnt[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(GM_BLOCK) k_status_write(GmDev d, uint32_t nI, uint32_t n, uint32_t k, double thr,
+                                                          uint32_t *ixyz, uint32_t *gid) {
+    uint32_t lo, hi;
+    chunk_of(n, lo, hi);
+    uint32_t run = d.bcnt[blockIdx.x];
+    for (uint32_t b0 = lo; b0 < hi; b0 += GM_BLOCK) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t s = i < hi ? item_slot(d, nI, i) : 0u;
+        const uint32_t cs = i < hi ? d.cstep[s] : GM_CS_INACTIVE;
+        const uint32_t act = cs != GM_CS_INACTIVE ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t incl = block_incl_scan(act, &tot);
+        if (act) {
+            const GmRec *r = d.rec + s;
+            const RecA a = rec_a(r);
+            const RecC c = rec_c(r);
+            double f = a.frac0;
+            if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, k - a.s0);
+            const uint32_t o = run + incl - 1u;
+            ixyz[o] = c.cell;
+            gid[o] = f < thr ? c.defg : d.ndefg + r->rxg;
+        }
+        run += tot;
+    }
+}
+
+hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k, double thr, uint32_t *ixyz,
+                            uint32_t *gid, hipStream_t s) {
+    const uint32_t n = nI + nB;
+    if (n == 0) {
+        k_fill_u32<<<1, 64, 0, s>>>(&d.st->dcount, 0u, 1);
+        return hipGetLastError();
+    }
+    const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
+    k_status_count<<<nb, GM_BLOCK, 0, s>>>(d, nI, n);
+    k_scan_row<<<1, 1024, 0, s>>>(d, nb);
+    k_status_write<<<nb, GM_BLOCK, 0, s>>>(d, nI, n, k, thr, ixyz, gid);
     return hipGetLastError();
 }

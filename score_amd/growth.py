@@ -104,6 +104,20 @@ class GrowthMachine:
                 p("rxg", C.c_uint32)))
         return out
 
+    def status(self, rxthreshold: float = 0.5):
+        """(linear cell index, grain id shown) of every ACTIVE front record, Inside
+        list then Border list in slot order (ompcrit_clarify_status_infectedcells,
+        SCORE_Io_Characterization.cpp:925-997)."""
+        n = C.c_uint32(0)
+        nul = C.cast(None, C.POINTER(C.c_uint32))
+        abi.check(self._lib, self._lib.score_gm_get_status(self._handle(), rxthreshold, 0, C.byref(n), nul, nul))
+        ixyz, gid = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint32)
+        if n.value:
+            p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+            abi.check(self._lib, self._lib.score_gm_get_status(self._handle(), rxthreshold, n.value, C.byref(n),
+                                                               p(ixyz), p(gid)))
+        return ixyz, gid
+
     def set_profiling(self, on: bool = True) -> None:
         abi.check(self._lib, self._lib.score_gm_set_profiling(self._handle(), 1 if on else 0))
 
@@ -145,3 +159,44 @@ def same_trajectory(a: np.ndarray, b: np.ndarray) -> bool:
     a["npending"] = 0
     b["npending"] = 0
     return a.tobytes() == b.tobytes()
+
+
+def batch_run(domains, params=None, max_inflight: int = 8, device: int = 0, grainmaps: bool = False,
+              encoding: int = abi.MAP_CELLGRID) -> list[dict]:
+    """Every domain to completion through score_gm_batch_run (the rank's loop
+    over its domains in ensembleHdl::SIMULATE_myCAs, SCORE_Kernel.cpp:1693-1757),
+    up to `max_inflight` at a time.  `params` is one ParamsSpec for all or one per
+    domain.  Returns per domain: status, result, seconds, counts and optionally the
+    grain map.  Raises on the first failed domain, after all have run."""
+    lib = abi.load()
+    n = len(domains)
+    if params is None or isinstance(params, ParamsSpec):
+        params = [params or ParamsSpec()] * n
+    if len(params) != n:
+        raise ValueError("one ParamsSpec per domain, or one for all")
+    keep, items = [], (abi.BatchItem * max(n, 1))()
+    outs = []
+    for i, (dom, prm) in enumerate(zip(domains, params)):
+        cd, kd = dom.to_c()
+        cp, kp = prm.to_c()
+        dc = np.zeros(dom.n_defg, np.uint32)
+        rc = np.zeros(max(dom.n_rxg, 1), np.uint32)
+        gm = np.zeros(dom.N, np.uint32) if grainmaps else None
+        keep.append((cd, kd, cp, kp))
+        it = items[i]
+        it.domain = C.pointer(cd)
+        it.params = C.pointer(cp)
+        it.defg_counts = dc.ctypes.data_as(C.POINTER(C.c_uint32))
+        it.rxg_counts = rc.ctypes.data_as(C.POINTER(C.c_uint32))
+        it.grainmap = gm.ctypes.data_as(C.POINTER(C.c_uint32)) if gm is not None else None
+        it.map_encoding = encoding
+        outs.append((dc, rc, gm))
+    status = lib.score_gm_batch_run(items, n, device, max_inflight)
+    res = []
+    for i, (dom, (dc, rc, gm)) in enumerate(zip(domains, outs)):
+        it = items[i]
+        res.append(dict(status=it.status, result=result_dict(it.result), seconds=it.seconds,
+                        defg_counts=dc, rxg_counts=rc[: dom.n_rxg], grainmap=gm))
+    if status != abi.OK:
+        abi.check(lib, status)
+    return res

@@ -65,7 +65,8 @@ def test_struct_layouts_match_c():
     if not cc:
         pytest.skip("no C compiler")
     structs = {"score_gm_domain": abi.Domain, "score_gm_params": abi.Params, "score_gm_traj_row": abi.TrajRow,
-               "score_gm_result": abi.Result, "score_gm_profile": abi.Profile}
+               "score_gm_result": abi.Result, "score_gm_profile": abi.Profile,
+               "score_gm_batch_item": abi.BatchItem}
     body = []
     for cname, py in structs.items():
         for f, _ in py._fields_:
@@ -118,3 +119,19 @@ def test_invalid_arguments_rejected_before_device_use():
     cp.zener_mode = 1
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_UNSUPPORTED
     assert not h.value
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a GPU")
+def test_batch_reports_every_domain_without_device():
+    lib = abi.load()
+    doms = [synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=s) for s in (1, 2, 3)]
+    keep, items = [], (abi.BatchItem * 3)()
+    for i, d in enumerate(doms):
+        cd, kd = d.to_c()
+        cp, kp = synth.ParamsSpec().to_c()
+        keep.append((cd, kd, cp, kp))
+        items[i].domain, items[i].params = C.pointer(cd), C.pointer(cp)
+    rc = lib.score_gm_batch_run(items, 3, 0, 2)
+    assert rc in (abi.ERR_NODEVICE, abi.ERR_HIP)
+    assert all(items[i].status == rc for i in range(3))
+    assert lib.score_gm_batch_run(None, 0, 0, 0) == abi.OK

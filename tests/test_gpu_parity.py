@@ -154,6 +154,26 @@ def test_defragmentation_voronoi100(require_gpu):
     assert_final_equal(o, g)
 
 
+def test_status_vector(require_gpu):
+    """(cell, grain shown) of every ACTIVE record before each step, as
+    ompcrit_clarify_status_infectedcells collects it, at the reference's
+    threshold 0.5 and at 0.9"""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=6, n_nuclei=12, seed=2)
+    prm = ParamsSpec(defrag_at=(0.3 * dom.N,))
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        for thr in (0.5, 0.9):
+            a, b = o.status(thr), g.status(thr)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (k, thr)
+        do, dg = o.step(1), g.step(1)
+        assert do == dg
+        if do:
+            break
+        k += 1
+    assert k > 100
+
+
 def test_single_grain_many_nuclei_xmax1(require_gpu):
     """one deformed grain, nuclei on the faces and edges, run to X = 1 (all cells)"""
     dom = synth.make_voronoi_domain(40, 40, 40, n_grains=1, n_nuclei=60, seed=7, log10_rho=(15.0, 15.0))
@@ -218,6 +238,33 @@ def test_border_overflow_reported(require_gpu):
         g.solve_rxgrowth()
     assert eg.value.code == abi.ERR_CAPACITY
     assert getattr(eo.value, "code", None) == abi.ERR_CAPACITY
+
+
+def test_batch_run_matches_oracle_per_domain(require_gpu):
+    """Seven domains of different shapes and settings, four in flight at once;
+    one of them overflows its Border list and fails alone."""
+    from score_amd.growth import batch_run
+    doms = [synth.make_voronoi_domain(24 + 6 * i, 20 + 4 * i, 18 + 5 * i, n_grains=4 + i, n_nuclei=8 + 3 * i,
+                                      seed=20 + i) for i in range(6)]
+    prms = [ParamsSpec(), ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN),
+            ParamsSpec(sched_temp=(950.0, 1050.0)), ParamsSpec(defrag_at=(0.2 * doms[3].N, 0.6 * doms[3].N)),
+            ParamsSpec(maxfill=0.05), ParamsSpec(xmax=0.5)]
+    doms.insert(3, overflow_slab())
+    prms.insert(3, ParamsSpec(xmax=1.0))
+    with pytest.raises(abi.ScoreGmError) as e:
+        batch_run(doms, prms, max_inflight=4, grainmaps=True)
+    assert e.value.code == abi.ERR_CAPACITY
+    res = batch_run(doms[:3] + doms[4:], prms[:3] + prms[4:], max_inflight=4, grainmaps=True)
+    for dom, prm, r in zip(doms[:3] + doms[4:], prms[:3] + prms[4:], res):
+        assert r["status"] == abi.OK and r["seconds"] > 0.0
+        o = Oracle(dom, prm)
+        ro = o.solve_rxgrowth()
+        for k in ("t", "X", "xcells", "step", "completed", "sum_sv", "nuclei_placed"):
+            assert ro[k] == r["result"][k], k
+        assert np.array_equal(o.grainmap(), r["grainmap"])
+        dc, rc = o.counts()
+        assert np.array_equal(dc, r["defg_counts"]) and np.array_equal(rc, r["rxg_counts"])
+        o.close()
 
 
 def test_repeatable(require_gpu):

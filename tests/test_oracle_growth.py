@@ -341,3 +341,28 @@ def test_defragmentation_revives_records_left_behind():
     assert int(rc.sum()) == r["xcells"] > rx_cells
     assert int(dc.max()) > dom.N
     o.close()
+
+
+def test_status_vector_follows_front_and_threshold():
+    """ompcrit_clarify_status_infectedcells (SCORE_Io_Characterization.cpp:925-997):
+    one entry per ACTIVE record, Inside then Border; the nucleus id (offset by
+    n_defg) once rxFrac >= threshold, else the deformed grain.  Checked against
+    the front lists themselves at three thresholds, one that puts every cell on
+    the deformed side and one that puts them all on the nucleus side."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=6, n_nuclei=12, seed=2)
+    o = Oracle(dom, ParamsSpec())
+    o.step(25)
+    fr = [o.front(lst) for lst in (abi.LIST_INSIDE, abi.LIST_BORDER)]
+    act = [f["activity"].astype(bool) for f in fr]
+    cells = np.concatenate([f["cell"][a] for f, a in zip(fr, act)])
+    frac = np.concatenate([f["rxfrac"][a] for f, a in zip(fr, act)])
+    defg = np.concatenate([f["defg"][a] for f, a in zip(fr, act)])
+    rxg = np.concatenate([f["rxg"][a] for f, a in zip(fr, act)])
+    assert 0.0 < frac.min() < 0.5 < frac.max()
+    for thr in (0.5, 2.0, 0.0):
+        ixyz, gid = o.status(thr)
+        assert np.array_equal(ixyz, cells)
+        assert np.array_equal(gid, np.where(frac < thr, defg, dom.n_defg + rxg))
+    assert np.array_equal(o.status(2.0)[1], defg)
+    assert np.array_equal(o.status(0.0)[1], dom.n_defg + rxg)
+    o.close()
