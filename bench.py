@@ -17,6 +17,7 @@ workload on the host.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for PMC scaling
     python bench.py --batch 512 --inflight 8 # ensemble of independent domains, domains/hour
+    python bench.py --serve 64 --inflight 8  # Poisson stream of requests, p50/p99 time to X=0.99
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -137,6 +138,86 @@ def batch_bench(a) -> int:
     return 0
 
 
+def serve_bench(a) -> int:
+    """Served stream: requests arrive as a Poisson process (edges log-uniform in
+    [64, 320], grain and nucleus densities of the 512^3 workload, T uniform in
+    [900, 1100] K) and run through the request loop (score_gm_server_*) with
+    `--inflight` workers and the default memory budget.  The arrival rate is
+    `--rate` per second, or 80% of the throughput the same requests reach when
+    all are queued at once.  Latency is arrival to X = 0.99 result.  The same
+    arrivals are then served one request at a time for comparison."""
+    import numpy as np
+    from score_amd import synth
+    from score_amd.growth import Server, batch_run, device_count
+
+    if device_count() < 1:
+        print("no gfx950 device visible", file=sys.stderr)
+        return 3
+    rng = np.random.default_rng(a.seed + 7)
+    t0 = time.time()
+    doms, prms = [], []
+    for i in range(a.serve):
+        e = int(round(np.exp(rng.uniform(np.log(64.0), np.log(320.0)))))
+        n = e ** 3
+        T = float(rng.uniform(900.0, 1100.0))
+        doms.append(synth.make_voronoi_domain(e, e, e, n_grains=max(1, round(n * 8000 / 512 ** 3)),
+                                              n_nuclei=max(1, round(n * 100000 / 512 ** 3)), seed=a.seed * 7919 + i))
+        prms.append(synth.ParamsSpec(sched_temp=(T, T)))
+    voxels = sum(d.N for d in doms)
+    log(f"served stream of {a.serve} requests, {voxels} voxels, generated in {time.time() - t0:.1f}s")
+    batch_run(doms[:2], prms[:2], max_inflight=2)  # warm-up
+    rate = a.rate
+    if not rate:
+        t1 = time.perf_counter()
+        batch_run(doms, prms, max_inflight=a.inflight)
+        cap = a.serve / (time.perf_counter() - t1)
+        rate = 0.8 * cap
+        log(f"capacity with {a.inflight} in flight: {cap:.1f} requests/s; arrival rate {rate:.1f}/s")
+    gaps = rng.exponential(1.0 / rate, a.serve)
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def serve(inflight):
+        with Server(max_inflight=inflight) as srv:
+            reqs = []
+            t1 = time.perf_counter()
+            for i in range(a.serve):
+                dt = arrivals[i] - (time.perf_counter() - t1)
+                if dt > 0:
+                    time.sleep(dt)
+                reqs.append(srv.submit(doms[i], prms[i]))
+            for r in reqs:
+                srv.wait(r)
+            span = time.perf_counter() - t1
+        lat = np.array([r.item.queued_seconds + r.item.seconds for r in reqs])
+        if any(r.item.status != 0 for r in reqs):
+            raise RuntimeError("a served request failed")
+        return span, lat, [r.outcome() for r in reqs]
+
+    span, lat, outs = serve(a.inflight)
+    log(f"served with {a.inflight} in flight: {span:.2f}s, p50 {np.percentile(lat, 50):.3f}s, "
+        f"p99 {np.percentile(lat, 99):.3f}s")
+    span1, lat1, outs1 = serve(1)
+    log(f"served one at a time: {span1:.2f}s, p50 {np.percentile(lat1, 50):.3f}s, p99 {np.percentile(lat1, 99):.3f}s")
+    same = all(x["result"]["xcells"] == y["result"]["xcells"] and x["result"]["t"] == y["result"]["t"] and
+               np.array_equal(x["defg_counts"], y["defg_counts"]) for x, y in zip(outs, outs1))
+    if not same:
+        raise RuntimeError("served results differ between the mix and one at a time")
+    out = {
+        "metric": "domains/hour", "value": a.serve / span * 3600.0, "unit": "domains/hour", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "p50_time_to_x099_s": float(np.percentile(lat, 50)), "p99_time_to_x099_s": float(np.percentile(lat, 99)),
+        "arrival_rate_per_s": rate,
+        "one_at_a_time": {"domains_per_hour": a.serve / span1 * 3600.0,
+                          "p50_time_to_x099_s": float(np.percentile(lat1, 50)),
+                          "p99_time_to_x099_s": float(np.percentile(lat1, 99))},
+        "config": {"workload": f"served stream of {a.serve} Voronoi domains, Poisson arrivals, edges log-U[64,320], "
+                               f"grain and nucleus densities of the 512^3 workload, T~U[900,1100] K, to X=0.99",
+                   "voxels": voxels, "max_inflight": a.inflight, "seed": a.seed},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +232,10 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="ensemble mode: this many independent domains, domains/hour batched vs one at a time")
-    ap.add_argument("--inflight", type=int, default=8, help="domains in flight in ensemble mode")
+    ap.add_argument("--inflight", type=int, default=8, help="domains in flight in ensemble and served modes")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="served mode: this many requests arriving as a Poisson stream; p50/p99 time to X=0.99")
+    ap.add_argument("--rate", type=float, default=0.0, help="arrival rate of the served mode, requests/s")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -162,6 +246,8 @@ def main(argv=None) -> int:
         return copy_calibration()
     if a.batch:
         return batch_bench(a)
+    if a.serve:
+        return serve_bench(a)
 
     from score_amd import abi, synth
     from score_amd.growth import GrowthMachine, device_count, same_trajectory

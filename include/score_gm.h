@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 4
+#define SCORE_GM_ABI_VERSION 5
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -243,6 +243,7 @@ typedef struct score_gm_batch_item {
     int32_t status;                /* out: SCORE_GM_OK or this domain's error */
     score_gm_result result;        /* out */
     double seconds;                /* out: wall time of this domain, create to destroy */
+    double queued_seconds;         /* out: from submission (or the batch call) to its start */
 } score_gm_batch_item;
 
 /* Runs every domain of `items` to completion on one device, as the rank's loop
@@ -253,6 +254,33 @@ typedef struct score_gm_batch_item {
  * SCORE_GM_OK when every item succeeded, else the status of the lowest failed
  * item; every item's own status is set either way. */
 int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, uint32_t max_inflight);
+
+/* Device memory one domain needs at most, in bytes: the arrays score_gm_create
+ * allocates plus the per-step lists at the size a front of N/4 cells needs. */
+int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, uint64_t *bytes);
+
+/* A request loop over domains that arrive while others run (continuous
+ * batching): each submitted domain starts as soon as a worker is free and its
+ * footprint fits into the memory budget next to the domains running (in
+ * submission order; a domain alone is always started), runs on its own handle
+ * and stream, and is then released.  SCORE has no counterpart; it serves the
+ * same call, solve_RXGROWTH, to a stream of requests. */
+typedef struct score_gm_server score_gm_server;
+
+/* max_inflight 0 selects 8; hbm_budget 0 selects 90% of the device's memory. */
+int score_gm_server_create(int32_t device, uint32_t max_inflight, uint64_t hbm_budget, score_gm_server **out);
+
+/* Queues one domain.  The item and every array it points to must stay valid
+ * until the request is complete; its outputs are written as in
+ * score_gm_batch_run.  *ticket identifies the request for score_gm_server_wait. */
+int score_gm_server_submit(score_gm_server *srv, score_gm_batch_item *item, uint64_t *ticket);
+
+/* Waits up to timeout_ms (negative: no limit) for a request; returns 1 once it
+ * is complete (its item's status says how it ended), 0 on time-out. */
+int score_gm_server_wait(score_gm_server *srv, uint64_t ticket, int32_t timeout_ms);
+
+/* Completes every queued request, then releases the workers. */
+int score_gm_server_destroy(score_gm_server *srv);
 
 /* Switch per-phase event timing on (1) or off (0); switching on clears the
  * counters.  Replaces SCORE's OutputThreadProfiling switch (SCORE_Kernel.cpp:6677-6683). */

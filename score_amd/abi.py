@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK = 0
 ERR_ARG = -1
@@ -48,6 +48,11 @@ EXPORTED = (
     "score_gm_get_front",
     "score_gm_get_status",
     "score_gm_batch_run",
+    "score_gm_footprint",
+    "score_gm_server_create",
+    "score_gm_server_submit",
+    "score_gm_server_wait",
+    "score_gm_server_destroy",
     "score_gm_set_profiling",
     "score_gm_get_profile",
     "score_gm_destroy",
@@ -120,7 +125,7 @@ class BatchItem(C.Structure):
         ("defg_counts", C.POINTER(C.c_uint32)), ("rxg_counts", C.POINTER(C.c_uint32)),
         ("grainmap", C.POINTER(C.c_uint32)),
         ("map_encoding", C.c_int32), ("status", C.c_int32),
-        ("result", Result), ("seconds", C.c_double),
+        ("result", Result), ("seconds", C.c_double), ("queued_seconds", C.c_double),
     ]
 
 
@@ -183,6 +188,16 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.score_gm_get_status.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), _u32p, _u32p]
     lib.score_gm_batch_run.restype = C.c_int
     lib.score_gm_batch_run.argtypes = [C.POINTER(BatchItem), C.c_uint32, C.c_int32, C.c_uint32]
+    lib.score_gm_footprint.restype = C.c_int
+    lib.score_gm_footprint.argtypes = [C.POINTER(Domain), C.POINTER(Params), C.POINTER(C.c_uint64)]
+    lib.score_gm_server_create.restype = C.c_int
+    lib.score_gm_server_create.argtypes = [C.c_int32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.score_gm_server_submit.restype = C.c_int
+    lib.score_gm_server_submit.argtypes = [C.c_void_p, C.POINTER(BatchItem), C.POINTER(C.c_uint64)]
+    lib.score_gm_server_wait.restype = C.c_int
+    lib.score_gm_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int32]
+    lib.score_gm_server_destroy.restype = C.c_int
+    lib.score_gm_server_destroy.argtypes = [C.c_void_p]
     lib.score_gm_set_profiling.restype = C.c_int
     lib.score_gm_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     lib.score_gm_get_profile.restype = C.c_int

@@ -21,6 +21,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <new>
 #include <string>
@@ -1133,10 +1134,12 @@ int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, u
         items[i].status = SCORE_GM_ERR_STATE;
         memset(&items[i].result, 0, sizeof(items[i].result));
         items[i].seconds = 0.0;
+        items[i].queued_seconds = 0.0;
     }
     const uint32_t nt = std::min<uint32_t>(n, max_inflight ? max_inflight : 8u);
     std::atomic<uint32_t> next{0};
     BatchGate gate;
+    const auto tcall = std::chrono::steady_clock::now();
     auto worker = [&]() {
         for (;;) {
             const uint32_t i = next.fetch_add(1);
@@ -1146,6 +1149,7 @@ int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, u
                 gate.inflight++;
             }
             const auto t0 = std::chrono::steady_clock::now();
+            items[i].queued_seconds = std::chrono::duration<double>(t0 - tcall).count();
             items[i].status = batch_one(items[i], device, gate);
             items[i].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             {
@@ -1167,6 +1171,173 @@ int score_gm_batch_run(score_gm_batch_item *items, uint32_t n, int32_t device, u
     for (uint32_t i = 0; i < n; i++)
         if (items[i].status != SCORE_GM_OK) return items[i].status;
     return SCORE_GM_OK;
+}
+
+int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, uint64_t *bytes) {
+    if (!bytes) return fail(SCORE_GM_ERR_ARG, "null argument");
+    int rc = validate(dom, prm);
+    if (rc) return rc;
+    const uint64_t N = (uint64_t)dom->nx * dom->ny * dom->nz;
+    const uint64_t capB = 2ull * ((uint64_t)dom->nx * dom->ny + (uint64_t)dom->nx * dom->nz + (uint64_t)dom->ny * dom->nz);
+    const uint64_t capTot = (N + 63) / 64 * 64 + (capB + 63) / 64 * 64 + 256;
+    const uint64_t bricks = (uint64_t)((dom->nx + 15) / 16) * ((dom->ny + 7) / 8) * ((dom->nz + 7) / 8);
+    const uint64_t hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * dom->n_rxg));
+    const uint64_t front = std::max<uint64_t>(65536, N / 4);
+    uint64_t b = 0;
+    b += 8 * N;                                   /* grid, claim */
+    b += 32 * bricks;                             /* deformed bitmap */
+    b += (4 + sizeof(GmRec)) * capTot;            /* completion steps, records */
+    b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */
+    b += 4 * std::min<uint64_t>(N, 26 * front);   /* new-cell list */
+    b += 16 * (capTot / 64 + 4);                  /* tile masks */
+    b += 4 * (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt + (double)capB * prm->maxfill * kFullRecyclingAtt);
+    b += 4 * 8 * GM_MAX_BLOCKS;
+    b += 52ull * dom->n_defg + 52ull * (dom->n_rxg + 1);
+    b += 28 * hcap;                               /* weight cache: keys, values, pending slots and keys */
+    b += sizeof(GmStats) + 8 * GM_SCAN_BLOCKS;
+    *bytes = b;
+    return 0;
+}
+
+struct score_gm_server {
+    int32_t device = 0;
+    uint64_t budget = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    struct Req {
+        score_gm_batch_item *item;
+        uint64_t ticket, bytes;
+        std::chrono::steady_clock::time_point submitted;
+    };
+    std::deque<Req> queue;
+    std::vector<uint8_t> done;      /* by ticket */
+    uint64_t in_use = 0;            /* footprints of the running requests */
+    uint32_t running = 0;
+    uint64_t finished = 0;
+    bool stopping = false;
+    std::vector<std::thread> pool;
+    BatchGate gate;                 /* retries after an allocation failure despite the budget */
+
+    void work() {
+        for (;;) {
+            Req r;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv_work.wait(lk, [&] {
+                    return (stopping && queue.empty()) ||
+                           (!queue.empty() && (running == 0 || in_use + queue.front().bytes <= budget));
+                });
+                if (queue.empty()) return;
+                r = queue.front();
+                queue.pop_front();
+                running++;
+                in_use += r.bytes;
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            r.item->queued_seconds = std::chrono::duration<double>(t0 - r.submitted).count();
+            {
+                std::lock_guard<std::mutex> lk(gate.m);
+                gate.inflight++;
+            }
+            r.item->status = batch_one(*r.item, device, gate);
+            r.item->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            {
+                std::lock_guard<std::mutex> lk(gate.m);
+                gate.inflight--;
+                gate.finished++;
+            }
+            gate.cv.notify_all();
+            {
+                std::lock_guard<std::mutex> lk(m);
+                running--;
+                in_use -= r.bytes;
+                finished++;
+                done[r.ticket] = 1;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+int score_gm_server_create(int32_t device, uint32_t max_inflight, uint64_t hbm_budget, score_gm_server **out) {
+    if (!out) return fail(SCORE_GM_ERR_ARG, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return fail(SCORE_GM_ERR_NODEVICE, "no such device");
+    }
+    if (hipSetDevice(device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+    if (!hbm_budget) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        hbm_budget = (uint64_t)((double)tot * 0.9);
+    }
+    score_gm_server *srv = new (std::nothrow) score_gm_server;
+    if (!srv) return fail(SCORE_GM_ERR_OOM, "server");
+    srv->device = device;
+    srv->budget = hbm_budget;
+    const uint32_t nt = max_inflight ? max_inflight : 8u;
+    try {
+        for (uint32_t t = 0; t < nt; t++) srv->pool.emplace_back([srv] { srv->work(); });
+    } catch (...) {
+        if (srv->pool.empty()) {
+            delete srv;
+            return fail(SCORE_GM_ERR_OOM, "server threads");
+        }
+    }
+    *out = srv;
+    return 0;
+}
+
+int score_gm_server_submit(score_gm_server *srv, score_gm_batch_item *item, uint64_t *ticket) {
+    if (!srv || !item || !ticket) return fail(SCORE_GM_ERR_ARG, "null argument");
+    uint64_t bytes = 0;
+    int rc = score_gm_footprint(item->domain, item->params, &bytes);
+    item->status = SCORE_GM_ERR_STATE;
+    memset(&item->result, 0, sizeof(item->result));
+    item->seconds = 0.0;
+    item->queued_seconds = 0.0;
+    {
+        std::lock_guard<std::mutex> lk(srv->m);
+        if (srv->stopping) return fail(SCORE_GM_ERR_STATE, "server is shutting down");
+        *ticket = srv->done.size();
+        if (rc) {
+            /* refused before it is queued: complete at once with its error */
+            item->status = rc;
+            srv->done.push_back(1);
+            return rc;
+        }
+        srv->done.push_back(0);
+        srv->queue.push_back({item, *ticket, bytes, std::chrono::steady_clock::now()});
+    }
+    srv->cv_work.notify_all();
+    return 0;
+}
+
+int score_gm_server_wait(score_gm_server *srv, uint64_t ticket, int32_t timeout_ms) {
+    if (!srv) return fail(SCORE_GM_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(srv->m);
+    if (ticket >= srv->done.size()) return fail(SCORE_GM_ERR_ARG, "unknown ticket");
+    auto ready = [&] { return srv->done[ticket] != 0; };
+    if (timeout_ms < 0) {
+        srv->cv_done.wait(lk, ready);
+        return 1;
+    }
+    return srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready) ? 1 : 0;
+}
+
+int score_gm_server_destroy(score_gm_server *srv) {
+    if (!srv) return 0;
+    {
+        std::lock_guard<std::mutex> lk(srv->m);
+        srv->stopping = true;
+    }
+    srv->cv_work.notify_all();
+    for (std::thread &t : srv->pool) t.join();
+    delete srv;
+    return 0;
 }
 
 int score_gm_set_profiling(score_gm_handle *h, int32_t enable) {

@@ -161,6 +161,89 @@ def same_trajectory(a: np.ndarray, b: np.ndarray) -> bool:
     return a.tobytes() == b.tobytes()
 
 
+def footprint(dom: DomainArrays, prm: ParamsSpec | None = None) -> int:
+    """Device bytes one domain needs at most (score_gm_footprint)."""
+    lib = abi.load()
+    cd, kd = dom.to_c()
+    cp, kp = (prm or ParamsSpec()).to_c()
+    b = C.c_uint64(0)
+    abi.check(lib, lib.score_gm_footprint(C.byref(cd), C.byref(cp), C.byref(b)))
+    return b.value
+
+
+class Request:
+    """One submitted domain; its arrays live here until the request is complete."""
+
+    def __init__(self, dom: DomainArrays, prm: ParamsSpec, grainmap: bool, encoding: int):
+        self.dom = dom
+        self._cd, self._kd = dom.to_c()
+        self._cp, self._kp = prm.to_c()
+        self.defg_counts = np.zeros(dom.n_defg, np.uint32)
+        self._rxg = np.zeros(max(dom.n_rxg, 1), np.uint32)
+        self.grainmap = np.zeros(dom.N, np.uint32) if grainmap else None
+        it = abi.BatchItem()
+        it.domain = C.pointer(self._cd)
+        it.params = C.pointer(self._cp)
+        it.defg_counts = self.defg_counts.ctypes.data_as(C.POINTER(C.c_uint32))
+        it.rxg_counts = self._rxg.ctypes.data_as(C.POINTER(C.c_uint32))
+        it.grainmap = self.grainmap.ctypes.data_as(C.POINTER(C.c_uint32)) if grainmap else None
+        it.map_encoding = encoding
+        self.item = it
+        self.ticket = None
+
+    @property
+    def rxg_counts(self):
+        return self._rxg[: self.dom.n_rxg]
+
+    def outcome(self) -> dict:
+        it = self.item
+        return dict(status=it.status, result=result_dict(it.result), seconds=it.seconds,
+                    queued_seconds=it.queued_seconds, defg_counts=self.defg_counts, rxg_counts=self.rxg_counts,
+                    grainmap=self.grainmap)
+
+
+class Server:
+    """Request loop over arriving domains (score_gm_server_*): each starts when a
+    worker is free and its footprint fits the memory budget next to the running ones."""
+
+    def __init__(self, max_inflight: int = 8, hbm_budget: int = 0, device: int = 0):
+        self._lib = abi.load()
+        s = C.c_void_p()
+        abi.check(self._lib, self._lib.score_gm_server_create(device, max_inflight, hbm_budget, C.byref(s)))
+        self._s = s
+        self._reqs = []
+
+    def submit(self, dom: DomainArrays, prm: ParamsSpec | None = None, grainmap: bool = False,
+               encoding: int = abi.MAP_CELLGRID) -> Request:
+        r = Request(dom, prm or ParamsSpec(), grainmap, encoding)
+        t = C.c_uint64(0)
+        abi.check(self._lib, self._lib.score_gm_server_submit(self._s, C.byref(r.item), C.byref(t)))
+        r.ticket = t.value
+        self._reqs.append(r)
+        return r
+
+    def wait(self, req: Request, timeout: float | None = None) -> bool:
+        ms = -1 if timeout is None else int(timeout * 1000)
+        return bool(abi.check(self._lib, self._lib.score_gm_server_wait(self._s, req.ticket, ms)))
+
+    def close(self) -> None:
+        if self._s is not None:
+            self._lib.score_gm_server_destroy(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def batch_run(domains, params=None, max_inflight: int = 8, device: int = 0, grainmaps: bool = False,
               encoding: int = abi.MAP_CELLGRID) -> list[dict]:
     """Every domain to completion through score_gm_batch_run (the rank's loop
@@ -196,7 +279,7 @@ def batch_run(domains, params=None, max_inflight: int = 8, device: int = 0, grai
     for i, (dom, (dc, rc, gm)) in enumerate(zip(domains, outs)):
         it = items[i]
         res.append(dict(status=it.status, result=result_dict(it.result), seconds=it.seconds,
-                        defg_counts=dc, rxg_counts=rc[: dom.n_rxg], grainmap=gm))
+                        queued_seconds=it.queued_seconds, defg_counts=dc, rxg_counts=rc[: dom.n_rxg], grainmap=gm))
     if status != abi.OK:
         abi.check(lib, status)
     return res

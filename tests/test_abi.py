@@ -135,3 +135,27 @@ def test_batch_reports_every_domain_without_device():
     assert rc in (abi.ERR_NODEVICE, abi.ERR_HIP)
     assert all(items[i].status == rc for i in range(3))
     assert lib.score_gm_batch_run(None, 0, 0, 0) == abi.OK
+
+
+def test_footprint_covers_the_domain_arrays():
+    from score_amd.growth import footprint
+    small = synth.make_voronoi_domain(32, 32, 32, n_grains=4, n_nuclei=8, seed=1)
+    big = synth.make_voronoi_domain(64, 48, 40, n_grains=20, n_nuclei=30, seed=1)
+    fs, fb = footprint(small), footprint(big)
+    assert fb > fs
+    # grid and claims (8 B), completion steps and 64-byte records (68 B) per cell at least
+    assert fb >= 76 * big.N and fs >= 76 * small.N
+    dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
+    dom.cellgrid = dom.cellgrid.copy()
+    dom.cellgrid[0] = 9
+    with pytest.raises(abi.ScoreGmError) as e:
+        footprint(dom)
+    assert e.value.code == abi.ERR_ARG
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a GPU")
+def test_server_needs_a_device():
+    from score_amd.growth import Server
+    with pytest.raises(abi.ScoreGmError) as e:
+        Server(max_inflight=2)
+    assert e.value.code in (abi.ERR_NODEVICE, abi.ERR_HIP)

@@ -267,6 +267,46 @@ def test_batch_run_matches_oracle_per_domain(require_gpu):
         o.close()
 
 
+def _alone(dom, prm):
+    g = _gm(dom, prm)
+    r = g.solve_rxgrowth()
+    out = (r, g.counts(), g.grainmap())
+    g.close()
+    return out
+
+
+def _assert_same_as_alone(alone, outcome):
+    r, (dc, rc), gmap = alone
+    assert outcome["status"] == abi.OK
+    for k in ("t", "X", "xcells", "step", "completed", "sum_sv", "nuclei_placed", "pmax"):
+        assert r[k] == outcome["result"][k], k
+    assert np.array_equal(dc, outcome["defg_counts"]) and np.array_equal(rc, outcome["rxg_counts"])
+    assert np.array_equal(gmap, outcome["grainmap"])
+
+
+def test_server_results_same_alone_and_in_the_mix(require_gpu):
+    """Requests of uneven size arriving while others run give the same results
+    as each domain run alone, with a roomy and with a tight memory budget."""
+    import time
+    from score_amd.growth import Server, footprint
+    doms = [synth.make_voronoi_domain(e, e - 4, e + 6, n_grains=max(2, e ** 3 // 2000), n_nuclei=5 + e // 4,
+                                      seed=40 + i) for i, e in enumerate((24, 64, 40, 80, 32, 56, 48))]
+    prms = [ParamsSpec(sched_temp=(900.0 + 30 * i, 900.0 + 30 * i)) for i in range(len(doms))]
+    alone = [_alone(d, p) for d, p in zip(doms, prms)]
+    for budget in (0, max(footprint(d, p) for d, p in zip(doms, prms)) + 1):
+        with Server(max_inflight=4, hbm_budget=budget) as srv:
+            reqs = []
+            for i, (d, p) in enumerate(zip(doms, prms)):
+                reqs.append(srv.submit(d, p, grainmap=True))
+                if i % 2:
+                    time.sleep(0.01)
+            for r in reqs:
+                assert srv.wait(r, timeout=120.0)
+        for a, r in zip(alone, reqs):
+            _assert_same_as_alone(a, r.outcome())
+        print("budget", budget, "queued", [round(r.outcome()["queued_seconds"], 4) for r in reqs], flush=True)
+
+
 def test_repeatable(require_gpu):
     dom = synth.workload("voronoi100", seed=4, scale=0.4)
     a = _gm(dom)
