@@ -49,8 +49,9 @@ GM_HD bool gm_odd(double v) { return v - 2.0 * __builtin_floor(v * 0.5) != 0.0; 
  * the binade of f, as one exact multiply-add, then one hardware addition (which
  * leaves the binade, or makes f/u even in a tie).  Counts are integer-valued
  * doubles (exact below 2^53).  Returns the number of additions done, or 0 if f
- * can never change again.  The run never reaches 2^(e+1), so a value below 1.0
- * can only reach 1.0 in the hardware addition that ends the round. */
+ * can never change again.  A run may end exactly on 2^(e+1) (the rounded sum
+ * of its last addition); the round then ends there, so a value below 1.0
+ * reaches 1.0 only as the last addition of a round. */
 GM_HD double gm_radd_round(double &f, double inc, double budget) {
     const uint64_t be = (gm_dbits(f) >> 52) & 0x7FFu;
     /* zero, negative, tiny or huge values and increments that are not positive
@@ -58,10 +59,16 @@ GM_HD double gm_radd_round(double &f, double inc, double budget) {
     const bool plain = !(f > 0.0) || be <= 53u || be >= 0x7FEu || !(inc > 0.0);
     const uint64_t bs = plain ? 1023u : be; /* keeps the bit patterns below valid */
     const double top = gm_dfrom((bs + 1u) << 52);                /* 2^(e+1) */
+    /* the exact sum f + inc stays in the binade; then inc < 2^e and inc/u < 2^52 */
+    const bool inside = !plain && (top - f) > inc;
+    if (!inside) {
+        const double g = f + inc;
+        if (gm_dbits(g) == gm_dbits(f)) return 0.0;
+        f = g;
+        return 1.0;
+    }
     const double u = gm_dfrom((bs - 52u) << 52);                 /* ulp of the binade */
     const double inv_u = gm_dfrom((uint64_t)(2098u - bs) << 52); /* exact power of two */
-    /* inc < 2^e when the sum stays in the binade, so inc/u < 2^52 */
-    const bool inside = !plain && (top - f) > inc;
     const double qd = __builtin_floor(inc * inv_u);
     const double R = inc - qd * u;
     const double F = f * inv_u;
@@ -70,29 +77,27 @@ GM_HD double gm_radd_round(double &f, double inc, double budget) {
     /* D*u is the exact change of one addition inside the binade */
     double D = twoR < u ? qd : qd + 1.0;
     if (tie) D = qd + (gm_odd(qd) ? 1.0 : 0.0);
-    const bool tie_odd = tie && gm_odd(F); /* one hardware step makes f/u even */
-    const double lim = 9007199254740990.0; /* 2^53 - 2 */
-    const double FQ = F + qd;
-    const bool run = inside && !tie_odd && FQ < lim && D > 0.0;
-    /* additions whose exact sum stays below 2^(e+1): F + i*D + Q + 1 <= 2^53 - 1.
-     * num and D are exact and the rounded quotient lies below 2^53, so its floor
-     * is m or m+1. */
-    double m = 0.0;
-    if (run) {
-        const double num = lim - FQ;
-        m = __builtin_floor(num / D);
-        if (m * D > num) m -= 1.0;
+    if (tie && gm_odd(F)) { /* one hardware step makes f/u even */
+        f = f + inc;
+        return 1.0;
     }
-    if (inside && !tie_odd && D == 0.0) return 0.0; /* the increment is absorbed by rounding */
+    if (D == 0.0) return 0.0; /* the increment is absorbed by rounding */
+    /* addition i of the run keeps its exact sum below 2^(e+1) when
+     * F + (i-1)*D + Q + 1 <= 2^53, so the run takes floor(num/D) + 1 additions
+     * with num = 2^53 - 1 - F - Q >= 0 (exact).  The rounded quotient lies below
+     * 2^53, so its floor is the true floor or one more. */
+    const double num = (9007199254740991.0 - F) - qd;
+    double m = __builtin_floor(num / D);
+    if (m * D > num) m -= 1.0;
+    m += 1.0;
     const double step = D * u;
     if (m >= budget) {
         f = f + budget * step;
         return budget;
     }
-    if (m > 0.0) f = f + m * step;
-    const double g = f + inc;
-    if (m == 0.0 && gm_dbits(g) == gm_dbits(f)) return 0.0;
-    f = g;
+    f = f + m * step;
+    if (f == top) return m;
+    f = f + inc;
     return m + 1.0;
 }
 

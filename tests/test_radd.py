@@ -1,8 +1,9 @@
 """The closed-form repeated rounded addition (score_amd/csrc/gm_radd.h) against the
 plain loop f = f + inc, bit for bit, over random and adversarial cases: ties at
 every binade, increments below half an ulp, starts at 0 and at 1.0, and the
-first-reaching-1.0 count that decides a front cell's completion step; increments
-that are zero or negative fall back to single additions."""
+first-reaching-1.0 count that decides a front cell's completion step, sums that
+land exactly on a binade top or on 1.0; increments that are zero or negative fall
+back to single additions."""
 import os
 import shutil
 import subprocess
@@ -46,7 +47,7 @@ int main(int argc, char **argv) {
     int bad = 0;
     for (long k = 0; k < cases && bad < 5; k++) {
         double f0, inc;
-        int kind = (int)(nx() % 7);
+        int kind = (int)(nx() % 9);
         if (kind == 0) { f0 = ur() * 0.01; inc = pow(10.0, -5.0 * ur()) * 0.1; }
         else if (kind == 1) { f0 = 0.0; inc = ur() * 0.1; }
         else if (kind == 2) { f0 = 1.0; inc = ur() * 0.1; }
@@ -65,6 +66,17 @@ int main(int argc, char **argv) {
         } else if (kind == 5) {          /* increments that are not positive, starts at and above 1.0 */
             f0 = (nx() & 1) ? 1.0 + ur() * 0.01 : ur();
             inc = (nx() % 4 == 0) ? 0.0 : -ur() * 0.01;
+        } else if (kind == 6) {          /* exact sums: runs that end on a binade top and on 1.0 itself */
+            int p = 8 + (int)(nx() % 30);
+            f0 = ldexp((double)(nx() % (1ull << (p - 2))), -p);
+            inc = ldexp((double)(1 + nx() % 7), -p);
+        } else if (kind == 7) {          /* a run whose last sum rounds up onto the binade top, 1.0 included */
+            int e = -(int)(nx() % 12);
+            double u = ldexp(1.0, e - 53);
+            double q = (double)(1 + nx() % (1u << 20));
+            inc = q * u + u * (0.5 + 0.5 * ur());
+            double F0 = 9007199254740991.0 - q - (double)(nx() % 50) * (q + 1.0);
+            f0 = F0 * u;
         } else {                         /* carries and increments as the growth produces them */
             f0 = ur() * 0.0068 * ur();
             inc = 0.1 * ur() * (0.1 + 0.9 * ur()) * (0.6494 + 0.3506 * ur());
