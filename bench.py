@@ -61,9 +61,10 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
         "k_sweep": (4 * scanned + 16 * tiles, "4 B per occupied slot + 16 B per tile"),
         "k_compact": (16 * tiles + 4 * full, "16 B per tile + 4 B per FullRX entry"),
         "k_recycle": (4 * pf["n_recycled"], "4 B per recycled slot handed out"),
-        "k_propose": (216 * full + 4 * att, "216 B per completed cell (26 cell-grid entries 104, record 64 read, "
-                                           "parent data 48 written) + 4 B per valid attempt"),
-        "k_decide": (104 * full + 4 * att + 8 * full, "104 B per completed cell + 4 B per valid attempt + 8 B masks"),
+        "k_propose": (220 * full + 4 * att, "220 B per completed cell (26 cell-grid entries 104, record 64 read, "
+                                           "parent data 48 and attempt mask 4 written) + 4 B per valid attempt"),
+        "k_decide": (28 * full + 4 * att, "28 B per completed cell (position 16, attempt mask 4, winner masks 8) "
+                                          "+ 4 B per valid attempt (claim)"),
         "k_commit": (34 * new + 42 * full, "34 B per new record + 42 B per completed cell (record 30, retire 12)"),
     }
     return table.get(name, (0, "scan of per-block counts"))

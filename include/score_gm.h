@@ -131,7 +131,7 @@ typedef struct score_gm_result {
     int32_t completed;            /* 1 once the termination test of SCORE_Kernel.cpp:6619 fired */
     uint64_t sum_sv;              /* sum of Sv over all steps (cell updates) */
     uint32_t nuclei_placed;       /* nmynuclei */
-    uint32_t reserved;
+    uint32_t windows;             /* launch windows the steps ran in (one host round trip each) */
     double pmax;                  /* myMobilityWeightMax as tracked by the host */
     double loop_seconds;          /* host wall time spent inside step/run calls */
 } score_gm_result;

@@ -114,7 +114,7 @@ class Result(C.Structure):
         ("t", C.c_double), ("X", C.c_double), ("xcells", C.c_double),
         ("step", C.c_uint32), ("completed", C.c_int32),
         ("sum_sv", C.c_uint64),
-        ("nuclei_placed", C.c_uint32), ("reserved", C.c_uint32),
+        ("nuclei_placed", C.c_uint32), ("windows", C.c_uint32),
         ("pmax", C.c_double), ("loop_seconds", C.c_double),
     ]
 

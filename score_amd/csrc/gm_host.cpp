@@ -220,6 +220,7 @@ struct score_gm_handle {
     bool have_last_sp = false;
     uint32_t nnew = 0; /* cells infected in the previous step */
     uint32_t defrag_cnt = 0;     /* defragmentations done (reg_defrag_cnt) */
+    uint32_t windows = 0;        /* launch windows run */
     bool force_reanchor = false; /* records moved: every cell is predicted again */
     /* deformed grains by descending rho and whether they still own cells */
     std::vector<double> rho_host;
@@ -567,8 +568,84 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     return 0;
 }
 
-/* one iteration of SCORE's main loop (SCORE_Kernel.cpp:6615-6747) */
-int step_once(score_gm_handle *h) {
+/* the per-step scalars the sweep uses, from the handle's current state */
+GmSweepParams sweep_params(const score_gm_handle *h) {
+    GmSweepParams sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.model = h->p.mobility_model;
+    sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
+    sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
+    return sp;
+}
+
+/* t += dt and the scalar updates of the master section (SCORE_Kernel.cpp:6725-6734) */
+void advance_scalars(score_gm_handle *h) {
+    h->t += h->dt;
+    update_temperature(h);
+    update_atomistic(h);
+    h->pzmin = 0.0;
+    update_mobilities(h);
+    h->dt = dt_minimum(h);
+}
+
+/* Steps that may follow the current one in a launch window: the termination
+ * tests on t and the step count must not fire and the scalars must stay
+ * bit-identical, found by running the scalar updates ahead on a copy; and a
+ * cell infected in the window must not be able to complete before its
+ * deferred prediction at the window's end.  Only the Rollett-Holm model
+ * qualifies: its time step does not depend on the weights of new pairs. */
+uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &sp) {
+    if (h->prof || wmax <= 1 || h->p.mobility_model != SCORE_GM_MOB_ROLLETTHOLM) return 1;
+    /* a new cell starts at most at 0.0682 times the largest increment and takes
+     * at least imin additions to reach 1.0; the increment is at most maxfill
+     * because the time step is bounded by maxfill*dx/vmax and P <= 1 */
+    const double incmax = h->p.maxfill * (1.0 + 1e-9);
+    const double imin = ceil((1.0 - 0.0682 * incmax) / incmax);
+    uint32_t w = std::min<uint32_t>(wmax, GM_WMAX);
+    if (imin - 1.0 < (double)w) w = imin >= 2.0 ? (uint32_t)(imin - 1.0) : 1u;
+    if (w <= 1) return 1;
+    const double t0 = h->t, T0 = h->T, G0 = h->G, b0 = h->b, gb0 = h->Gbhalfsq, dt0 = h->dt, pz0 = h->pzmin;
+    const double m0 = h->mLAGB, m1 = h->mHAGB, m2 = h->mGS, m3 = h->mRHHAGB;
+    const uint32_t step0 = h->step;
+    uint32_t n = 1;
+    while (n < w) {
+        advance_scalars(h);
+        h->step++;
+        if (done(h) || !same_scalars(sweep_params(h), sp)) break;
+        n++;
+    }
+    h->t = t0; h->T = T0; h->G = G0; h->b = b0; h->Gbhalfsq = gb0; h->dt = dt0; h->pzmin = pz0;
+    h->mLAGB = m0; h->mHAGB = m1; h->mGS = m2; h->mRHHAGB = m3;
+    h->step = step0;
+    return n;
+}
+
+/* the smallest Xcells at which the host acts: X > XMAX, Xcells >= N, or the next
+ * defragmentation threshold */
+double window_xstop(const score_gm_handle *h) {
+    const double N = (double)h->d.N;
+    double c = floor(h->p.xmax * N);
+    if (c < 0.0) c = 0.0;
+    while (c > 0.0 && (c / N) > h->p.xmax) c -= 1.0;
+    while (!((c / N) > h->p.xmax) && c <= N) c += 1.0;
+    double x = std::min(c, N);
+    if (h->defrag_cnt < h->defrag_at.size()) x = std::min(x, h->defrag_at[h->defrag_cnt]);
+    return x;
+}
+
+int step_error(uint32_t err) {
+    if (err & GM_ERRF_BORDER_FULL)
+        return fail(SCORE_GM_ERR_CAPACITY, "Border front list full (SCORE would write out of bounds)");
+    if (err & GM_ERRF_INSIDE_FULL) return fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
+    if (err & GM_ERRF_MISSED) return fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
+    return fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
+}
+
+/* Iterations of SCORE's main loop (SCORE_Kernel.cpp:6615-6747), up to wmax of
+ * them launched back to back as one window.  Returns 1 when the termination
+ * test fired before any step, 0 after *nsteps >= 1 steps, < 0 on error. */
+int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
+    *nsteps = 0;
     if (h->err) return h->err;
     if (h->completed) return 1;
     if (done(h)) { h->completed = 1; return 1; }
@@ -590,39 +667,56 @@ int step_once(score_gm_handle *h) {
     if (rc) return h->err = rc;
     GmSweepArgs a;
     memset(&a, 0, sizeof(a));
-    GmSweepParams &sp = a.sp;
-    sp.model = h->p.mobility_model;
-    sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
-    sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
+    a.sp = sweep_params(h);
+    const GmSweepParams sp = a.sp;
     a.k = h->step;
     a.reanchor = (h->force_reanchor || !h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
-    a.nI = h->nI;
-    a.nB = h->nB;
     a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_CS_MAXSTEP - 1);
-    a.nnew = h->nnew;
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
     /* infection keys are p*32 + rank in 32 bits: at most 2^27 completed cells per step */
     if (slots0 >= (1u << 27))
         return h->err = fail(SCORE_GM_ERR_UNSUPPORTED, "front larger than 2^27 slots (infection keys are 32-bit)");
+    const uint32_t W = window_length(h, wmax, sp);
+    /* grids: exact for one step, with room for growth over a window (the kernels
+     * take the lengths from the device and cover any grid) */
+    const uint32_t nIe = W == 1 ? h->nI : (uint32_t)std::min<uint64_t>(d.capI, h->nI + h->nI / 2 + 4096u);
+    const uint32_t nBe = W == 1 ? h->nB : (uint32_t)std::min<uint64_t>(d.capB, h->nB + h->nB / 2 + 4096u);
+    a.nI = nIe;
+    a.nB = nBe;
+    a.nnew = W == 1 ? h->nnew : std::max<uint32_t>(h->nnew + h->nnew / 2, 4096u);
+    GmWindow w;
+    memset(&w, 0, sizeof(w));
+    w.xstop = window_xstop(h);
+    w.top_grain = h->rho_top < h->rho_order.size() ? h->rho_order[h->rho_top] : 0xFFFFFFFFu;
+    w.pend_limit = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, (uint64_t)d.hcap / 2 > h->hcount ? (uint64_t)d.hcap / 2 - h->hcount : 0ull);
+    w.defer = W > 1 ? 1u : 0u;
     hipEvent_t *ev = prof ? h->ev : nullptr;
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
-    if ((e = gm_launch_sweep(d, a, h->stream, ev)) != hipSuccess ||
-        (e = gm_launch_infect(d, h->pp, a, slots0, h->stream, ev)) != hipSuccess ||
-        (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+    if ((e = gm_launch_window_begin(d, h->step, h->nnew, h->Xcells, h->stream)) != hipSuccess)
+        return h->err = hip_fail(e, "growth step");
+    for (uint32_t j = 0; j < W; j++) {
+        GmSweepArgs aj = a;
+        if (j) aj.reanchor = 0;
+        if ((e = gm_launch_sweep(d, aj, h->stream, ev)) != hipSuccess ||
+            (e = gm_launch_infect(d, h->pp, aj, std::max(slots0, nIe + nBe), h->stream, ev)) != hipSuccess ||
+            (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
+            return h->err = hip_fail(e, "growth step");
+    }
+    if ((e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
     h->last_sp = sp;
     h->have_last_sp = true;
     h->force_reanchor = false;
     GmStats st = *h->hst;
-    st.nfull = st.nfullI + st.nfullB;
-    st.sv = slots0 - (st.totRecI + st.totRecB);
-    /* increments computed for the listed new cells (a re-anchoring step counts its cells separately) */
-    const uint32_t npredicted = a.reanchor ? 0u : std::min(a.nnew, d.newCap);
-    h->nnew = st.winI + st.winB;
+    const uint32_t nexec = st.nexec;
+    if (st.err) return h->err = step_error(st.err);
+    if (nexec < 1 || nexec > W) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: window step count");
     if (prof) {
+        /* one step per window when profiling */
+        const GmStepRow &r0 = st.rows[0];
         score_gm_profile &pf = h->profc;
         for (int i = 0; i < GM_NK; i++) {
             float ms;
@@ -633,85 +727,103 @@ int step_once(score_gm_handle *h) {
         float ms;
         if ((e = hipEventElapsedTime(&ms, h->ev[0], h->ev[GM_NK])) != hipSuccess)
             return h->err = hip_fail(e, "hipEventElapsedTime");
+        const uint32_t sv0 = slots0 - (r0.totRecI + r0.totRecB);
         pf.ms_step += ms;
         pf.steps++;
-        pf.sum_sv += st.sv;
+        pf.sum_sv += sv0;
         pf.slots_scanned += slots0;
-        pf.n_full += st.nfull;
-        pf.n_recycled += (uint64_t)std::min(st.winI, st.recLenI) + std::min(st.winB, st.recLenB);
-        pf.n_attempts += st.nattempt;
-        pf.n_new += (uint64_t)st.winI + st.winB;
-        pf.n_predicted += npredicted;
+        pf.n_full += (uint64_t)r0.nfullI + r0.nfullB;
+        pf.n_recycled += (uint64_t)std::min(r0.winI, r0.recLenI) + std::min(r0.winB, r0.recLenB);
+        pf.n_attempts += r0.nattempt;
+        pf.n_new += (uint64_t)r0.winI + r0.winB;
+        /* increments computed for the listed new cells (a re-anchoring step counts its cells separately) */
+        pf.n_predicted += a.reanchor ? 0u : std::min(h->nnew, d.newCap);
         if (a.reanchor) {
             pf.n_reanchor_steps++;
-            pf.n_reanchored += st.sv;
+            pf.n_reanchored += sv0;
         }
-        pf.pending_resolved += st.npend;
+        pf.pending_resolved += r0.npend;
     }
-    if (st.err) {
-        if (st.err & GM_ERRF_BORDER_FULL)
-            return h->err = fail(SCORE_GM_ERR_CAPACITY, "Border front list full (SCORE would write out of bounds)");
-        if (st.err & GM_ERRF_INSIDE_FULL) return h->err = fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
-        if (st.err & GM_ERRF_MISSED) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
-        return h->err = fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
-    }
+    /* the window's new pairs, evaluated with glibc and stored in the device cache */
     if ((rc = resolve_pending(h, st))) return h->err = rc;
     if (st.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
-    h->nI = st.nI;
-    h->nB = st.nB;
-    if ((rc = grow_pcache(h))) return h->err = rc;
-    /* master section (SCORE_Kernel.cpp:6688-6742) */
-    score_gm_traj_row r;
-    memset(&r, 0, sizeof(r));
-    r.step = h->step;
-    r.t = h->t;
-    r.dt = h->dt;
-    r.temperature = h->T;
-    double mymobmax = kDefaultPmax;
-    if (h->regPmax >= mymobmax) mymobmax = h->regPmax;
-    h->Pmax = mymobmax;
-    h->Xcells = h->Xcells + ((double)st.nfull);
-    h->X = (h->Xcells / ((double)d.N));
-    h->sum_sv += st.sv;
-    if (st.nzeroed) {
-        /* grains that lost their last cell leave the rhomax search */
-        const uint32_t *ids = st.zeroed_ids;
-        if (st.nzeroed > GM_ZEROED_INLINE) {
-            h->zbuf.resize(st.nzeroed);
-            if ((e = hipMemcpy(h->zbuf.data(), d.zlist, sizeof(uint32_t) * st.nzeroed, hipMemcpyDeviceToHost)) !=
-                hipSuccess)
-                return h->err = hip_fail(e, "exhausted grains");
-            ids = h->zbuf.data();
+    /* master section of every step (SCORE_Kernel.cpp:6688-6742) */
+    uint32_t prev_slots = slots0;
+    for (uint32_t j = 0; j < nexec; j++) {
+        const GmStepRow &row = st.rows[j];
+        const uint32_t nfull = row.nfullI + row.nfullB;
+        const uint32_t sv = prev_slots - (row.totRecI + row.totRecB);
+        prev_slots = row.nI + row.nB;
+        score_gm_traj_row r;
+        memset(&r, 0, sizeof(r));
+        r.step = h->step;
+        r.t = h->t;
+        r.dt = h->dt;
+        r.temperature = h->T;
+        double mymobmax = kDefaultPmax;
+        if (h->regPmax >= mymobmax) mymobmax = h->regPmax;
+        h->Pmax = mymobmax;
+        h->Xcells = h->Xcells + ((double)nfull);
+        h->X = (h->Xcells / ((double)d.N));
+        h->sum_sv += sv;
+        if (j + 1 == nexec) {
+            /* grains that lost their last cell leave the rhomax search; before the
+             * last step of a window none of them was the one that sets rhomax */
+            if (st.nzeroed) {
+                const uint32_t *ids = st.zeroed_ids;
+                if (st.nzeroed > GM_ZEROED_INLINE) {
+                    h->zbuf.resize(st.nzeroed);
+                    if ((e = hipMemcpy(h->zbuf.data(), d.zlist, sizeof(uint32_t) * std::min(st.nzeroed, d.ndefg),
+                                       hipMemcpyDeviceToHost)) != hipSuccess)
+                        return h->err = hip_fail(e, "exhausted grains");
+                    ids = h->zbuf.data();
+                }
+                for (uint32_t i = 0; i < std::min(st.nzeroed, d.ndefg); i++) h->alive[ids[i]] = 0;
+            }
+            if (st.cwrap) {
+                /* a count went below zero and wrapped: cellcount > 0 again, as in SCORE */
+                std::vector<uint32_t> cnt(d.ndefg);
+                if ((e = hipMemcpy(cnt.data(), d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost)) !=
+                    hipSuccess)
+                    return h->err = hip_fail(e, "deformed grain counts");
+                for (uint32_t g = 0; g < d.ndefg; g++) h->alive[g] = cnt[g] > 0 ? 1 : 0;
+                h->rho_top = 0;
+            }
         }
-        for (uint32_t i = 0; i < st.nzeroed; i++) h->alive[ids[i]] = 0;
+        update_rhomax(h);
+        r.sv = sv;
+        r.nfull_inside = row.nfullI;
+        r.nfull_border = row.nfullB;
+        r.nslots_inside = row.nI;
+        r.nslots_border = row.nB;
+        r.ninfected = row.winI + row.winB;
+        r.npending = row.npend;
+        r.X = h->X;
+        r.xcells = h->Xcells;
+        r.rhomax = h->rhomax;
+        advance_scalars(h);
+        h->step++;
+        h->traj.push_back(r);
+        if (j + 1 < nexec && !same_scalars(sweep_params(h), sp))
+            return h->err = fail(SCORE_GM_ERR_STATE, "internal error: scalars changed inside a launch window");
     }
-    if (st.cwrap) {
-        /* a count went below zero and wrapped: cellcount > 0 again, as in SCORE */
-        std::vector<uint32_t> cnt(d.ndefg);
-        if ((e = hipMemcpy(cnt.data(), d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost)) != hipSuccess)
-            return h->err = hip_fail(e, "deformed grain counts");
-        for (uint32_t g = 0; g < d.ndefg; g++) h->alive[g] = cnt[g] > 0 ? 1 : 0;
-        h->rho_top = 0;
+    const GmStepRow &last = st.rows[nexec - 1];
+    h->nI = last.nI;
+    h->nB = last.nB;
+    h->nnew = last.winI + last.winB;
+    if ((rc = grow_pcache(h))) return h->err = rc;
+    if (st.ndeferred) {
+        /* cells that waited for a weight: predicted under the window's scalars */
+        GmSweepArgs ad = a;
+        ad.sp = sp;
+        ad.k = h->step;
+        ad.nI = h->nI;
+        ad.nB = h->nB;
+        if ((e = gm_launch_predict_deferred(d, ad, h->stream)) != hipSuccess)
+            return h->err = hip_fail(e, "deferred predictions");
     }
-    update_rhomax(h);
-    r.sv = st.sv;
-    r.nfull_inside = st.nfullI;
-    r.nfull_border = st.nfull - st.nfullI;
-    r.nslots_inside = st.nI;
-    r.nslots_border = st.nB;
-    r.ninfected = st.winI + st.winB;
-    r.npending = st.npend;
-    r.X = h->X;
-    r.xcells = h->Xcells;
-    r.rhomax = h->rhomax;
-    h->t += h->dt;
-    update_temperature(h);
-    update_atomistic(h);
-    h->pzmin = 0.0;
-    update_mobilities(h);
-    h->dt = dt_minimum(h);
-    h->step++;
-    h->traj.push_back(r);
+    *nsteps = nexec;
+    h->windows++;
     if (prof) h->profc.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     return 0;
 }
@@ -938,9 +1050,11 @@ int score_gm_step(score_gm_handle *h, uint32_t max_steps, int32_t *completed) {
     if (hipSetDevice(h->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
     auto t0 = std::chrono::steady_clock::now();
     int rc = 0;
-    for (uint32_t k = 0; k < max_steps; k++) {
-        rc = step_once(h);
+    for (uint32_t k = 0; k < max_steps;) {
+        uint32_t n = 0;
+        rc = step_window(h, max_steps - k, &n);
         if (rc != 0) break;
+        k += n;
     }
     h->loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (completed) *completed = h->completed;
@@ -957,6 +1071,7 @@ int score_gm_get_result(const score_gm_handle *h, score_gm_result *res) {
     res->completed = h->completed;
     res->sum_sv = h->sum_sv;
     res->nuclei_placed = h->nuclei;
+    res->windows = h->windows;
     res->pmax = h->Pmax;
     res->loop_seconds = h->loop_seconds;
     return h->err;

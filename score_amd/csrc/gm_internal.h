@@ -42,6 +42,7 @@
 #define GM_CS_INACTIVE 0xFFFFFFFFu
 #define GM_CS_NEVER 0xFFFFFFFEu   /* active, does not reach rxFrac >= 1 within the run */
 #define GM_CS_UNKNOWN 0xFFFFFFFDu /* infected in the previous step, increment not yet known */
+#define GM_CS_DEFERRED 0xFFFFFFFCu /* mobility weight still pending: predicted at the end of the window */
 #define GM_CS_MAXSTEP 0xFFFFFFF0u
 
 /* error flags raised on the device */
@@ -56,6 +57,7 @@
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
 #define GM_PEND_INLINE 64         /* new P-cache keys returned with the per-step stats */
 #define GM_NK 10                  /* kernels per step (SCORE_GM_K_* order) */
+#define GM_WMAX 8                 /* steps per launch window */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
  * anchor + increment + predicted final value.  The first 32-byte sector holds
@@ -83,6 +85,20 @@ struct __attribute__((aligned(16))) GmPar {
 };
 static_assert(sizeof(GmPar) == 48, "GmPar is three 16-byte quarters");
 
+/* what one step of a window leaves for the host's master section */
+struct GmStepRow {
+    uint32_t nI, nB;          /* list lengths after the step */
+    uint32_t nfullI, nfullB;
+    uint32_t totRecI, totRecB;
+    uint32_t recLenI, recLenB;
+    uint32_t winI, winB;
+    uint32_t npend;           /* P-cache entries the step inserted */
+    uint32_t nattempt;
+    uint32_t nzeroed;         /* exhausted grains so far in the window */
+    uint32_t err;
+    uint32_t pad[2];
+};
+
 struct GmStats {
     uint32_t nI, nB;          /* nextSlotNeverActive of both lists */
     uint32_t sv;              /* ACTIVE slots seen by the sweep */
@@ -103,6 +119,24 @@ struct GmStats {
     uint32_t dcount;          /* defragmentation: ACTIVE slots moved; status vector: entries */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
+    /* launch windows: several steps run back to back; the host reads the rows at the end */
+    uint32_t kcur;            /* index of the next step */
+    uint32_t nnew;            /* cells infected in the last step, listed in newl */
+    uint32_t halt;            /* set after a step that needs the host: later steps of the window do nothing */
+    uint32_t nexec;           /* steps executed in the window */
+    uint32_t ndeferred;       /* cells whose prediction waits for a pending weight */
+    uint32_t npend0;          /* npend before the current step */
+    double xcells;            /* Xcells, advanced by the device within a window */
+    GmStepRow rows[GM_WMAX];
+};
+
+/* stop conditions of a window, checked after every step */
+struct GmWindow {
+    double xstop;             /* Xcells at which the host must act (termination, defragmentation) */
+    uint32_t top_grain;       /* the grain that sets rhomax (0xFFFFFFFF: none) */
+    uint32_t pend_limit;      /* new P-cache entries that keep the cache at most half full */
+    uint32_t defer;           /* pending weights may wait for the end of the window */
+    uint32_t pad;
 };
 
 struct GmDev {
@@ -158,13 +192,16 @@ struct GmSweepParams {
     double Gbhalfsq, zener, dt, inv_dx;
 };
 
+/* The step index, list lengths and new-cell count live in GmStats (kcur, nI,
+ * nB, nnew) so that a window of steps runs without the host; nI, nB and nnew
+ * here only size the grids (upper estimates for the window). */
 struct GmSweepArgs {
     GmSweepParams sp;
-    uint32_t k;                 /* step index */
+    uint32_t k;                 /* step index (kernels read st->kcur) */
     uint32_t reanchor;          /* scalars changed since the previous step */
-    uint32_t nI, nB;            /* occupied slots */
+    uint32_t nI, nB;            /* occupied slots, for the grid sizes */
     uint32_t stepcap;           /* last step a completion can fall in */
-    uint32_t nnew;              /* cells infected in the previous step (listed in newl) */
+    uint32_t nnew;              /* cells infected in the previous step, for the grid size */
 };
 
 /* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
@@ -172,6 +209,15 @@ struct GmSweepArgs {
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
 hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                             hipStream_t s, hipEvent_t *ev);
+/* start of a window: step index, cells infected in the previous step, Xcells;
+ * clears the window's accumulators */
+hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, double xcells, hipStream_t s);
+/* end of every step of a window: records the step's row, advances the step
+ * index, and sets halt when the host must act before the next step */
+hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s);
+/* cells left DEFERRED in a window, with their weights now resolved: replayed to
+ * step a.k under the window's scalars and predicted */
+hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
 /* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */

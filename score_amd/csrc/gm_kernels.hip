@@ -273,23 +273,40 @@ static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const Gm
     return cs;
 }
 
-/* a cell infected in the previous step (anchor at step k): resolve its weight and predict */
+/* a cell infected in the previous step (anchor at step k): resolve its weight and
+ * predict.  A weight the host has not evaluated yet (a new pair met inside a
+ * window of steps) leaves the cell DEFERRED: it cannot complete before the
+ * window ends, and is predicted then. */
 static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmSweepArgs &A, uint32_t s) {
     GmRec *r = d.rec + s;
     const RecA a = rec_a(r);
     const RecC c = rec_c(r);
     const double P = resolved(d, c.P);
+    if (is_pending(P)) {
+        d.cstep[s] = GM_CS_DEFERRED;
+        atomicAdd(&d.st->ndeferred, 1u);
+        return GM_CS_DEFERRED;
+    }
     if (is_pending(c.P)) r->P = P;
     return predict_slot(d, A, s, a.frac0, a.s0, P, c.defg, a.inf);
+}
+
+/* the arguments of the step the device is at */
+static __device__ __forceinline__ GmSweepArgs step_args(const GmDev &d, const GmSweepArgs &A) {
+    GmSweepArgs B = A;
+    B.k = d.st->kcur;
+    return B;
 }
 
 /* The per-step scalars changed: every active cell is advanced to the start of
  * step k with its old increment (its anchor moves to k) and predicted again
  * with the new one.  One thread per occupied slot. */
-__global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A) {
-    const uint32_t n = A.nI + A.nB;
+__global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A0) {
+    if (d.st->halt) return;
+    const GmSweepArgs A = step_args(d, A0);
+    const uint32_t nI = d.st->nI, n = nI + d.st->nB;
     for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
-        const uint32_t s = i < A.nI ? i : d.capI + (i - A.nI);
+        const uint32_t s = i < nI ? i : d.capI + (i - nI);
         const uint32_t cs = d.cstep[s];
         if (cs == GM_CS_INACTIVE) continue;
         GmRec *r = d.rec + s;
@@ -305,9 +322,19 @@ __global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A) {
 
 /* cells infected in the previous step, listed by the commit pass, predicted
  * with every lane of a wave busy */
-__global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A, uint32_t n) {
+__global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A0) {
+    if (d.st->halt) return;
+    const GmSweepArgs A = step_args(d, A0);
+    const uint32_t n = d.st->nnew < d.newCap ? d.st->nnew : d.newCap;
     for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK)
         predict_new(d, A, d.newl[i]);
+}
+
+/* list lengths of the current step as tiles: the Inside list rounded up to
+ * whole 256-slot chunks (tI), then the Border list (T in all) */
+static __device__ __forceinline__ void step_tiles(uint32_t nI, uint32_t nB, uint32_t &tI, uint32_t &T) {
+    tI = ((nI + 63) / 64 + 3u) & ~3u;
+    T = tI + (nB + 63) / 64;
 }
 
 /* per-block list counts: [0] full Inside, [1] recycled Inside, [2] full Border,
@@ -341,8 +368,13 @@ static __device__ __forceinline__ uint32_t tile_slot0(const GmDev &d, uint32_t t
  * per-tile masks and counted per block.  The Inside list spans tI tiles (a
  * multiple of four), the Border list follows from tile tI. */
 #define GM_SW_UNROLL 4
-__global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint32_t tI, uint32_t T) {
+__global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
     __shared__ uint32_t red[GM_BLOCK / 64][5];
+    if (d.st->halt) return;
+    const GmSweepArgs A = step_args(d, A0);
+    const uint32_t nI = d.st->nI, nB = d.st->nB;
+    uint32_t tI, T;
+    step_tiles(nI, nB, tI, T);
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t lo, hi;
     part_of(T, GM_BLOCK / 64, lo, hi); /* multiples of four tiles, except an end at T */
@@ -358,7 +390,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint
             const uint32_t tile0 = c * 4u;
             base[u] = tile_slot0(d, tile0, tI) + 4u * lane;
             /* slots below lim are occupied and inside this block's tiles */
-            const uint32_t lend = tile0 < tI ? A.nI : d.capI + A.nB;
+            const uint32_t lend = tile0 < tI ? nI : d.capI + nB;
             const uint32_t tend = tile_slot0(d, tile0, tI) + 64u * (hi - tile0 < 4u ? hi - tile0 : 4u);
             lim[u] = c < c_hi ? (lend < tend ? lend : tend) : 0u;
             cs[u] = base[u] < lim[u] ? *reinterpret_cast<const uint4 *>(d.cstep + base[u])
@@ -378,7 +410,8 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A, uint
                     if (x == GM_CS_INACTIVE) pack |= 16u << q;
                     else if (x == A.k) pack |= 1u << q;
                     /* every cell is predicted before the sweep (k_predict_new or
-                     * k_reanchor): a past or missing prediction is an internal error */
+                     * k_reanchor) or DEFERRED (above every step): a past or missing
+                     * prediction is an internal error */
                     else if (x < A.k || x == GM_CS_UNKNOWN) unknown++;
                 }
             }
@@ -462,10 +495,10 @@ static __device__ __forceinline__ void scan_block_counts(uint32_t *bcnt, uint32_
     }
 }
 
-/* pass 2: per-block offsets and the list lengths; also resets the counters the
- * infection phase accumulates.  Cells the sweep met without a valid prediction
- * raise GM_ERRF_MISSED. */
+/* pass 2: per-block offsets and the list lengths.  Cells the sweep met without
+ * a valid prediction raise GM_ERRF_MISSED. */
 __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
+    if (d.st->halt) return;
     uint32_t tot[5];
     scan_block_counts<5>(d.bcnt, nb, tot);
     if (threadIdx.x == 0) {
@@ -477,10 +510,6 @@ __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
         st->totRecB = tot[BC_RB];
         st->recLenI = tot[BC_RI] < d.recCapI ? tot[BC_RI] : d.recCapI;
         st->recLenB = tot[BC_RB] < d.recCapB ? tot[BC_RB] : d.recCapB;
-        st->npend = 0;
-        st->pone = 0;
-        st->nzeroed = 0;
-        st->cwrap = 0;
     }
 }
 
@@ -506,8 +535,11 @@ static __device__ __forceinline__ void tile_offsets(const GmDev &d, uint32_t tI,
  * ordered compaction: the completed slots in ascending order, Inside then
  * Border, in one array.  One wave per sweep block, one lane per tile: the lane
  * writes its tile's completed slots at its offset. */
-__global__ void __launch_bounds__(64) k_compact(GmDev d, uint32_t tI, uint32_t T) {
+__global__ void __launch_bounds__(64) k_compact(GmDev d) {
+    if (d.st->halt) return;
     const uint32_t lane = threadIdx.x;
+    uint32_t tI, T;
+    step_tiles(d.st->nI, d.st->nB, tI, T);
     uint32_t lo, hi;
     part_of(T, GM_BLOCK / 64, lo, hi);
     uint32_t run[4];
@@ -529,9 +561,13 @@ __global__ void __launch_bounds__(64) k_compact(GmDev d, uint32_t tI, uint32_t T
  * recycled slots in ascending order, of which only the first min(winners, cap)
  * are ever handed out; written after the winners are counted, so each sweep
  * block writes only the part of its INACTIVE slots that falls below that. */
-__global__ void __launch_bounds__(64) k_recycle(GmDev d, uint32_t tI, uint32_t T) {
-    const uint32_t lane = threadIdx.x;
+__global__ void __launch_bounds__(64) k_recycle(GmDev d) {
     const GmStats *st = d.st;
+    if (st->halt) return;
+    const uint32_t lane = threadIdx.x;
+    /* the sweep's tiles and partition: the lengths before this step's appends */
+    uint32_t tI, T;
+    step_tiles(st->nIold, st->nBold, tI, T);
     const uint32_t limI = st->winI < st->recLenI ? st->winI : st->recLenI;
     const uint32_t limB = st->winB < st->recLenB ? st->winB : st->recLenB;
     uint32_t run[4];
@@ -567,6 +603,12 @@ static inline hipError_t mark(hipEvent_t *ev, int i, hipStream_t s) {
     return ev ? hipEventRecord(ev[i], s) : hipSuccess;
 }
 
+/* grid of the sweep, compaction and recycling passes for lists of about nI, nB */
+static uint32_t sweep_blocks(uint32_t nI, uint32_t nB) {
+    const uint32_t tI = ((nI + 63) / 64 + 3u) & ~3u, T = tI + (nB + 63) / 64;
+    return grid_for(T, 4 * (GM_BLOCK / 64));
+}
+
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
     GmSweepArgs A = a;
@@ -574,19 +616,14 @@ hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, 
     if ((e = mark(ev, 0, s)) != hipSuccess) return e;
     if (a.reanchor) k_reanchor<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
     if ((e = mark(ev, 1, s)) != hipSuccess) return e;
-    if (!a.reanchor && a.nnew) {
-        const uint32_t n = a.nnew < d.newCap ? a.nnew : d.newCap;
-        k_predict_new<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, A, n);
-    }
-    /* Inside tiles rounded up to whole 256-slot chunks */
-    const uint32_t tI = ((a.nI + 63) / 64 + 3u) & ~3u, T = tI + (a.nB + 63) / 64;
-    const uint32_t nb = grid_for(T, 4 * (GM_BLOCK / 64));
+    if (!a.reanchor) k_predict_new<<<grid_for(a.nnew < d.newCap ? a.nnew : d.newCap, GM_BLOCK), GM_BLOCK, 0, s>>>(d, A);
+    const uint32_t nb = sweep_blocks(a.nI, a.nB);
     if ((e = mark(ev, 2, s)) != hipSuccess) return e;
-    k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A, tI, T);
+    k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A);
     if ((e = mark(ev, 3, s)) != hipSuccess) return e;
     k_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
     if ((e = mark(ev, 4, s)) != hipSuccess) return e;
-    k_compact<<<nb, 64, 0, s>>>(d, tI, T);
+    k_compact<<<nb, 64, 0, s>>>(d);
     return hipGetLastError();
 }
 
@@ -666,21 +703,24 @@ static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, u
     return x + y * d.nx + z * d.nxy;
 }
 
-/* Parents are handled by groups of eight lanes; lane l of a group makes the
- * attempts of rank l, l+8, l+16 and l+24 (< 26), so the four bitmap reads of a
- * lane are independent and a wave keeps eight parents in flight. */
-#define GM_PG 8                       /* lanes per parent */
-#define GM_PPB (GM_BLOCK / GM_PG)     /* parents per block and round */
+/* Parents are cut into chunks of GM_PCHUNK for the decide and commit passes, so
+ * that a 64-lane wave takes a whole chunk. */
+#define GM_PCHUNK 64
+#define GM_PG 8                       /* lanes per parent in the propose pass */
 
-/* The parent data in FullRX order (lane 0 of a group copies what the infection
- * needs of the completed cell into the dense parent array, read by the later
- * passes), then all attempts on deformed cells claim their target: the
- * smallest key p*32+r is the serial winner. */
+/* Eight lanes per parent: lane 0 copies what the infection needs of the
+ * completed cell from its record into the dense parent array (FullRX order);
+ * lane l makes the attempts of rank l, l+8, l+16 and l+24 (< 26), so the four
+ * bitmap reads of a lane are independent and a wave keeps eight parents in
+ * flight.  Every attempt on a deformed cell claims its target (the smallest key
+ * p*32+r is the serial winner); their mask over the ranks is kept in pmI for
+ * the decide pass. */
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
+    if (d.st->halt) return;
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
     const uint32_t l8 = threadIdx.x & (GM_PG - 1);
-    const uint32_t ngroups = gridDim.x * GM_PPB;
-    for (uint32_t p = blockIdx.x * GM_PPB + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
+    const uint32_t ngroups = gridDim.x * (GM_BLOCK / GM_PG);
+    for (uint32_t p = blockIdx.x * (GM_BLOCK / GM_PG) + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
         uint4 q2 = make_uint4(0u, 0u, 0u, 0u);
         if (l8 == 0) {
             const uint32_t s = d.pslot[p];
@@ -700,71 +740,72 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
         q2.y = (uint32_t)__shfl((int)q2.y, 0, GM_PG);
         q2.z = (uint32_t)__shfl((int)q2.z, 0, GM_PG);
         q2.w = (uint32_t)__shfl((int)q2.w, 0, GM_PG);
+        const bool fromB = q2.w != 0u;
+        const uint32_t gsh = (threadIdx.x & 63) & ~(GM_PG - 1); /* first lane of the group in the wave */
+        uint32_t am = 0;
 #pragma unroll
         for (uint32_t i = 0; i < 4; i++) {
-            const uint32_t r = l8 + GM_PG * i;
-            if (r >= 26) break;
-            uint32_t tx, ty, tz;
-            bool toB;
-            if (!attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)r, tx, ty, tz, toB)) continue;
-            if (is_deformed(d, tx, ty, tz)) atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + r);
+            const uint32_t rk = l8 + GM_PG * i;
+            bool att = false;
+            if (rk < 26) {
+                uint32_t tx, ty, tz;
+                bool toB;
+                if (attempt_target(d, fromB, q2.x, q2.y, q2.z, (int)rk, tx, ty, tz, toB) && is_deformed(d, tx, ty, tz)) {
+                    att = true;
+                    atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + rk);
+                }
+            }
+            am |= (uint32_t)((__ballot(att) >> gsh) & 0xFFu) << (GM_PG * i);
         }
+        if (l8 == 0) d.pmI[p] = am;
     }
 }
 
-/* winners per parent and destination list, as 26-bit masks over the ranks;
- * per-block totals for the ranking scan (and the attempts on deformed cells) */
-__global__ void __launch_bounds__(GM_BLOCK) k_decide(GmDev d) {
-    __shared__ uint32_t red[GM_PPB][3];
+/* One lane per parent: which of its attempts won, as 26-bit masks per
+ * destination list; per-block totals for the ranking scan and the attempts.
+ * The partition into blocks is the commit pass's. */
+__global__ void __launch_bounds__(64) k_decide(GmDev d) {
+    if (d.st->halt) return;
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
-    const uint32_t l8 = threadIdx.x & (GM_PG - 1), grp = threadIdx.x / GM_PG;
-    const uint32_t gsh = (threadIdx.x & 63) & ~(GM_PG - 1); /* first lane of the group in the wave */
     uint32_t lo, hi;
-    part_of(nfull, GM_PPB, lo, hi);
+    part_of(nfull, GM_PCHUNK, lo, hi);
     uint32_t cI = 0, cB = 0, cA = 0;
-    for (uint32_t p = lo + grp; p < hi; p += GM_PPB) {
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += 64) {
         const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
-        uint32_t mI = 0, mB = 0, nA = 0;
-#pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
-            const uint32_t r = l8 + GM_PG * i;
-            bool wI = false, wB = false;
-            if (r < 26) {
-                uint32_t tx, ty, tz;
-                bool toB;
-                if (attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)r, tx, ty, tz, toB) &&
-                    is_deformed(d, tx, ty, tz)) {
-                    nA++;
-                    if (d.claim[cell_of(d, tx, ty, tz)] == p * 32u + r) {
-                        wB = toB;
-                        wI = !toB;
-                    }
-                }
+        const uint32_t am = d.pmI[p];
+        uint32_t mI = 0, mB = 0;
+        for (uint32_t m = am; m; m &= m - 1u) {
+            const uint32_t rk = (uint32_t)__ffs(m) - 1u;
+            uint32_t tx, ty, tz;
+            bool toB;
+            attempt_target(d, q2.w != 0u, q2.x, q2.y, q2.z, (int)rk, tx, ty, tz, toB);
+            if (d.claim[cell_of(d, tx, ty, tz)] == p * 32u + rk) {
+                if (toB) mB |= 1u << rk;
+                else mI |= 1u << rk;
             }
-            mI |= (uint32_t)((__ballot(wI) >> gsh) & 0xFFu) << (GM_PG * i);
-            mB |= (uint32_t)((__ballot(wB) >> gsh) & 0xFFu) << (GM_PG * i);
         }
-        for (int o = GM_PG / 2; o > 0; o >>= 1) nA += __shfl_down(nA, o, GM_PG);
-        if (l8 == 0) {
-            d.pmI[p] = mI;
-            d.pmB[p] = mB;
-            cI += (uint32_t)__popc(mI);
-            cB += (uint32_t)__popc(mB);
-            cA += nA;
-        }
+        d.pmI[p] = mI;
+        d.pmB[p] = mB;
+        cI += (uint32_t)__popc(mI);
+        cB += (uint32_t)__popc(mB);
+        cA += (uint32_t)__popc(am);
     }
-    if (l8 == 0) { red[grp][0] = cI; red[grp][1] = cB; red[grp][2] = cA; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        uint32_t t = 0;
-        for (int i = 0; i < GM_PPB; i++) t += red[i][threadIdx.x];
-        d.bcnt[(BC_WI + threadIdx.x) * GM_MAX_BLOCKS + blockIdx.x] = t;
+    for (int o = 32; o > 0; o >>= 1) {
+        cI += __shfl_down(cI, o, 64);
+        cB += __shfl_down(cB, o, 64);
+        cA += __shfl_down(cA, o, 64);
+    }
+    if (threadIdx.x == 0) {
+        d.bcnt[BC_WI * GM_MAX_BLOCKS + blockIdx.x] = cI;
+        d.bcnt[BC_WB * GM_MAX_BLOCKS + blockIdx.x] = cB;
+        d.bcnt[BC_WA * GM_MAX_BLOCKS + blockIdx.x] = cA;
     }
 }
 
 /* one block: offsets of every block's winners in each list, the new list lengths
  * (recycled slots first, then appended) */
 __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
+    if (d.st->halt) return;
     uint32_t tot[3];
     scan_block_counts<3>(d.bcnt + BC_WI * GM_MAX_BLOCKS, nb, tot);
     if (threadIdx.x == 0) {
@@ -796,16 +837,18 @@ static __device__ __forceinline__ void rec_write_new(GmRec *r, double frac0, uin
  * takes the block's parents (the partition of k_decide) 64 at a time; their
  * winners are flattened into a list so that every lane commits one infection.
  * The slots of the new cells are listed for the next step's prediction. */
-__global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k) {
+__global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
     __shared__ uint32_t sI[64], sB[64], oI[64], oB[64];
     __shared__ uint16_t wl[26 * 64];
     GmStats *st = d.st;
+    if (st->halt) return;
+    const uint32_t k = st->kcur;
     const uint32_t nfull = st->nfullI + st->nfullB;
     const uint32_t recLenI = st->recLenI, recLenB = st->recLenB;
     const uint32_t nI = st->nIold, nB = st->nBold, winI = st->winI;
     const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
-    part_of(nfull, GM_PPB, lo, hi);
+    part_of(nfull, GM_PCHUNK, lo, hi);
     uint32_t runI = d.bcnt[BC_WI * GM_MAX_BLOCKS + blockIdx.x], runB = d.bcnt[BC_WB * GM_MAX_BLOCKS + blockIdx.x];
     for (uint32_t batch = lo; batch < hi; batch += 64) {
         const uint32_t pt = batch + lane;
@@ -889,7 +932,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k
                 if (old == n) {
                     const uint32_t i = atomicAdd(&st->nzeroed, 1u);
                     if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
-                    d.zlist[i] = g;
+                    if (i < d.ndefg) d.zlist[i] = g;
                 } else if (old < n) {
                     /* only a cell held by two records retires twice; the count
                      * wraps as SCORE's uint32 does and the host re-reads them all */
@@ -908,29 +951,21 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, uint32_t k
     }
 }
 
-static unsigned parent_blocks(uint32_t nupper) {
-    uint32_t b = (nupper + GM_PPB - 1) / GM_PPB;
-    if (b > 8192) b = 8192;
-    if (b < 1) b = 1;
-    return b;
-}
-
 hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                             hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
     if ((e = mark(ev, 5, s)) != hipSuccess) return e;
-    k_propose<<<parent_blocks(parents_upper), GM_BLOCK, 0, s>>>(d);
-    const uint32_t nb = grid_for(parents_upper, GM_PPB);
+    k_propose<<<grid_for(parents_upper, GM_BLOCK / GM_PG), GM_BLOCK, 0, s>>>(d);
+    const uint32_t nb = grid_for(parents_upper, GM_PCHUNK);
     if ((e = mark(ev, 6, s)) != hipSuccess) return e;
-    k_decide<<<nb, GM_BLOCK, 0, s>>>(d);
+    k_decide<<<nb, 64, 0, s>>>(d);
     if ((e = mark(ev, 7, s)) != hipSuccess) return e;
     k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
-    /* the sweep's tiles and partition, for the recycling lists */
-    const uint32_t tI = ((a.nI + 63) / 64 + 3u) & ~3u, T = tI + (a.nB + 63) / 64;
+    /* the sweep's grid, for the recycling lists */
     if ((e = mark(ev, 8, s)) != hipSuccess) return e;
-    k_recycle<<<grid_for(T, 4 * (GM_BLOCK / 64)), 64, 0, s>>>(d, tI, T);
+    k_recycle<<<sweep_blocks(a.nI, a.nB), 64, 0, s>>>(d);
     if ((e = mark(ev, 9, s)) != hipSuccess) return e;
-    k_commit<<<nb, 64, 0, s>>>(d, pp, a.k);
+    k_commit<<<nb, 64, 0, s>>>(d, pp);
     if ((e = mark(ev, 10, s)) != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -1377,5 +1412,110 @@ hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k
     k_status_count<<<nb, GM_BLOCK, 0, s>>>(d, nI, n);
     k_scan_row<<<1, 1024, 0, s>>>(d, nb);
     k_status_write<<<nb, GM_BLOCK, 0, s>>>(d, nI, n, k, thr, ixyz, gid);
+    return hipGetLastError();
+}
+
+/* ----------------------------------------------------------- launch windows
+ * Up to GM_WMAX steps are launched back to back.  Every kernel of a step reads
+ * the step index and list lengths from the stats and returns at once once a
+ * step has set `halt`, so that the host takes over at the step after it. */
+
+__global__ void k_window_begin(GmDev d, uint32_t k, uint32_t nnew, double xcells) {
+    GmStats *st = d.st;
+    st->kcur = k;
+    st->nnew = nnew;
+    st->halt = 0;
+    st->nexec = 0;
+    st->ndeferred = 0;
+    st->npend = 0;
+    st->npend0 = 0;
+    st->pone = 0;
+    st->nzeroed = 0;
+    st->cwrap = 0;
+    st->xcells = xcells;
+}
+
+hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, double xcells, hipStream_t s) {
+    k_window_begin<<<1, 1, 0, s>>>(d, k, nnew, xcells);
+    return hipGetLastError();
+}
+
+/* The row of the step just done, and whether the next one may run without the
+ * host: not after an error, a count that wrapped, the exhaustion of the grain
+ * that sets rhomax, Xcells reaching a threshold the host acts on, a weight the
+ * window cannot wait for, or lists outgrowing their allocations. */
+__global__ void __launch_bounds__(64) k_step_tail(GmDev d, GmWindow w) {
+    GmStats *st = d.st;
+    if (st->halt) return;
+    const uint32_t j = st->nexec;
+    const uint32_t z0 = j ? st->rows[j - 1].nzeroed : 0u, z1 = st->nzeroed < d.ndefg ? st->nzeroed : d.ndefg;
+    /* the grains exhausted in this step, one lane each */
+    bool top = false;
+    for (uint32_t i = z0 + threadIdx.x; i < z1; i += 64) top |= d.zlist[i] == w.top_grain;
+    const bool any_top = __ballot(top) != 0ull;
+    if (threadIdx.x != 0) return;
+    GmStepRow r;
+    r.nI = st->nI;
+    r.nB = st->nB;
+    r.nfullI = st->nfullI;
+    r.nfullB = st->nfullB;
+    r.totRecI = st->totRecI;
+    r.totRecB = st->totRecB;
+    r.recLenI = st->recLenI;
+    r.recLenB = st->recLenB;
+    r.winI = st->winI;
+    r.winB = st->winB;
+    r.npend = st->npend - st->npend0;
+    r.nattempt = st->nattempt;
+    r.nzeroed = z1;
+    r.err = st->err;
+    r.pad[0] = r.pad[1] = 0u;
+    st->rows[j] = r;
+    st->nexec = j + 1;
+    st->kcur = st->kcur + 1u;
+    st->nnew = r.winI + r.winB;
+    st->npend0 = st->npend;
+    const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
+    st->xcells = xc;
+    const uint64_t slots = (uint64_t)r.nI + r.nB;
+    const uint64_t tiles = (uint64_t)(((r.nI + 63) / 64 + 3u) & ~3u) + (r.nB + 63) / 64 + 4u;
+    const uint64_t newmax = 26ull * slots < d.N ? 26ull * slots : d.N;
+    bool halt = r.err != 0u || st->cwrap != 0u || any_top || xc >= w.xstop;
+    halt = halt || (st->npend != 0u && !w.defer) || st->npend > w.pend_limit;
+    halt = halt || slots > d.parentCap || tiles > d.tileCap || newmax > d.newCap || slots >= (1ull << 27);
+    st->halt = halt ? 1u : 0u;
+}
+
+hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s) {
+    k_step_tail<<<1, 64, 0, s>>>(d, w);
+    return hipGetLastError();
+}
+
+/* DEFERRED cells: anchored at the step after their infection with their carry;
+ * every step since then added the increment of the window's scalars, so the
+ * value before step a.k is replayed from the anchor, then predicted. */
+__global__ void __launch_bounds__(GM_BLOCK) k_predict_deferred(GmDev d, GmSweepArgs A) {
+    const uint32_t nI = d.st->nI, n = nI + d.st->nB;
+    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
+        const uint32_t s = i < nI ? i : d.capI + (i - nI);
+        if (d.cstep[s] != GM_CS_DEFERRED) continue;
+        GmRec *r = d.rec + s;
+        const RecA a = rec_a(r);
+        const RecC c = rec_c(r);
+        const double P = resolved(d, c.P);
+        if (is_pending(P)) {
+            atomicOr(&d.st->err, GM_ERRF_MISSED);
+            continue;
+        }
+        r->P = P;
+        double inc, f = a.frac0;
+        if (cell_increment(d, A.sp, P, c.defg, a.inf, inc)) f = gm_radd_n(f, inc, A.k - a.s0);
+        predict_slot(d, A, s, f, A.k, P, c.defg, a.inf);
+    }
+}
+
+hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, hipStream_t s) {
+    if (a.nI + a.nB == 0) return hipSuccess;
+    k_predict_deferred<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
     return hipGetLastError();
 }

@@ -91,6 +91,27 @@ def test_tiny_stepwise(require_gpu):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("seed,chunks", [(3, (3, 8, 5, 7, 2)), (4, (8,)), (5, (6, 1, 4))])
+def test_launch_windows_match_at_every_boundary(require_gpu, seed, chunks):
+    """Several steps per launch window (the kernels take the step index and list
+    lengths from the device, weights of new pairs are resolved at the window's
+    end): the fronts agree with the oracle wherever the host takes over."""
+    dom = synth.make_voronoi_domain(48, 40, 32, n_grains=9, n_nuclei=25, seed=seed)
+    o, g = Oracle(dom, ParamsSpec()), _gm(dom, ParamsSpec())
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+
+
 def test_small_stepwise_sg(require_gpu):
     dom = synth.make_voronoi_domain(30, 30, 30, n_grains=8, n_nuclei=30, seed=5)
     prm = ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN)
