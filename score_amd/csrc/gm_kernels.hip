@@ -831,6 +831,25 @@ static __device__ __forceinline__ void rec_write_new(GmRec *r, double frac0, uin
     *reinterpret_cast<uint4 *>(&r->rxg) = make_uint4(rxg, 0u, 0u, 0u);
 }
 
+/* The new record of a won attempt of rank r on target t = (tx, ty, tz) in slot
+ * gslot: the carry of the parent's overshoot by the direction's class
+ * (updateFullRX*, SCORE_Kernel.cpp:4568-4599), the parent's weight unless the
+ * target lies in another deformed grain (:4297-4302), anchored at step k+1;
+ * the cell leaves the deformed bitmap. */
+static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParams &pp, uint32_t k, const GmPar *pr,
+                                                  const uint4 &pq0, uint32_t r, uint32_t t, uint32_t tx, uint32_t ty,
+                                                  uint32_t tz, uint32_t gslot) {
+    const double overshoot = pr->fin - 1.0;
+    const int cls = c_cls[r];
+    const double carry = cls == 1 ? overshoot * 0.0682 : (cls == 2 ? overshoot * 0.0542 : overshoot * 0.0484);
+    const uint32_t g = d.grid[t];
+    double Pn = pr->P;
+    if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
+    rec_write_new(d.rec + gslot, carry, k + 1u, c_code[r], Pn, t, g, pq0.w);
+    d.cstep[gslot] = GM_CS_UNKNOWN;
+    clear_deformed(d, tx, ty, tz);
+}
+
 /* New records as omp_infect_OutOfRXFront* write them (SCORE_Kernel.cpp:4281-4310,
  * :4402-4426), slots as omp_getNextFreeSlotIn* hand them out (:3757-3874); then
  * the parents retire as updateFullRX* does it (:4640-4653).  One wave per block
@@ -896,15 +915,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
             }
             if (gslot == GM_CS_INACTIVE) continue;
             if (nidx < d.newCap) d.newl[nidx] = gslot;
-            const double overshoot = pr->fin - 1.0;
-            const int cls = c_cls[r];
-            const double carry = cls == 1 ? overshoot * 0.0682 : (cls == 2 ? overshoot * 0.0542 : overshoot * 0.0484);
-            const uint32_t g = d.grid[t];
-            double Pn = pr->P;
-            if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
-            rec_write_new(d.rec + gslot, carry, k + 1u, c_code[r], Pn, t, g, pq0.w);
-            d.cstep[gslot] = GM_CS_UNKNOWN;
-            clear_deformed(d, tx, ty, tz);
+            infect_one(d, pp, k, pr, pq0, r, t, tx, ty, tz, gslot);
         }
         /* retire the batch's parents; a deformed grain losing its last cell is
          * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165).
@@ -1442,17 +1453,44 @@ hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, dou
 
 /* The row of the step just done, and whether the next one may run without the
  * host: not after an error, a count that wrapped, the exhaustion of the grain
- * that sets rhomax, Xcells reaching a threshold the host acts on, a weight the
- * window cannot wait for, or lists outgrowing their allocations. */
+ * that sets rhomax (any_top), Xcells reaching a threshold the host acts on, a
+ * weight the window cannot wait for, or lists outgrowing their allocations.
+ * One thread. */
+static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmStepRow &r, bool any_top) {
+    GmStats *st = d.st;
+    const uint32_t j = st->nexec;
+    st->rows[j] = r;
+    st->nexec = j + 1;
+    st->kcur = st->kcur + 1u;
+    st->nnew = r.winI + r.winB;
+    const uint32_t npend = __hip_atomic_load(&st->npend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->npend0 = npend;
+    const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
+    st->xcells = xc;
+    const uint64_t slots = (uint64_t)r.nI + r.nB;
+    const uint64_t tiles = (uint64_t)(((r.nI + 63) / 64 + 3u) & ~3u) + (r.nB + 63) / 64 + 4u;
+    const uint64_t newmax = 26ull * slots < d.N ? 26ull * slots : d.N;
+    bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                any_top || xc >= w.xstop;
+    halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;
+    halt = halt || slots > d.parentCap || tiles > d.tileCap || newmax > d.newCap || slots >= (1ull << 27);
+    st->halt = halt ? 1u : 0u;
+}
+
+/* whether one of the grains exhausted in this step (zlist[z0, z1)) sets rhomax;
+ * every lane of the calling wave takes part */
+static __device__ __forceinline__ bool top_exhausted(const GmDev &d, const GmWindow &w, uint32_t z0, uint32_t z1) {
+    bool top = false;
+    for (uint32_t i = z0 + (threadIdx.x & 63u); i < z1; i += 64) top |= d.zlist[i] == w.top_grain;
+    return __ballot(top) != 0ull;
+}
+
 __global__ void __launch_bounds__(64) k_step_tail(GmDev d, GmWindow w) {
     GmStats *st = d.st;
     if (st->halt) return;
     const uint32_t j = st->nexec;
     const uint32_t z0 = j ? st->rows[j - 1].nzeroed : 0u, z1 = st->nzeroed < d.ndefg ? st->nzeroed : d.ndefg;
-    /* the grains exhausted in this step, one lane each */
-    bool top = false;
-    for (uint32_t i = z0 + threadIdx.x; i < z1; i += 64) top |= d.zlist[i] == w.top_grain;
-    const bool any_top = __ballot(top) != 0ull;
+    const bool any_top = top_exhausted(d, w, z0, z1);
     if (threadIdx.x != 0) return;
     GmStepRow r;
     r.nI = st->nI;
@@ -1470,20 +1508,7 @@ __global__ void __launch_bounds__(64) k_step_tail(GmDev d, GmWindow w) {
     r.nzeroed = z1;
     r.err = st->err;
     r.pad[0] = r.pad[1] = 0u;
-    st->rows[j] = r;
-    st->nexec = j + 1;
-    st->kcur = st->kcur + 1u;
-    st->nnew = r.winI + r.winB;
-    st->npend0 = st->npend;
-    const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
-    st->xcells = xc;
-    const uint64_t slots = (uint64_t)r.nI + r.nB;
-    const uint64_t tiles = (uint64_t)(((r.nI + 63) / 64 + 3u) & ~3u) + (r.nB + 63) / 64 + 4u;
-    const uint64_t newmax = 26ull * slots < d.N ? 26ull * slots : d.N;
-    bool halt = r.err != 0u || st->cwrap != 0u || any_top || xc >= w.xstop;
-    halt = halt || (st->npend != 0u && !w.defer) || st->npend > w.pend_limit;
-    halt = halt || slots > d.parentCap || tiles > d.tileCap || newmax > d.newCap || slots >= (1ull << 27);
-    st->halt = halt ? 1u : 0u;
+    step_finish(d, w, r, any_top);
 }
 
 hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s) {
