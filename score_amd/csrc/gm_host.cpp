@@ -232,22 +232,26 @@ struct score_gm_handle {
 
 namespace {
 
+/* Device memory comes from the device's stream-ordered pool on the handle's
+ * stream: a free does not wait for the whole device (as hipFree does), so
+ * handles created and destroyed next to running ones do not stall them, and
+ * freed blocks are reused by later handles. */
 template <typename T>
-int dalloc(T **p, size_t n) {
+int dalloc(score_gm_handle *h, T **p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    hipError_t e = hipMallocAsync((void **)p, n * sizeof(T), h->stream);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
-        g_last_error = std::string("hipMalloc: ") + hipGetErrorString(e);
+        g_last_error = std::string("hipMallocAsync: ") + hipGetErrorString(e);
         return SCORE_GM_ERR_OOM;
     }
     return 0;
 }
 
-void dfree(void *p) {
-    if (p) (void)hipFree(p);
+void dfree(score_gm_handle *h, void *p) {
+    if (p) (void)hipFreeAsync(p, h->stream);
 }
 
 void free_device(score_gm_handle *h) {
@@ -256,7 +260,7 @@ void free_device(score_gm_handle *h) {
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, h->d_pend_vals};
-    for (void *q : ptrs) dfree(q);
+    for (void *q : ptrs) dfree(h, q);
     memset(&h->d, 0, sizeof(h->d));
     h->d_pend_vals = nullptr;
     if (h->hst) (void)hipHostFree(h->hst);
@@ -404,22 +408,22 @@ int ensure_step_capacity(score_gm_handle *h) {
     const uint32_t tiles = (((h->nI + 63) / 64 + 3u) & ~3u) + (h->nB + 63) / 64 + 4u;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
-        dfree(d.mF); dfree(d.mR);
+        dfree(h, d.mF); dfree(h, d.mR);
         d.mF = d.mR = nullptr;
         int rc;
-        if ((rc = dalloc(&d.mF, nc)) || (rc = dalloc(&d.mR, nc))) return rc;
+        if ((rc = dalloc(h, &d.mF, nc)) || (rc = dalloc(h, &d.mR, nc))) return rc;
         d.tileCap = nc;
     }
     const uint32_t parents = h->nI + h->nB;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
-        dfree(d.pmI); dfree(d.pmB); dfree(d.pslot); dfree(d.par);
+        dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par);
         d.pmI = d.pmB = d.pslot = nullptr;
         d.par = nullptr;
         d.parentCap = 0;
         int rc;
-        if ((rc = dalloc(&d.pmI, nc)) || (rc = dalloc(&d.pmB, nc)) || (rc = dalloc(&d.pslot, nc)) ||
-            (rc = dalloc(&d.par, nc)))
+        if ((rc = dalloc(h, &d.pmI, nc)) || (rc = dalloc(h, &d.pmB, nc)) || (rc = dalloc(h, &d.pslot, nc)) ||
+            (rc = dalloc(h, &d.par, nc)))
             return rc;
         d.parentCap = nc;
     }
@@ -428,13 +432,13 @@ int ensure_step_capacity(score_gm_handle *h) {
     if (newmax > d.newCap) {
         uint32_t nc = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(newmax, d.newCap + d.newCap / 2));
         uint32_t *nl = nullptr;
-        int rc = dalloc(&nl, nc);
+        int rc = dalloc(h, &nl, nc);
         if (rc) return rc;
         /* the previous step's list moves along (it is predicted in this step) */
         const uint32_t keep = std::min(h->nnew, d.newCap);
         if (keep) HIPCHK(hipMemcpyAsync(nl, d.newl, sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
-        dfree(d.newl);
+        dfree(h, d.newl);
         d.newl = nl;
         d.newCap = nc;
     }
@@ -458,12 +462,12 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
     if (npend > h->pend_vals_cap) {
         /* both staging buffers are idle here: the stream was synchronised after the step */
         const uint32_t nc = std::max<uint32_t>(std::max<uint32_t>(npend, 2 * h->pend_vals_cap), 1024);
-        dfree(h->d_pend_vals);
+        dfree(h, h->d_pend_vals);
         h->d_pend_vals = nullptr;
         if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
         h->hpend_vals = nullptr;
         h->pend_vals_cap = 0;
-        int rc = dalloc(&h->d_pend_vals, nc);
+        int rc = dalloc(h, &h->d_pend_vals, nc);
         if (rc) return rc;
         if (hipHostMalloc((void **)&h->hpend_vals, sizeof(double) * nc, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
@@ -504,24 +508,24 @@ int grow_pcache(score_gm_handle *h) {
     for (uint32_t i = 0; i < d.hcap; i++)
         if (keys[i]) { k2.push_back(keys[i]); v2.push_back(vals[i]); }
     const uint32_t ncap = d.hcap * 2u;
-    dfree(d.hkeys); dfree(d.hvals); dfree(d.pend_slot); dfree(d.pend_key);
+    dfree(h, d.hkeys); dfree(h, d.hvals); dfree(h, d.pend_slot); dfree(h, d.pend_key);
     d.hkeys = nullptr; d.hvals = nullptr; d.pend_slot = nullptr; d.pend_key = nullptr;
     int rc;
-    if ((rc = dalloc(&d.hkeys, ncap)) || (rc = dalloc(&d.hvals, ncap)) || (rc = dalloc(&d.pend_slot, ncap)) ||
-        (rc = dalloc(&d.pend_key, ncap)))
+    if ((rc = dalloc(h, &d.hkeys, ncap)) || (rc = dalloc(h, &d.hvals, ncap)) || (rc = dalloc(h, &d.pend_slot, ncap)) ||
+        (rc = dalloc(h, &d.pend_key, ncap)))
         return rc;
     d.hcap = ncap;
     HIPCHK(gm_launch_fill_u64(d.hkeys, 0ull, ncap, h->stream));
     HIPCHK(gm_launch_fill_u64((unsigned long long *)d.hvals, GM_PENDING_TAG, ncap, h->stream));
     unsigned long long *dk = nullptr;
     double *dv = nullptr;
-    if ((rc = dalloc(&dk, k2.size())) || (rc = dalloc(&dv, v2.size()))) { dfree(dk); return rc; }
+    if ((rc = dalloc(h, &dk, k2.size())) || (rc = dalloc(h, &dv, v2.size()))) { dfree(h, dk); return rc; }
     HIPCHK(hipMemcpyAsync(dk, k2.data(), sizeof(unsigned long long) * k2.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(dv, v2.data(), sizeof(double) * v2.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(gm_launch_hash_insert(d, dk, dv, (uint32_t)k2.size(), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(dk);
-    dfree(dv);
+    dfree(h, dk);
+    dfree(h, dv);
     return 0;
 }
 
@@ -550,20 +554,20 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     GmRec *trec = nullptr;
     uint32_t *tcs = nullptr;
     int rc;
-    if ((rc = dalloc(&trec, n)) || (rc = dalloc(&tcs, n))) {
-        dfree(trec);
+    if ((rc = dalloc(h, &trec, n)) || (rc = dalloc(h, &tcs, n))) {
+        dfree(h, trec);
         return rc;
     }
     hipError_t e;
     if ((e = gm_launch_defrag_move(d, base, first, nnever, h->step, trec, tcs, s)) != hipSuccess ||
         (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess) {
-        dfree(trec);
-        dfree(tcs);
+        dfree(h, trec);
+        dfree(h, tcs);
         return hip_fail(e, "defragmentation");
     }
-    dfree(trec);
-    dfree(tcs);
+    dfree(h, trec);
+    dfree(h, tcs);
     nnever = first + h->hst->dcount;
     return 0;
 }
@@ -878,6 +882,15 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(SCORE_GM_ERR_NODEVICE, "device is not gfx950");
     HIPCHK(hipSetDevice(device));
+    {
+        /* freed handle memory stays in the device's pool for later handles */
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
+    }
 
     score_gm_handle *h = new (std::nothrow) score_gm_handle();
     if (!h) return fail(SCORE_GM_ERR_OOM, "handle");
@@ -955,20 +968,20 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
     }
-    if ((rc = dalloc(&d.grid, d.N)) || (rc = dalloc(&d.claim, d.N)) || (rc = dalloc(&d.cstep, capTot)) ||
-        (rc = dalloc(&d.rec, capTot)) || (rc = dalloc(&d.newl, d.newCap)) ||
-        (rc = dalloc(&d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(&d.pslot, d.parentCap)) ||
-        (rc = dalloc(&d.par, d.parentCap)) || (rc = dalloc(&d.recI, d.recCapI)) || (rc = dalloc(&d.recB, d.recCapB)) ||
-        (rc = dalloc(&d.mF, d.tileCap)) || (rc = dalloc(&d.mR, d.tileCap)) || (rc = dalloc(&d.pmI, d.parentCap)) ||
-        (rc = dalloc(&d.pmB, d.parentCap)) || (rc = dalloc(&d.bcnt, 8 * GM_MAX_BLOCKS)) ||
-        (rc = dalloc(&d.rho, d.ndefg)) || (rc = dalloc(&d.defq, 4 * (size_t)d.ndefg)) ||
-        (rc = dalloc(&d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(&d.defcnt, d.ndefg)) ||
-        (rc = dalloc(&d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.zlist, d.ndefg)) ||
-        (rc = dalloc(&d.hkeys, d.hcap)) ||
-        (rc = dalloc(&d.hvals, d.hcap)) || (rc = dalloc(&d.pend_slot, d.hcap)) || (rc = dalloc(&d.pend_key, d.hcap)) ||
-        (rc = dalloc(&d.nflag, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.nI_, (size_t)d.nrxg + 1)) ||
-        (rc = dalloc(&d.nB_, (size_t)d.nrxg + 1)) || (rc = dalloc(&d.scan_bsum, 2 * GM_SCAN_BLOCKS)) ||
-        (rc = dalloc(&d.st, 1))) {
+    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.cstep, capTot)) ||
+        (rc = dalloc(h, &d.rec, capTot)) || (rc = dalloc(h, &d.newl, d.newCap)) ||
+        (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
+        (rc = dalloc(h, &d.par, d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||
+        (rc = dalloc(h, &d.mF, d.tileCap)) || (rc = dalloc(h, &d.mR, d.tileCap)) || (rc = dalloc(h, &d.pmI, d.parentCap)) ||
+        (rc = dalloc(h, &d.pmB, d.parentCap)) || (rc = dalloc(h, &d.bcnt, 8 * GM_MAX_BLOCKS)) ||
+        (rc = dalloc(h, &d.rho, d.ndefg)) || (rc = dalloc(h, &d.defq, 4 * (size_t)d.ndefg)) ||
+        (rc = dalloc(h, &d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(h, &d.defcnt, d.ndefg)) ||
+        (rc = dalloc(h, &d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(h, &d.zlist, d.ndefg)) ||
+        (rc = dalloc(h, &d.hkeys, d.hcap)) ||
+        (rc = dalloc(h, &d.hvals, d.hcap)) || (rc = dalloc(h, &d.pend_slot, d.hcap)) || (rc = dalloc(h, &d.pend_key, d.hcap)) ||
+        (rc = dalloc(h, &d.nflag, (size_t)d.nrxg + 1)) || (rc = dalloc(h, &d.nI_, (size_t)d.nrxg + 1)) ||
+        (rc = dalloc(h, &d.nB_, (size_t)d.nrxg + 1)) || (rc = dalloc(h, &d.scan_bsum, 2 * GM_SCAN_BLOCKS)) ||
+        (rc = dalloc(h, &d.st, 1))) {
         free_device(h);
         delete h;
         return rc;
@@ -997,7 +1010,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         HIPCHK(gm_launch_fill_u64((unsigned long long *)d.hvals, GM_PENDING_TAG, d.hcap, s));
         HIPCHK(gm_launch_fill_u32((uint32_t *)d.st, 0u, sizeof(GmStats) / 4, s));
         int r2;
-        if ((r2 = dalloc(&d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(&d_nucP, (size_t)d.nrxg + 1))) return r2;
+        if ((r2 = dalloc(h, &d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(h, &d_nucP, (size_t)d.nrxg + 1))) return r2;
         if (d.nrxg) {
             HIPCHK(hipMemcpyAsync(d_site, dom->rxg_site, sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(d_nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
@@ -1008,9 +1021,9 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         return 0;
     };
     rc = setup();
-    dfree(d_site);
-    dfree(d_nucP);
-    dfree(d.nflag); dfree(d.nI_); dfree(d.nB_);
+    dfree(h, d_site);
+    dfree(h, d_nucP);
+    dfree(h, d.nflag); dfree(h, d.nI_); dfree(h, d.nB_);
     d.nflag = d.nI_ = d.nB_ = nullptr;
     if (rc == 0 && h->hst->err) rc = fail(SCORE_GM_ERR_CAPACITY, "Border front list full while placing nuclei");
     if (rc) {
@@ -1096,7 +1109,7 @@ int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding) {
     HIPCHK(hipSetDevice(h->device));
     /* the grid with the front's cells marked infected, assembled in a device buffer */
     uint32_t *tmp = nullptr;
-    int rc = dalloc(&tmp, h->d.N);
+    int rc = dalloc(h, &tmp, h->d.N);
     if (rc) return rc;
     auto copy = [&]() -> int {
         HIPCHK(hipMemcpyAsync(tmp, h->d.grid, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToDevice, h->stream));
@@ -1106,7 +1119,7 @@ int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding) {
         return 0;
     };
     rc = copy();
-    dfree(tmp);
+    dfree(h, tmp);
     if (rc) return rc;
     if (encoding == SCORE_GM_MAP_RAW2GID)
         for (uint64_t i = 0; i < h->d.N; i++) out[i] = out[i] == GM_INFECTED ? 1u : 2u + out[i];
@@ -1146,7 +1159,7 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
         /* activity and rxFrac as SCORE holds them before the next step, P without pending markers */
         char *buf = nullptr;
         const size_t bytes = (size_t)m * (1 + 1 + 4 + 4 + 4 + 8 + 8);
-        int rc = dalloc(&buf, bytes);
+        int rc = dalloc(h, &buf, bytes);
         if (rc) return rc;
         GmFrontOut o;
         o.frac = reinterpret_cast<double *>(buf);
@@ -1169,7 +1182,7 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
             return 0;
         };
         rc = copy();
-        dfree(buf);
+        dfree(h, buf);
         return rc;
     }
     return 0;
@@ -1183,7 +1196,7 @@ int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, ui
     hipStream_t s = h->stream;
     const uint32_t m = h->nI + h->nB;
     uint32_t *buf = nullptr;
-    int rc = dalloc(&buf, 2ull * std::max<uint32_t>(m, 1u));
+    int rc = dalloc(h, &buf, 2ull * std::max<uint32_t>(m, 1u));
     if (rc) return rc;
     auto run = [&]() -> int {
         HIPCHK(gm_launch_status(h->d, h->nI, h->nB, h->step, rxthreshold, buf, buf + m, s));
@@ -1198,7 +1211,7 @@ int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, ui
         return 0;
     };
     rc = run();
-    dfree(buf);
+    dfree(h, buf);
     return rc;
 }
 
