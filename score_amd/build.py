@@ -54,6 +54,25 @@ def build_synth(out: str | None = None) -> str:
     return out
 
 
+HDF5_PREFIX = os.environ.get("SCORE_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io(out: str | None = None) -> str | None:
+    """libscore_io.so: grain-map files in SCORE's formats (raw, HDF5, XDMF); needs the
+    HDF5 C library (headers and libhdf5 under HDF5_PREFIX)."""
+    out = out or os.path.join(HERE, "libscore_io.so")
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"score_io: no HDF5 headers under {HDF5_PREFIX}; libscore_io.so not built", flush=True)
+        return None
+    cc = shutil.which("gcc") or "cc"
+    tmp = out + ".tmp"
+    _run([cc, "-O2", "-std=c99", "-fPIC", "-shared", "-Wall", "-I", inc, "-I", INCLUDE,
+          os.path.join(CSRC, "score_io.c"), "-o", tmp, "-L", lib, "-lhdf5", "-Wl,-rpath," + lib])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> str:
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "-B"])
@@ -63,6 +82,7 @@ def build_oracle() -> str:
 def build_all(oracle: bool = True) -> None:
     build_synth()
     build_gm()
+    build_io()
     if oracle:
         build_oracle()
 
@@ -70,12 +90,14 @@ def build_all(oracle: bool = True) -> None:
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true", help="also build the oracle (test infrastructure)")
-    ap.add_argument("--only", choices=["gm", "synth", "oracle"], default=None)
+    ap.add_argument("--only", choices=["gm", "synth", "io", "oracle"], default=None)
     a = ap.parse_args(argv)
     if a.only == "gm":
         build_gm()
     elif a.only == "synth":
         build_synth()
+    elif a.only == "io":
+        build_io()
     elif a.only == "oracle":
         build_oracle()
     else:
