@@ -230,6 +230,8 @@ def main(argv=None) -> int:
                     help="bound of the CPU-baseline sample (oracle steps on the same domain)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the profiled run (no roofline)")
+    ap.add_argument("--profile-run-only", action="store_true",
+                    help="only the profiled run (one step per launch window, events between kernels); for rocprofv3")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="ensemble mode: this many independent domains, domains/hour batched vs one at a time")
@@ -276,6 +278,12 @@ def main(argv=None) -> int:
         h = hashlib.sha256(gm.grainmap().tobytes()).hexdigest()[:16]
         gm.close()
         return t2 - t1, res, pf, h
+
+    if a.profile_run_only:
+        psecs, pres, pf, h = one_run(True)
+        log(f"profiled run: {psecs:.3f}s, {pf['steps']} steps, grain map {h}; per-kernel ms "
+            f"{json.dumps({k: round(v, 3) for k, v in pf['ms_kernel'].items()})}")
+        return 0
 
     runs = []
     digest = None

@@ -256,7 +256,7 @@ void dfree(score_gm_handle *h, void *p) {
 
 void free_device(score_gm_handle *h) {
     GmDev &d = h->d;
-    void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot,
+    void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, h->d_pend_vals};
@@ -823,7 +823,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         ad.k = h->step;
         ad.nI = h->nI;
         ad.nB = h->nB;
-        if ((e = gm_launch_predict_deferred(d, ad, h->stream)) != hipSuccess)
+        if ((e = gm_launch_predict_deferred(d, ad, st.ndeferred, h->stream)) != hipSuccess)
             return h->err = hip_fail(e, "deferred predictions");
     }
     *nsteps = nexec;
@@ -958,6 +958,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 256;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
     d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
+    d.dlistCap = (uint32_t)std::min<uint64_t>(d.N, 1u << 20);
     d.dnbx = (d.nx + 15) / 16;
     d.dnby = (d.ny + 7) / 8;
     d.dnbz = (d.nz + 7) / 8;
@@ -970,6 +971,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     }
     if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.cstep, capTot)) ||
         (rc = dalloc(h, &d.rec, capTot)) || (rc = dalloc(h, &d.newl, d.newCap)) ||
+        (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
         (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
         (rc = dalloc(h, &d.par, d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||
         (rc = dalloc(h, &d.mF, d.tileCap)) || (rc = dalloc(h, &d.mR, d.tileCap)) || (rc = dalloc(h, &d.pmI, d.parentCap)) ||
@@ -1317,6 +1319,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     b += (4 + sizeof(GmRec)) * capTot;            /* completion steps, records */
     b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */
     b += 4 * std::min<uint64_t>(N, 26 * front);   /* new-cell list */
+    b += 4 * std::min<uint64_t>(N, 1u << 20);     /* deferred-cell list */
     b += 16 * (capTot / 64 + 4);                  /* tile masks */
     b += 4 * (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt + (double)capB * prm->maxfill * kFullRecyclingAtt);
     b += 4 * 8 * GM_MAX_BLOCKS;

@@ -161,6 +161,8 @@ struct GmDev {
     /* pslot, par, pmI and pmB hold parentCap entries */
     uint32_t *bcnt;             /* per-block counts and offsets, 8 x GM_MAX_BLOCKS */
     uint32_t *newl;             /* global slots of the cells infected in the last step */
+    uint32_t *dlist;            /* slots of the cells left DEFERRED in a window (dlistCap entries) */
+    uint32_t dlistCap;
     uint32_t newCap;
     uint32_t tileCap, parentCap;
     /* grains */
@@ -217,7 +219,7 @@ hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, dou
 hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s);
 /* cells left DEFERRED in a window, with their weights now resolved: replayed to
  * step a.k under the window's scalars and predicted */
-hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
+hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, uint32_t ndeferred, hipStream_t s);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
 /* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */

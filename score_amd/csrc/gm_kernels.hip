@@ -284,7 +284,8 @@ static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmS
     const double P = resolved(d, c.P);
     if (is_pending(P)) {
         d.cstep[s] = GM_CS_DEFERRED;
-        atomicAdd(&d.st->ndeferred, 1u);
+        const uint32_t i = atomicAdd(&d.st->ndeferred, 1u);
+        if (i < d.dlistCap) d.dlist[i] = s;
         return GM_CS_DEFERRED;
     }
     if (is_pending(c.P)) r->P = P;
@@ -1518,11 +1519,15 @@ hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s)
 
 /* DEFERRED cells: anchored at the step after their infection with their carry;
  * every step since then added the increment of the window's scalars, so the
- * value before step a.k is replayed from the anchor, then predicted. */
+ * value before step a.k is replayed from the anchor, then predicted.  The cells
+ * come from the deferred list, or from a scan of every occupied slot when the
+ * list overflowed. */
 __global__ void __launch_bounds__(GM_BLOCK) k_predict_deferred(GmDev d, GmSweepArgs A) {
-    const uint32_t nI = d.st->nI, n = nI + d.st->nB;
+    const uint32_t nd = d.st->ndeferred;
+    const bool listed = nd <= d.dlistCap;
+    const uint32_t nI = d.st->nI, n = listed ? nd : nI + d.st->nB;
     for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
-        const uint32_t s = i < nI ? i : d.capI + (i - nI);
+        const uint32_t s = listed ? d.dlist[i] : (i < nI ? i : d.capI + (i - nI));
         if (d.cstep[s] != GM_CS_DEFERRED) continue;
         GmRec *r = d.rec + s;
         const RecA a = rec_a(r);
@@ -1539,8 +1544,9 @@ __global__ void __launch_bounds__(GM_BLOCK) k_predict_deferred(GmDev d, GmSweepA
     }
 }
 
-hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, hipStream_t s) {
+hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, uint32_t ndeferred, hipStream_t s) {
     if (a.nI + a.nB == 0) return hipSuccess;
-    k_predict_deferred<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
+    const uint32_t n = ndeferred <= d.dlistCap ? ndeferred : a.nI + a.nB;
+    k_predict_deferred<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
     return hipGetLastError();
 }
