@@ -20,7 +20,12 @@
  *
  * Parity of this file against the reference is pinned by tests/test_oracle_*.py
  * (closed forms, brute force over the cubic group, invariants, hand-derived
- * first steps, and the statistics of the reference's shipped example/jmak run).
+ * first steps, stable compaction, and the statistics of the reference's shipped
+ * example/jmak run).  One part is parity unpinned: the bit-level values of the
+ * mobility weight (oracle_mobility_weight), which pass through glibc acos/pow/exp and
+ * for which the reference ships no golden vector; the tests pin its limits,
+ * classes and monotonicity, and the library evaluates the same expressions
+ * with the same glibc.
  */
 #include <math.h>
 #include <stdint.h>
