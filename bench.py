@@ -56,8 +56,6 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
     table = {
         "k_reanchor": (4 * scanned + 100 * pf["n_reanchored"],
                        "4 B per occupied slot + 100 B per re-anchored cell (record 64 read, 32 written, step 4)"),
-        "k_predict_new": (49 * pf["n_predicted"], "49 B per predicted cell (list 4, completion step 4 read and "
-                                                  "written, record 64 read and 32 written, less what is cached)"),
         "k_sweep": (4 * scanned + 16 * tiles, "4 B per occupied slot + 16 B per tile"),
         "k_compact": (16 * tiles + 4 * full, "16 B per tile + 4 B per FullRX entry"),
         "k_recycle": (4 * pf["n_recycled"], "4 B per recycled slot handed out"),
@@ -325,7 +323,7 @@ def main(argv=None) -> int:
         # effective bandwidth of the reference algorithm's bytes per phase
         sweep_ref = 30 * pf["sum_sv"] + (pf["slots_scanned"] - pf["sum_sv"]) + 4 * (pf["n_full"] + pf["n_recycled"])
         infect_ref = (pf["n_full"] * (26 * 4 + 30) + 4 * pf["n_attempts"] + pf["n_new"] * 34 + pf["n_full"] * 12)
-        sweep_ms = sum(kms[k] for k in ("k_reanchor", "k_predict_new", "k_sweep", "k_sweep_scan", "k_compact"))
+        sweep_ms = sum(kms[k] for k in ("k_reanchor", "k_sweep", "k_sweep_scan", "k_compact"))
         infect_ms = sum(kms[k] for k in ("k_propose", "k_decide", "k_decide_scan", "k_commit"))
         roofline = {
             "bound": "hbm", "kernel": dom_k,

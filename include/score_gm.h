@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 5
+#define SCORE_GM_ABI_VERSION 6
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -138,16 +138,15 @@ typedef struct score_gm_result {
 
 /* Kernels of one growth step, in launch order (index into ms_kernel). */
 #define SCORE_GM_K_REANCHOR 0    /* steps whose scalars changed: every cell advanced and predicted again */
-#define SCORE_GM_K_PREDICT 1     /* other steps: increments and completion steps of last step's new cells */
-#define SCORE_GM_K_SWEEP 2       /* calcGrowthStepInside/Border over the occupied slots */
-#define SCORE_GM_K_SWEEP_SCAN 3  /* list offsets of the sweep's blocks */
-#define SCORE_GM_K_COMPACT 4     /* FullRX list */
-#define SCORE_GM_K_PROPOSE 5     /* parent data; every infection attempt claims its target */
-#define SCORE_GM_K_DECIDE 6      /* winners per parent */
-#define SCORE_GM_K_DECIDE_SCAN 7 /* ranks of the winners' blocks */
-#define SCORE_GM_K_RECYCLE 8     /* the recycled slots that are handed out */
-#define SCORE_GM_K_COMMIT 9      /* new front records, retirement of the parents */
-#define SCORE_GM_NKERNELS 10
+#define SCORE_GM_K_SWEEP 1       /* calcGrowthStepInside/Border over the occupied slots */
+#define SCORE_GM_K_SWEEP_SCAN 2  /* list offsets of the sweep's blocks */
+#define SCORE_GM_K_COMPACT 3     /* FullRX list */
+#define SCORE_GM_K_PROPOSE 4     /* parent data; every infection attempt claims its target */
+#define SCORE_GM_K_DECIDE 5      /* winners per parent */
+#define SCORE_GM_K_DECIDE_SCAN 6 /* ranks of the winners' blocks */
+#define SCORE_GM_K_RECYCLE 7     /* the recycled slots that are handed out */
+#define SCORE_GM_K_COMMIT 8      /* new front records with their predicted completion steps, retirement of the parents */
+#define SCORE_GM_NKERNELS 9
 
 /* Per-kernel device time and work counters, the counterpart of SCORE's per-thread
  * phase timers (omp_log_rxfrontstats, SCORE_Kernel.cpp:5019-5063; written to
@@ -165,7 +164,7 @@ typedef struct score_gm_profile {
     uint64_t n_recycled;          /* recycled slots handed out (recycling-list entries written) */
     uint64_t n_attempts;          /* infection attempts that found a deformed target */
     uint64_t n_new;               /* infections committed */
-    uint64_t n_predicted;         /* increments computed and completion steps predicted */
+    uint64_t n_predicted;         /* new cells given their increment and completion step when infected */
     uint64_t n_reanchor_steps;    /* steps whose scalars changed: every cell re-anchored */
     uint64_t n_reanchored;        /* slots re-anchored */
     uint64_t pending_resolved;    /* mobility weights resolved on the host */

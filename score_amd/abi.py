@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK = 0
 ERR_ARG = -1
@@ -130,7 +130,7 @@ class BatchItem(C.Structure):
 
 
 # kernels of one growth step, in launch order (SCORE_GM_K_* in the header)
-KERNELS = ("k_reanchor", "k_predict_new", "k_sweep", "k_sweep_scan", "k_compact", "k_propose", "k_decide",
+KERNELS = ("k_reanchor", "k_sweep", "k_sweep_scan", "k_compact", "k_propose", "k_decide",
            "k_decide_scan", "k_recycle", "k_commit")
 NKERNELS = len(KERNELS)
 

@@ -218,7 +218,6 @@ struct score_gm_handle {
     /* scalars of the previous sweep: a change forces re-anchoring of rxFrac */
     GmSweepParams last_sp{};
     bool have_last_sp = false;
-    uint32_t nnew = 0; /* cells infected in the previous step */
     uint32_t defrag_cnt = 0;     /* defragmentations done (reg_defrag_cnt) */
     uint32_t windows = 0;        /* launch windows run */
     bool force_reanchor = false; /* records moved: every cell is predicted again */
@@ -257,7 +256,7 @@ void dfree(score_gm_handle *h, void *p) {
 void free_device(score_gm_handle *h) {
     GmDev &d = h->d;
     void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
-                    d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.newl, d.rho, d.defq,
+                    d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, h->d_pend_vals};
     for (void *q : ptrs) dfree(h, q);
@@ -426,21 +425,6 @@ int ensure_step_capacity(score_gm_handle *h) {
             (rc = dalloc(h, &d.par, nc)))
             return rc;
         d.parentCap = nc;
-    }
-    /* a step infects distinct deformed cells, at most 26 per completed cell */
-    const uint32_t newmax = (uint32_t)std::min<uint64_t>(d.N, 26ull * parents);
-    if (newmax > d.newCap) {
-        uint32_t nc = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(newmax, d.newCap + d.newCap / 2));
-        uint32_t *nl = nullptr;
-        int rc = dalloc(h, &nl, nc);
-        if (rc) return rc;
-        /* the previous step's list moves along (it is predicted in this step) */
-        const uint32_t keep = std::min(h->nnew, d.newCap);
-        if (keep) HIPCHK(hipMemcpyAsync(nl, d.newl, sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        dfree(h, d.newl);
-        d.newl = nl;
-        d.newCap = nc;
     }
     return 0;
 }
@@ -688,7 +672,6 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     const uint32_t nBe = W == 1 ? h->nB : (uint32_t)std::min<uint64_t>(d.capB, h->nB + h->nB / 2 + 4096u);
     a.nI = nIe;
     a.nB = nBe;
-    a.nnew = W == 1 ? h->nnew : std::max<uint32_t>(h->nnew + h->nnew / 2, 4096u);
     GmWindow w;
     memset(&w, 0, sizeof(w));
     w.xstop = window_xstop(h);
@@ -698,7 +681,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     hipEvent_t *ev = prof ? h->ev : nullptr;
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
-    if ((e = gm_launch_window_begin(d, h->step, h->nnew, h->Xcells, h->stream)) != hipSuccess)
+    if ((e = gm_launch_window_begin(d, h->step, h->Xcells, h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
     for (uint32_t j = 0; j < W; j++) {
         GmSweepArgs aj = a;
@@ -740,8 +723,8 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         pf.n_recycled += (uint64_t)std::min(r0.winI, r0.recLenI) + std::min(r0.winB, r0.recLenB);
         pf.n_attempts += r0.nattempt;
         pf.n_new += (uint64_t)r0.winI + r0.winB;
-        /* increments computed for the listed new cells (a re-anchoring step counts its cells separately) */
-        pf.n_predicted += a.reanchor ? 0u : std::min(h->nnew, d.newCap);
+        /* increments computed when the cells were infected (re-anchored cells are counted separately) */
+        pf.n_predicted += (uint64_t)r0.winI + r0.winB;
         if (a.reanchor) {
             pf.n_reanchor_steps++;
             pf.n_reanchored += sv0;
@@ -814,7 +797,6 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     const GmStepRow &last = st.rows[nexec - 1];
     h->nI = last.nI;
     h->nB = last.nB;
-    h->nnew = last.winI + last.winB;
     if ((rc = grow_pcache(h))) return h->err = rc;
     if (st.ndeferred) {
         /* cells that waited for a weight: predicted under the window's scalars */
@@ -957,7 +939,6 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     /* slots plus a margin for the sweep's 256-slot chunks */
     const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 256;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
-    d.newCap = (uint32_t)std::min<uint64_t>(d.N, std::max<uint64_t>(65536, d.N / 16));
     d.dlistCap = (uint32_t)std::min<uint64_t>(d.N, 1u << 20);
     d.dnbx = (d.nx + 15) / 16;
     d.dnby = (d.ny + 7) / 8;
@@ -970,7 +951,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
     }
     if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.cstep, capTot)) ||
-        (rc = dalloc(h, &d.rec, capTot)) || (rc = dalloc(h, &d.newl, d.newCap)) ||
+        (rc = dalloc(h, &d.rec, capTot)) ||
         (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
         (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
         (rc = dalloc(h, &d.par, d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||

@@ -41,7 +41,7 @@
 /* cstep sentinels; real completion steps are below GM_CS_MAXSTEP */
 #define GM_CS_INACTIVE 0xFFFFFFFFu
 #define GM_CS_NEVER 0xFFFFFFFEu   /* active, does not reach rxFrac >= 1 within the run */
-#define GM_CS_UNKNOWN 0xFFFFFFFDu /* infected in the previous step, increment not yet known */
+#define GM_CS_UNKNOWN 0xFFFFFFFDu /* placed by nucleation, increment not yet known (the first step re-anchors) */
 #define GM_CS_DEFERRED 0xFFFFFFFCu /* mobility weight still pending: predicted at the end of the window */
 #define GM_CS_MAXSTEP 0xFFFFFFF0u
 
@@ -56,7 +56,7 @@
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
 #define GM_PEND_INLINE 64         /* new P-cache keys returned with the per-step stats */
-#define GM_NK 10                  /* kernels per step (SCORE_GM_K_* order) */
+#define GM_NK 9                   /* kernels per step (SCORE_GM_K_* order) */
 #define GM_WMAX 8                 /* steps per launch window */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
@@ -121,7 +121,6 @@ struct GmStats {
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
     /* launch windows: several steps run back to back; the host reads the rows at the end */
     uint32_t kcur;            /* index of the next step */
-    uint32_t nnew;            /* cells infected in the last step, listed in newl */
     uint32_t halt;            /* set after a step that needs the host: later steps of the window do nothing */
     uint32_t nexec;           /* steps executed in the window */
     uint32_t ndeferred;       /* cells whose prediction waits for a pending weight */
@@ -160,10 +159,8 @@ struct GmDev {
     uint32_t *pmI, *pmB;        /* per parent: winning ranks into Inside / Border */
     /* pslot, par, pmI and pmB hold parentCap entries */
     uint32_t *bcnt;             /* per-block counts and offsets, 8 x GM_MAX_BLOCKS */
-    uint32_t *newl;             /* global slots of the cells infected in the last step */
     uint32_t *dlist;            /* slots of the cells left DEFERRED in a window (dlistCap entries) */
     uint32_t dlistCap;
-    uint32_t newCap;
     uint32_t tileCap, parentCap;
     /* grains */
     double *rho, *defq, *rxq;
@@ -194,16 +191,15 @@ struct GmSweepParams {
     double Gbhalfsq, zener, dt, inv_dx;
 };
 
-/* The step index, list lengths and new-cell count live in GmStats (kcur, nI,
- * nB, nnew) so that a window of steps runs without the host; nI, nB and nnew
- * here only size the grids (upper estimates for the window). */
+/* The step index and list lengths live in GmStats (kcur, nI, nB) so that a
+ * window of steps runs without the host; nI and nB here only size the grids
+ * (upper estimates for the window). */
 struct GmSweepArgs {
     GmSweepParams sp;
     uint32_t k;                 /* step index (kernels read st->kcur) */
     uint32_t reanchor;          /* scalars changed since the previous step */
     uint32_t nI, nB;            /* occupied slots, for the grid sizes */
     uint32_t stepcap;           /* last step a completion can fall in */
-    uint32_t nnew;              /* cells infected in the previous step, for the grid size */
 };
 
 /* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
@@ -211,9 +207,8 @@ struct GmSweepArgs {
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
 hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                             hipStream_t s, hipEvent_t *ev);
-/* start of a window: step index, cells infected in the previous step, Xcells;
- * clears the window's accumulators */
-hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, double xcells, hipStream_t s);
+/* start of a window: step index and Xcells; clears the window's accumulators */
+hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, double xcells, hipStream_t s);
 /* end of every step of a window: records the step's row, advances the step
  * index, and sets halt when the host must act before the next step */
 hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s);

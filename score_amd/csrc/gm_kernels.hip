@@ -5,13 +5,11 @@
  * is, in serial order: sweep the Inside list, sweep the Border list, let every
  * completed Inside cell infect its 26 neighbours in list order, then every
  * completed Border cell.  A target is won by the first attempt in that order
- * that finds it deformed.  The same outcome is computed by ten kernels:
+ * that finds it deformed.  The same outcome is computed by nine kernels:
  *
  *   k_reanchor     only in steps whose scalars changed: every active cell is
  *                  advanced to the step with its old increment and predicted
  *                  again with the new one, one thread per slot.
- *   k_predict_new  in the other steps: increments and completion steps of the
- *                  cells infected in the previous step, from a dense list.
  *   k_sweep        pass over the occupied slots of both lists, statically
  *                  partitioned over the grid.  A slot costs one 4-byte read (its
  *                  predicted completion step).  Completed and INACTIVE slots are
@@ -31,9 +29,9 @@
  *                  k-th recycled slot, else appends.
  *   k_recycle      writes the recycled slots that will be handed out: the first
  *                  min(winners, cap) INACTIVE slots of each list, ascending.
- *   k_commit       one thread per winner writes the new record, lists it for the
- *                  next prediction and clears the cell's bitmap bit; then the
- *                  parents retire.
+ *   k_commit       one thread per winner writes the new record with its
+ *                  increment and predicted completion step, and clears the
+ *                  cell's bitmap bit; then the parents retire.
  *
  * No kernel waits on another block: every cross-block dependency is a kernel
  * boundary, so the passes use only relaxed device atomics.  FP64 throughout,
@@ -245,51 +243,41 @@ static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSw
     return true;
 }
 
-/* Completion step of slot s from its value f before step k (anchor step s0):
- * the first step >= k in which repeated rounded additions of the increment
- * reach 1.0.  Rewrites the record's first sector: the anchor, the increment
- * (0.0 for a cell SCORE does not advance) and the value after the completing
- * addition, whose excess over 1.0 the infection carries on; then stores the
- * completion step. */
-static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
-                                                        uint32_t s0, double P, uint32_t defg, uint32_t inf) {
-    const double f0 = f;
-    double inc, fin = 0.0;
+/* Completion step of a cell with value f before step k: the first step >= k
+ * in which repeated rounded additions of its increment reach 1.0 (NEVER when
+ * none up to the step cap does).  Sets the increment (0.0 for a cell SCORE does
+ * not advance) and the value after the completing addition, whose excess over
+ * 1.0 the infection carries on. */
+static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const GmSweepArgs &A, uint32_t k, double f,
+                                                          double P, uint32_t defg, uint32_t inf, double &inc,
+                                                          double &fin) {
     uint32_t cs = GM_CS_NEVER;
+    fin = 0.0;
     if (cell_increment(d, A.sp, P, defg, inf, inc)) {
-        if (A.k <= A.stepcap) {
-            const uint64_t j = gm_radd_until_one(f, inc, (uint64_t)(A.stepcap - A.k) + 1u);
+        if (k <= A.stepcap) {
+            const uint64_t j = gm_radd_until_one(f, inc, (uint64_t)(A.stepcap - k) + 1u);
             if (j) {
-                cs = A.k + (uint32_t)(j - 1u);
+                cs = k + (uint32_t)(j - 1u);
                 fin = f;
             }
         }
     } else {
         inc = 0.0;
     }
-    rec_set_a(d.rec + s, f0, s0, inf);
-    rec_set_b(d.rec + s, inc, fin);
-    d.cstep[s] = cs;
     return cs;
 }
 
-/* a cell infected in the previous step (anchor at step k): resolve its weight and
- * predict.  A weight the host has not evaluated yet (a new pair met inside a
- * window of steps) leaves the cell DEFERRED: it cannot complete before the
- * window ends, and is predicted then. */
-static __device__ __forceinline__ uint32_t predict_new(const GmDev &d, const GmSweepArgs &A, uint32_t s) {
-    GmRec *r = d.rec + s;
-    const RecA a = rec_a(r);
-    const RecC c = rec_c(r);
-    const double P = resolved(d, c.P);
-    if (is_pending(P)) {
-        d.cstep[s] = GM_CS_DEFERRED;
-        const uint32_t i = atomicAdd(&d.st->ndeferred, 1u);
-        if (i < d.dlistCap) d.dlist[i] = s;
-        return GM_CS_DEFERRED;
-    }
-    if (is_pending(c.P)) r->P = P;
-    return predict_slot(d, A, s, a.frac0, a.s0, P, c.defg, a.inf);
+/* Predicts slot s from its value f before step A.k (its new anchor): rewrites
+ * the record's first sector (anchor, increment, final value) and stores the
+ * completion step. */
+static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
+                                                        uint32_t s0, double P, uint32_t defg, uint32_t inf) {
+    double inc, fin;
+    const uint32_t cs = predict_values(d, A, A.k, f, P, defg, inf, inc, fin);
+    rec_set_a(d.rec + s, f, s0, inf);
+    rec_set_b(d.rec + s, inc, fin);
+    d.cstep[s] = cs;
+    return cs;
 }
 
 /* the arguments of the step the device is at */
@@ -319,16 +307,6 @@ __global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A0) 
         if (is_pending(c.P)) r->P = P;
         predict_slot(d, A, s, f, A.k, P, c.defg, a.inf);
     }
-}
-
-/* cells infected in the previous step, listed by the commit pass, predicted
- * with every lane of a wave busy */
-__global__ void __launch_bounds__(GM_BLOCK) k_predict_new(GmDev d, GmSweepArgs A0) {
-    if (d.st->halt) return;
-    const GmSweepArgs A = step_args(d, A0);
-    const uint32_t n = d.st->nnew < d.newCap ? d.st->nnew : d.newCap;
-    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK)
-        predict_new(d, A, d.newl[i]);
 }
 
 /* list lengths of the current step as tiles: the Inside list rounded up to
@@ -410,9 +388,9 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
                     const uint32_t x = v[q];
                     if (x == GM_CS_INACTIVE) pack |= 16u << q;
                     else if (x == A.k) pack |= 1u << q;
-                    /* every cell is predicted before the sweep (k_predict_new or
-                     * k_reanchor) or DEFERRED (above every step): a past or missing
-                     * prediction is an internal error */
+                    /* every cell is predicted before the sweep (when it is
+                     * infected, or by k_reanchor) or DEFERRED (above every step):
+                     * a past or missing prediction is an internal error */
                     else if (x < A.k || x == GM_CS_UNKNOWN) unknown++;
                 }
             }
@@ -616,14 +594,12 @@ hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, 
     A.reanchor = 0;
     if ((e = mark(ev, 0, s)) != hipSuccess) return e;
     if (a.reanchor) k_reanchor<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
-    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
-    if (!a.reanchor) k_predict_new<<<grid_for(a.nnew < d.newCap ? a.nnew : d.newCap, GM_BLOCK), GM_BLOCK, 0, s>>>(d, A);
     const uint32_t nb = sweep_blocks(a.nI, a.nB);
-    if ((e = mark(ev, 2, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
     k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A);
-    if ((e = mark(ev, 3, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 2, s)) != hipSuccess) return e;
     k_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
-    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 3, s)) != hipSuccess) return e;
     k_compact<<<nb, 64, 0, s>>>(d);
     return hipGetLastError();
 }
@@ -822,32 +798,38 @@ __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
     }
 }
 
-/* One front record written whole: two full 32-byte sectors.  The increment
- * and final value are filled in by the prediction of the next step. */
-static __device__ __forceinline__ void rec_write_new(GmRec *r, double frac0, uint32_t s0, uint32_t inf, double P,
-                                                     uint32_t cell, uint32_t defg, uint32_t rxg) {
-    rec_set_a(r, frac0, s0, inf);
-    rec_set_b(r, 0.0, 0.0);
-    rec_set_c(r, P, cell, defg);
-    *reinterpret_cast<uint4 *>(&r->rxg) = make_uint4(rxg, 0u, 0u, 0u);
-}
-
 /* The new record of a won attempt of rank r on target t = (tx, ty, tz) in slot
  * gslot: the carry of the parent's overshoot by the direction's class
  * (updateFullRX*, SCORE_Kernel.cpp:4568-4599), the parent's weight unless the
  * target lies in another deformed grain (:4297-4302), anchored at step k+1;
- * the cell leaves the deformed bitmap. */
-static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParams &pp, uint32_t k, const GmPar *pr,
-                                                  const uint4 &pq0, uint32_t r, uint32_t t, uint32_t tx, uint32_t ty,
-                                                  uint32_t tz, uint32_t gslot) {
+ * the cell leaves the deformed bitmap.  Its increment and completion step are
+ * predicted at once under the step's scalars (a later step whose scalars differ
+ * re-anchors every cell).  A weight the host has not evaluated yet (a pair met
+ * for the first time in this launch window) leaves the cell DEFERRED: it cannot
+ * complete before the window ends, and is predicted then. */
+static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParams &pp, const GmSweepArgs &A,
+                                                  uint32_t k, const GmPar *pr, const uint4 &pq0, uint32_t r, uint32_t t,
+                                                  uint32_t tx, uint32_t ty, uint32_t tz, uint32_t gslot) {
     const double overshoot = pr->fin - 1.0;
     const int cls = c_cls[r];
     const double carry = cls == 1 ? overshoot * 0.0682 : (cls == 2 ? overshoot * 0.0542 : overshoot * 0.0484);
     const uint32_t g = d.grid[t];
     double Pn = pr->P;
     if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
-    rec_write_new(d.rec + gslot, carry, k + 1u, c_code[r], Pn, t, g, pq0.w);
-    d.cstep[gslot] = GM_CS_UNKNOWN;
+    const uint32_t inf = c_code[r];
+    double inc = 0.0, fin = 0.0;
+    uint32_t cs = GM_CS_DEFERRED;
+    if (!is_pending(Pn)) cs = predict_values(d, A, k + 1u, carry, Pn, g, inf, inc, fin);
+    GmRec *rec = d.rec + gslot;
+    rec_set_a(rec, carry, k + 1u, inf);
+    rec_set_b(rec, inc, fin);
+    rec_set_c(rec, Pn, t, g);
+    *reinterpret_cast<uint4 *>(&rec->rxg) = make_uint4(pq0.w, 0u, 0u, 0u);
+    d.cstep[gslot] = cs;
+    if (cs == GM_CS_DEFERRED) {
+        const uint32_t i = atomicAdd(&d.st->ndeferred, 1u);
+        if (i < d.dlistCap) d.dlist[i] = gslot;
+    }
     clear_deformed(d, tx, ty, tz);
 }
 
@@ -855,9 +837,8 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
  * :4402-4426), slots as omp_getNextFreeSlotIn* hand them out (:3757-3874); then
  * the parents retire as updateFullRX* does it (:4640-4653).  One wave per block
  * takes the block's parents (the partition of k_decide) 64 at a time; their
- * winners are flattened into a list so that every lane commits one infection.
- * The slots of the new cells are listed for the next step's prediction. */
-__global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
+ * winners are flattened into a list so that every lane commits one infection. */
+__global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArgs A) {
     __shared__ uint32_t sI[64], sB[64], oI[64], oB[64];
     __shared__ uint16_t wl[26 * 64];
     GmStats *st = d.st;
@@ -865,7 +846,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
     const uint32_t k = st->kcur;
     const uint32_t nfull = st->nfullI + st->nfullB;
     const uint32_t recLenI = st->recLenI, recLenB = st->recLenB;
-    const uint32_t nI = st->nIold, nB = st->nBold, winI = st->winI;
+    const uint32_t nI = st->nIold, nB = st->nBold;
     const uint32_t lane = threadIdx.x;
     uint32_t lo, hi;
     part_of(nfull, GM_PCHUNK, lo, hi);
@@ -899,24 +880,21 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
             attempt_target(d, pq2.w != 0u, pq2.x, pq2.y, pq2.z, (int)r, tx, ty, tz, toB);
             const uint32_t t = cell_of(d, tx, ty, tz);
             const uint32_t wI = sI[j];
-            uint32_t gslot = GM_CS_INACTIVE, nidx;
+            uint32_t gslot = GM_CS_INACTIVE;
             if (wI & bit) {
                 const uint32_t rank = oI[j] + (uint32_t)__popc(wI & (bit - 1u));
                 const uint32_t rel = rank < recLenI ? d.recI[rank] : nI + (rank - recLenI);
                 if (rel >= d.capI) atomicOr(&st->err, GM_ERRF_INSIDE_FULL);
                 else gslot = rel;
-                nidx = rank;
             } else {
                 const uint32_t wB = sB[j];
                 const uint32_t rank = oB[j] + (uint32_t)__popc(wB & (bit - 1u));
                 const uint32_t rel = rank < recLenB ? d.recB[rank] : nB + (rank - recLenB);
                 if (rel >= d.capB) atomicOr(&st->err, GM_ERRF_BORDER_FULL);
                 else gslot = d.capI + rel;
-                nidx = winI + rank;
             }
             if (gslot == GM_CS_INACTIVE) continue;
-            if (nidx < d.newCap) d.newl[nidx] = gslot;
-            infect_one(d, pp, k, pr, pq0, r, t, tx, ty, tz, gslot);
+            infect_one(d, pp, A, k, pr, pq0, r, t, tx, ty, tz, gslot);
         }
         /* retire the batch's parents; a deformed grain losing its last cell is
          * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165).
@@ -966,19 +944,21 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp) {
 hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                             hipStream_t s, hipEvent_t *ev) {
     hipError_t e;
-    if ((e = mark(ev, 5, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
     k_propose<<<grid_for(parents_upper, GM_BLOCK / GM_PG), GM_BLOCK, 0, s>>>(d);
     const uint32_t nb = grid_for(parents_upper, GM_PCHUNK);
-    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 5, s)) != hipSuccess) return e;
     k_decide<<<nb, 64, 0, s>>>(d);
-    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
     k_decide_scan<<<1, 1024, 0, s>>>(d, nb);
     /* the sweep's grid, for the recycling lists */
-    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
+    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
     k_recycle<<<sweep_blocks(a.nI, a.nB), 64, 0, s>>>(d);
+    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
+    GmSweepArgs A = a;
+    A.reanchor = 0;
+    k_commit<<<nb, 64, 0, s>>>(d, pp, A);
     if ((e = mark(ev, 9, s)) != hipSuccess) return e;
-    k_commit<<<nb, 64, 0, s>>>(d, pp);
-    if ((e = mark(ev, 10, s)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -1432,10 +1412,9 @@ hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k
  * the step index and list lengths from the stats and returns at once once a
  * step has set `halt`, so that the host takes over at the step after it. */
 
-__global__ void k_window_begin(GmDev d, uint32_t k, uint32_t nnew, double xcells) {
+__global__ void k_window_begin(GmDev d, uint32_t k, double xcells) {
     GmStats *st = d.st;
     st->kcur = k;
-    st->nnew = nnew;
     st->halt = 0;
     st->nexec = 0;
     st->ndeferred = 0;
@@ -1447,8 +1426,8 @@ __global__ void k_window_begin(GmDev d, uint32_t k, uint32_t nnew, double xcells
     st->xcells = xcells;
 }
 
-hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, uint32_t nnew, double xcells, hipStream_t s) {
-    k_window_begin<<<1, 1, 0, s>>>(d, k, nnew, xcells);
+hipError_t gm_launch_window_begin(const GmDev &d, uint32_t k, double xcells, hipStream_t s) {
+    k_window_begin<<<1, 1, 0, s>>>(d, k, xcells);
     return hipGetLastError();
 }
 
@@ -1463,18 +1442,16 @@ static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmSt
     st->rows[j] = r;
     st->nexec = j + 1;
     st->kcur = st->kcur + 1u;
-    st->nnew = r.winI + r.winB;
     const uint32_t npend = __hip_atomic_load(&st->npend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st->npend0 = npend;
     const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
     st->xcells = xc;
     const uint64_t slots = (uint64_t)r.nI + r.nB;
     const uint64_t tiles = (uint64_t)(((r.nI + 63) / 64 + 3u) & ~3u) + (r.nB + 63) / 64 + 4u;
-    const uint64_t newmax = 26ull * slots < d.N ? 26ull * slots : d.N;
     bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                 any_top || xc >= w.xstop;
     halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;
-    halt = halt || slots > d.parentCap || tiles > d.tileCap || newmax > d.newCap || slots >= (1ull << 27);
+    halt = halt || slots > d.parentCap || tiles > d.tileCap || slots >= (1ull << 27);
     st->halt = halt ? 1u : 0u;
 }
 
