@@ -683,23 +683,26 @@ static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, u
 /* Parents are cut into chunks of GM_PCHUNK for the decide and commit passes, so
  * that a 64-lane wave takes a whole chunk. */
 #define GM_PCHUNK 64
-#define GM_PG 8                       /* lanes per parent in the propose pass */
+#ifndef GM_PG
+#define GM_PG 4                       /* lanes per parent in the propose pass (4, 8 or 16) */
+#endif
+#define GM_PG_ITERS ((26 + GM_PG - 1) / GM_PG)
+static_assert(GM_PG == 4 || GM_PG == 8 || GM_PG == 16, "lanes per parent");
 
-/* Eight lanes per parent: lane 0 copies what the infection needs of the
+/* GM_PG lanes per parent: lane 0 copies what the infection needs of the
  * completed cell from its record into the dense parent array (FullRX order);
- * lane l makes the attempts of rank l, l+8, l+16 and l+24 (< 26), so the four
- * bitmap reads of a lane are independent and a wave keeps eight parents in
- * flight.  Every attempt on a deformed cell claims its target (the smallest key
- * p*32+r is the serial winner); their mask over the ranks is kept in pmI for
- * the decide pass. */
+ * lane l makes the attempts of rank l, l+GM_PG, ... (< 26), so the bitmap reads
+ * of a lane are independent and a wave keeps 64/GM_PG parents in flight.  Every
+ * attempt on a deformed cell claims its target (the smallest key p*32+r is the
+ * serial winner); their mask over the ranks is kept in pmI for the decide pass. */
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
     if (d.st->halt) return;
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
-    const uint32_t l8 = threadIdx.x & (GM_PG - 1);
+    const uint32_t lg = threadIdx.x & (GM_PG - 1);
     const uint32_t ngroups = gridDim.x * (GM_BLOCK / GM_PG);
     for (uint32_t p = blockIdx.x * (GM_BLOCK / GM_PG) + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
         uint4 q2 = make_uint4(0u, 0u, 0u, 0u);
-        if (l8 == 0) {
+        if (lg == 0) {
             const uint32_t s = d.pslot[p];
             const GmRec *r = d.rec + s;
             const RecC c = rec_c(r);
@@ -721,8 +724,8 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
         const uint32_t gsh = (threadIdx.x & 63) & ~(GM_PG - 1); /* first lane of the group in the wave */
         uint32_t am = 0;
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
-            const uint32_t rk = l8 + GM_PG * i;
+        for (uint32_t i = 0; i < GM_PG_ITERS; i++) {
+            const uint32_t rk = lg + GM_PG * i;
             bool att = false;
             if (rk < 26) {
                 uint32_t tx, ty, tz;
@@ -732,9 +735,9 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
                     atomicMin(&d.claim[cell_of(d, tx, ty, tz)], p * 32u + rk);
                 }
             }
-            am |= (uint32_t)((__ballot(att) >> gsh) & 0xFFu) << (GM_PG * i);
+            am |= (uint32_t)((__ballot(att) >> gsh) & ((1ull << GM_PG) - 1u)) << (GM_PG * i);
         }
-        if (l8 == 0) d.pmI[p] = am;
+        if (lg == 0) d.pmI[p] = am;
     }
 }
 
@@ -808,13 +811,13 @@ __global__ void __launch_bounds__(1024) k_decide_scan(GmDev d, uint32_t nb) {
  * for the first time in this launch window) leaves the cell DEFERRED: it cannot
  * complete before the window ends, and is predicted then. */
 static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParams &pp, const GmSweepArgs &A,
-                                                  uint32_t k, const GmPar *pr, const uint4 &pq0, uint32_t r, uint32_t t,
-                                                  uint32_t tx, uint32_t ty, uint32_t tz, uint32_t gslot) {
-    const double overshoot = pr->fin - 1.0;
+                                                  uint32_t k, double pfin, double pP, const uint4 &pq0, uint32_t r,
+                                                  uint32_t t, uint32_t g, uint32_t tx, uint32_t ty, uint32_t tz,
+                                                  uint32_t gslot) {
+    const double overshoot = pfin - 1.0;
     const int cls = c_cls[r];
     const double carry = cls == 1 ? overshoot * 0.0682 : (cls == 2 ? overshoot * 0.0542 : overshoot * 0.0484);
-    const uint32_t g = d.grid[t];
-    double Pn = pr->P;
+    double Pn = pP;
     if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
     const uint32_t inf = c_code[r];
     double inc = 0.0, fin = 0.0;
@@ -837,9 +840,14 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
  * :4402-4426), slots as omp_getNextFreeSlotIn* hand them out (:3757-3874); then
  * the parents retire as updateFullRX* does it (:4640-4653).  One wave per block
  * takes the block's parents (the partition of k_decide) 64 at a time; their
- * winners are flattened into a list so that every lane commits one infection. */
+ * winners are flattened into a list so that every lane commits one infection.
+ * Each lane loads its own parent's data with its winner masks and shares it
+ * through LDS, and a winner issues its target's grid load and its slot lookup
+ * together, so a wave waits on few dependent memory round trips. */
 __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArgs A) {
     __shared__ uint32_t sI[64], sB[64], oI[64], oB[64];
+    __shared__ uint4 sq0[64], sq2[64];
+    __shared__ double sfin[64], sP[64];
     __shared__ uint16_t wl[26 * 64];
     GmStats *st = d.st;
     if (st->halt) return;
@@ -855,6 +863,15 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArg
         const uint32_t pt = batch + lane;
         const bool valid = pt < hi;
         const uint32_t mI = valid ? d.pmI[pt] : 0u, mB = valid ? d.pmB[pt] : 0u;
+        /* this lane's parent: slot, cell, defg, rxg | fin, P | x, y, z, fromB */
+        uint4 q0 = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu), q2 = make_uint4(0u, 0u, 0u, 0u);
+        double2 fp = make_double2(0.0, 0.0);
+        if (valid) {
+            const GmPar *pr = d.par + pt;
+            q0 = *reinterpret_cast<const uint4 *>(&pr->slot);
+            fp = *reinterpret_cast<const double2 *>(&pr->fin);
+            q2 = *reinterpret_cast<const uint4 *>(&pr->x);
+        }
         const uint32_t cI = (uint32_t)__popc(mI), cB = (uint32_t)__popc(mB);
         /* one scan of both counts (each batch total < 2^16) */
         const uint32_t packed = (cI << 16) | cB;
@@ -865,6 +882,10 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArg
         oB[lane] = runB + (ex & 0xFFFFu);
         sI[lane] = mI;
         sB[lane] = mB;
+        sq0[lane] = q0;
+        sq2[lane] = q2;
+        sfin[lane] = fp.x;
+        sP[lane] = fp.y;
         /* winners of this parent at [exI + exB, ...) in the flat list, by rank */
         uint32_t pos = (ex >> 16) + (ex & 0xFFFFu);
         for (uint32_t m = mI | mB; m; m &= m - 1u) wl[pos++] = (uint16_t)(lane | ((uint32_t)__ffs(m) - 1u) << 8);
@@ -872,37 +893,32 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArg
         const uint32_t nw = totI + totB;
         for (uint32_t w = lane; w < nw; w += 64) {
             const uint32_t e = wl[w], j = e & 0xFFu, r = e >> 8, bit = 1u << r;
-            const GmPar *pr = d.par + batch + j;
-            const uint4 pq0 = *reinterpret_cast<const uint4 *>(&pr->slot); /* slot, cell, defg, rxg */
-            const uint4 pq2 = *reinterpret_cast<const uint4 *>(&pr->x);
+            const uint4 pq0 = sq0[j], pq2 = sq2[j];
             uint32_t tx, ty, tz;
             bool toB;
             attempt_target(d, pq2.w != 0u, pq2.x, pq2.y, pq2.z, (int)r, tx, ty, tz, toB);
             const uint32_t t = cell_of(d, tx, ty, tz);
+            const uint32_t g = d.grid[t];
+            /* rank k into a list takes its k-th recycled slot, else appends */
             const uint32_t wI = sI[j];
-            uint32_t gslot = GM_CS_INACTIVE;
-            if (wI & bit) {
-                const uint32_t rank = oI[j] + (uint32_t)__popc(wI & (bit - 1u));
-                const uint32_t rel = rank < recLenI ? d.recI[rank] : nI + (rank - recLenI);
-                if (rel >= d.capI) atomicOr(&st->err, GM_ERRF_INSIDE_FULL);
-                else gslot = rel;
-            } else {
-                const uint32_t wB = sB[j];
-                const uint32_t rank = oB[j] + (uint32_t)__popc(wB & (bit - 1u));
-                const uint32_t rel = rank < recLenB ? d.recB[rank] : nB + (rank - recLenB);
-                if (rel >= d.capB) atomicOr(&st->err, GM_ERRF_BORDER_FULL);
-                else gslot = d.capI + rel;
+            const bool intoI = (wI & bit) != 0u;
+            const uint32_t wm = intoI ? wI : sB[j];
+            const uint32_t rank = (intoI ? oI[j] : oB[j]) + (uint32_t)__popc(wm & (bit - 1u));
+            const uint32_t recLen = intoI ? recLenI : recLenB;
+            const uint32_t *recl = intoI ? d.recI : d.recB;
+            const uint32_t rel = rank < recLen ? recl[rank] : (intoI ? nI : nB) + (rank - recLen);
+            if (rel >= (intoI ? d.capI : d.capB)) {
+                atomicOr(&st->err, intoI ? GM_ERRF_INSIDE_FULL : GM_ERRF_BORDER_FULL);
+                continue;
             }
-            if (gslot == GM_CS_INACTIVE) continue;
-            infect_one(d, pp, A, k, pr, pq0, r, t, tx, ty, tz, gslot);
+            const uint32_t gslot = intoI ? rel : d.capI + rel;
+            infect_one(d, pp, A, k, sfin[j], sP[j], pq0, r, t, g, tx, ty, tz, gslot);
         }
         /* retire the batch's parents; a deformed grain losing its last cell is
          * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165).
          * Consecutive parents often share their nucleus and deformed grain (siblings),
          * so each run of equal ids among neighbouring lanes makes one atomic. */
-        uint4 q0 = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu);
         if (valid) {
-            q0 = *reinterpret_cast<const uint4 *>(&d.par[pt].slot);
             d.grid[q0.y] = d.ndefg + q0.w;
             d.cstep[q0.x] = GM_CS_INACTIVE;
         }
