@@ -404,7 +404,7 @@ uint32_t pow2_at_least(uint64_t v) {
 /* per-step lists, tile masks and per-parent masks sized for the current occupancy */
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
-    const uint32_t tiles = (((h->nI + 63) / 64 + 3u) & ~3u) + (h->nB + 63) / 64 + 4u;
+    const uint32_t tiles = gm_tiles_inside(h->nI) + (h->nB + 63) / 64 + GM_SW_TPC;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
         dfree(h, d.mF); dfree(h, d.mR);
@@ -536,7 +536,7 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     }
     const uint32_t n = nnever - first - 1u;
     GmRec *trec = nullptr;
-    uint32_t *tcs = nullptr;
+    gm_cs_t *tcs = nullptr;
     int rc;
     if ((rc = dalloc(h, &trec, n)) || (rc = dalloc(h, &tcs, n))) {
         dfree(h, trec);
@@ -599,7 +599,8 @@ uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &s
     while (n < w) {
         advance_scalars(h);
         h->step++;
-        if (done(h) || !same_scalars(sweep_params(h), sp)) break;
+        /* a step at a multiple of the horizon re-anchors, so it starts a window */
+        if (done(h) || !same_scalars(sweep_params(h), sp) || h->step % h->d.horizon == 0) break;
         n++;
     }
     h->t = t0; h->T = T0; h->G = G0; h->b = b0; h->Gbhalfsq = gb0; h->dt = dt0; h->pzmin = pz0;
@@ -651,6 +652,9 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         h->defrag_cnt++;
         h->force_reanchor = true;
     }
+    /* every cell is predicted again at each multiple of the horizon, so that no
+     * stored completion step lies a horizon or more ahead */
+    if (h->step > 0 && h->step % d.horizon == 0) h->force_reanchor = true;
     rc = ensure_step_capacity(h);
     if (rc) return h->err = rc;
     GmSweepArgs a;
@@ -659,7 +663,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     const GmSweepParams sp = a.sp;
     a.k = h->step;
     a.reanchor = (h->force_reanchor || !h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
-    a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_CS_MAXSTEP - 1);
+    a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_STEP_MAX - 1);
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
     /* infection keys are p*32 + rank in 32 bits: at most 2^27 completed cells per step */
@@ -936,10 +940,22 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->pp.omega_hi = rh_omega_threshold(h->p);
     h->pp.fac = 1.0 / sqrt(2.0);
 
-    /* slots plus a margin for the sweep's 256-slot chunks */
-    const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 256;
+    /* slots plus a margin of one sweep chunk (512 slots) */
+    const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 512;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
     d.dlistCap = (uint32_t)std::min<uint64_t>(d.N, 1u << 20);
+    /* completion steps further ahead than the horizon are re-predicted at its
+     * multiples; SCORE_GM_HORIZON lowers it (tests use it to make that happen) */
+    d.horizon = GM_CS_HORIZON_MAX;
+    if (const char *ev = getenv("SCORE_GM_HORIZON")) {
+        char *end = nullptr;
+        const long v = strtol(ev, &end, 10);
+        if (end == ev || *end != '\0' || v < 16 || v > (long)GM_CS_HORIZON_MAX) {
+            delete h;
+            return fail(SCORE_GM_ERR_ARG, "SCORE_GM_HORIZON must be an integer in [16, 16384]");
+        }
+        d.horizon = (uint32_t)v;
+    }
     d.dnbx = (d.nx + 15) / 16;
     d.dnby = (d.ny + 7) / 8;
     d.dnbz = (d.nz + 7) / 8;
@@ -982,7 +998,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
         HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
         HIPCHK(gm_launch_dmap_init(d, s));
-        HIPCHK(gm_launch_fill_u32(d.cstep, GM_CS_INACTIVE, capTot, s));
+        /* two INACTIVE codes per word (capTot is a multiple of 64) */
+        HIPCHK(gm_launch_fill_u32((uint32_t *)d.cstep, 0xFFFFFFFFu, capTot / 2, s));
         HIPCHK(gm_launch_fill_u64((unsigned long long *)d.rec, 0ull, capTot * (sizeof(GmRec) / 8), s));
         HIPCHK(hipMemcpyAsync(d.rho, dom->defg_rho, sizeof(double) * d.ndefg, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.defq, h->defq.data(), sizeof(double) * 4 * d.ndefg, hipMemcpyHostToDevice, s));
@@ -1290,18 +1307,17 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     if (rc) return rc;
     const uint64_t N = (uint64_t)dom->nx * dom->ny * dom->nz;
     const uint64_t capB = 2ull * ((uint64_t)dom->nx * dom->ny + (uint64_t)dom->nx * dom->nz + (uint64_t)dom->ny * dom->nz);
-    const uint64_t capTot = (N + 63) / 64 * 64 + (capB + 63) / 64 * 64 + 256;
+    const uint64_t capTot = (N + 63) / 64 * 64 + (capB + 63) / 64 * 64 + 512;
     const uint64_t bricks = (uint64_t)((dom->nx + 15) / 16) * ((dom->ny + 7) / 8) * ((dom->nz + 7) / 8);
     const uint64_t hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * dom->n_rxg));
     const uint64_t front = std::max<uint64_t>(65536, N / 4);
     uint64_t b = 0;
     b += 8 * N;                                   /* grid, claim */
     b += 32 * bricks;                             /* deformed bitmap */
-    b += (4 + sizeof(GmRec)) * capTot;            /* completion steps, records */
+    b += (sizeof(gm_cs_t) + sizeof(GmRec)) * capTot; /* completion steps, records */
     b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */
-    b += 4 * std::min<uint64_t>(N, 26 * front);   /* new-cell list */
     b += 4 * std::min<uint64_t>(N, 1u << 20);     /* deferred-cell list */
-    b += 16 * (capTot / 64 + 4);                  /* tile masks */
+    b += 16 * (capTot / 64 + GM_SW_TPC);          /* tile masks */
     b += 4 * (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt + (double)capB * prm->maxfill * kFullRecyclingAtt);
     b += 4 * 8 * GM_MAX_BLOCKS;
     b += 52ull * dom->n_defg + 52ull * (dom->n_rxg + 1);

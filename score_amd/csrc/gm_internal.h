@@ -12,8 +12,8 @@
  *          that one 128-byte line holds a 16x8x8 brick of cells: a 3x3x3
  *          neighbourhood usually lies in one or two lines
  *   front  capI+capB slots: Inside list slots [0,capI), Border list slots [capI,capI+capB)
- *          cstep uint32[slots]   completion step or a GM_CS_* sentinel; the only
- *                                array every step reads in full
+ *          cstep uint16[slots]   completion step (mod GM_CS_MOD) or a GM_CS_*
+ *                                sentinel; the only array every step reads in full
  *          rec   GmRec[slots]    64-byte record, read and written only when a cell
  *                                is infected, predicted, re-anchored or completes
  * The two lists keep SCORE's slot numbering (SCORE_Kernel.cpp:3587-3874) so that
@@ -38,12 +38,21 @@
 #define GM_PENDING_TAG 0x7FF8000000000000ull  /* quiet NaN; low 32 bits carry the P-cache slot */
 #define GM_PENDING_MASK 0xFFF8000000000000ull
 
-/* cstep sentinels; real completion steps are below GM_CS_MAXSTEP */
-#define GM_CS_INACTIVE 0xFFFFFFFFu
-#define GM_CS_NEVER 0xFFFFFFFEu   /* active, does not reach rxFrac >= 1 within the run */
-#define GM_CS_UNKNOWN 0xFFFFFFFDu /* placed by nucleation, increment not yet known (the first step re-anchors) */
-#define GM_CS_DEFERRED 0xFFFFFFFCu /* mobility weight still pending: predicted at the end of the window */
-#define GM_CS_MAXSTEP 0xFFFFFFF0u
+/* Completion steps are stored in 16 bits.  A step cs is kept as cs % GM_CS_MOD
+ * when it lies less than the horizon ahead of the step that predicted it, else
+ * the cell is FAR; every cell is predicted again at each multiple of the
+ * horizon.  So at step k every stored step lies in [k, k + horizon), and
+ * cs % GM_CS_MOD names it.  Codes at or above GM_CS_MOD are sentinels. */
+typedef uint16_t gm_cs_t;
+#define GM_CS_MOD 0x8000u
+#define GM_CS_HORIZON_MAX 0x4000u  /* horizon in steps, at most GM_CS_MOD / 2 */
+#define GM_CS_INACTIVE 0xFFFFu
+#define GM_CS_NEVER 0xFFFEu       /* active, does not reach rxFrac >= 1 within the run */
+#define GM_CS_UNKNOWN 0xFFFDu     /* placed by nucleation, increment not yet known (the first step re-anchors) */
+#define GM_CS_DEFERRED 0xFFFCu    /* mobility weight still pending: predicted at the end of the window */
+#define GM_CS_FAR 0xFFFBu         /* active, completes at least a horizon after its prediction */
+#define GM_STEP_NONE 0xFFFFFFFFu  /* no completion step */
+#define GM_STEP_MAX 0xFFFFFFF0u   /* steps of a run stay below this */
 
 /* error flags raised on the device */
 #define GM_ERRF_BORDER_FULL 0x1u
@@ -52,6 +61,12 @@
 #define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
 
 #define GM_BLOCK 256
+/* The sweep reads a list in groups of GM_SW_TPC 64-slot tiles per wave; the
+ * Inside list is rounded up to whole groups and the Border list's tiles follow. */
+#define GM_SW_TPC 8u
+static inline __host__ __device__ uint32_t gm_tiles_inside(uint32_t nI) {
+    return ((nI + 63u) / 64u + GM_SW_TPC - 1u) & ~(GM_SW_TPC - 1u);
+}
 #define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
 #define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
@@ -149,7 +164,8 @@ struct GmDev {
     uint32_t *grid, *claim;
     uint32_t *dmap;
     uint32_t dnbx, dnby, dnbz;  /* bricks of the deformed-cell bitmap per axis */
-    uint32_t *cstep;
+    gm_cs_t *cstep;
+    uint32_t horizon;           /* steps, in [16, GM_CS_HORIZON_MAX] */
     GmRec *rec;
     /* per-step lists */
     uint32_t *pslot;            /* completed cells (global slots), Inside then Border, ascending each */
@@ -233,7 +249,7 @@ hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s);
  * and freezes the ACTIVE records left behind at their value before step k. */
 hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s);
 hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
-                                 GmRec *trec, uint32_t *tcs, hipStream_t s);
+                                 GmRec *trec, gm_cs_t *tcs, hipStream_t s);
 /* ompcrit_clarify_status_infectedcells: (cell, grain shown) of every ACTIVE record,
  * Inside then Border, in slot order, before step k; the count goes to st->dcount.
  * ixyz and gid hold nI+nB entries. */

@@ -244,14 +244,14 @@ static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSw
 }
 
 /* Completion step of a cell with value f before step k: the first step >= k
- * in which repeated rounded additions of its increment reach 1.0 (NEVER when
- * none up to the step cap does).  Sets the increment (0.0 for a cell SCORE does
- * not advance) and the value after the completing addition, whose excess over
- * 1.0 the infection carries on. */
+ * in which repeated rounded additions of its increment reach 1.0 (GM_STEP_NONE
+ * when none up to the step cap does).  Sets the increment (0.0 for a cell SCORE
+ * does not advance) and the value after the completing addition, whose excess
+ * over 1.0 the infection carries on. */
 static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const GmSweepArgs &A, uint32_t k, double f,
                                                           double P, uint32_t defg, uint32_t inf, double &inc,
                                                           double &fin) {
-    uint32_t cs = GM_CS_NEVER;
+    uint32_t cs = GM_STEP_NONE;
     fin = 0.0;
     if (cell_increment(d, A.sp, P, defg, inf, inc)) {
         if (k <= A.stepcap) {
@@ -267,17 +267,23 @@ static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const 
     return cs;
 }
 
+/* the stored code of completion step cs predicted at step kp */
+static __device__ __forceinline__ gm_cs_t cs_code(const GmDev &d, uint32_t cs, uint32_t kp) {
+    if (cs == GM_STEP_NONE) return (gm_cs_t)GM_CS_NEVER;
+    if (cs - kp >= d.horizon) return (gm_cs_t)GM_CS_FAR;
+    return (gm_cs_t)(cs & (GM_CS_MOD - 1u));
+}
+
 /* Predicts slot s from its value f before step A.k (its new anchor): rewrites
  * the record's first sector (anchor, increment, final value) and stores the
- * completion step. */
-static __device__ __forceinline__ uint32_t predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
-                                                        uint32_t s0, double P, uint32_t defg, uint32_t inf) {
+ * completion step's code. */
+static __device__ __forceinline__ void predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
+                                                    uint32_t s0, double P, uint32_t defg, uint32_t inf) {
     double inc, fin;
     const uint32_t cs = predict_values(d, A, A.k, f, P, defg, inf, inc, fin);
     rec_set_a(d.rec + s, f, s0, inf);
     rec_set_b(d.rec + s, inc, fin);
-    d.cstep[s] = cs;
-    return cs;
+    d.cstep[s] = cs_code(d, cs, A.k);
 }
 
 /* the arguments of the step the device is at */
@@ -310,9 +316,9 @@ __global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A0) 
 }
 
 /* list lengths of the current step as tiles: the Inside list rounded up to
- * whole 256-slot chunks (tI), then the Border list (T in all) */
+ * whole sweep chunks (tI), then the Border list (T in all) */
 static __device__ __forceinline__ void step_tiles(uint32_t nI, uint32_t nB, uint32_t &tI, uint32_t &T) {
-    tI = ((nI + 63) / 64 + 3u) & ~3u;
+    tI = gm_tiles_inside(nI);
     T = tI + (nB + 63) / 64;
 }
 
@@ -342,11 +348,16 @@ static __device__ __forceinline__ uint32_t tile_slot0(const GmDev &d, uint32_t t
     return tile < tI ? tile * 64u : d.capI + (tile - tI) * 64u;
 }
 
-/* pass 1: every occupied slot, four per lane with one 16-byte load, 256-slot
- * chunks of four tiles per wave; completed and INACTIVE slots are balloted into
- * per-tile masks and counted per block.  The Inside list spans tI tiles (a
- * multiple of four), the Border list follows from tile tI. */
-#define GM_SW_UNROLL 4
+/* pass 1: every occupied slot, eight per lane with one 16-byte load of 2-byte
+ * codes: a wave reads a 512-slot chunk of GM_SW_TPC = 8 tiles per load, and
+ * GM_SW_UNROLL chunks at once.  Each lane classifies its eight slots without
+ * branches (completed bits 0-7, INACTIVE bits 8-15); tile t's masks are balloted
+ * from lanes 8t..8t+7, lane t keeps them and one store writes the chunk's.  The
+ * Inside list spans tI tiles (whole chunks), the Border list follows from tile tI. */
+#ifndef GM_SW_UNROLL
+#define GM_SW_UNROLL 2
+#endif
+static_assert(GM_SW_TPC == 8, "a 16-byte load of 2-byte codes covers 8 slots per lane");
 __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
     __shared__ uint32_t red[GM_BLOCK / 64][5];
     if (d.st->halt) return;
@@ -356,59 +367,64 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
     step_tiles(nI, nB, tI, T);
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t lo, hi;
-    part_of(T, GM_BLOCK / 64, lo, hi); /* multiples of four tiles, except an end at T */
+    part_of(T, GM_SW_TPC, lo, hi); /* whole chunks, except an end at T */
     /* chunks [c_lo, c_hi); a block left empty (lo == hi == T) has none */
-    const uint32_t c_lo = lo / 4, c_hi = lo < hi ? (hi + 3) / 4 : c_lo;
+    const uint32_t c_lo = lo / GM_SW_TPC, c_hi = lo < hi ? (hi + GM_SW_TPC - 1u) / GM_SW_TPC : c_lo;
+    const uint32_t kc = A.k & (GM_CS_MOD - 1u), horizon = d.horizon;
     uint32_t cFI = 0, cRI = 0, cFB = 0, cRB = 0, unknown = 0;
     for (uint32_t c0 = c_lo + w; c0 < c_hi; c0 += (GM_BLOCK / 64) * GM_SW_UNROLL) {
         uint4 cs[GM_SW_UNROLL];
-        uint32_t base[GM_SW_UNROLL], lim[GM_SW_UNROLL];
+        uint32_t nocc[GM_SW_UNROLL];
 #pragma unroll
         for (int u = 0; u < GM_SW_UNROLL; u++) {
             const uint32_t c = c0 + u * (GM_BLOCK / 64);
-            const uint32_t tile0 = c * 4u;
-            base[u] = tile_slot0(d, tile0, tI) + 4u * lane;
+            const uint32_t tile0 = c * GM_SW_TPC;
+            const uint32_t base = tile_slot0(d, tile0, tI) + 8u * lane;
             /* slots below lim are occupied and inside this block's tiles */
             const uint32_t lend = tile0 < tI ? nI : d.capI + nB;
-            const uint32_t tend = tile_slot0(d, tile0, tI) + 64u * (hi - tile0 < 4u ? hi - tile0 : 4u);
-            lim[u] = c < c_hi ? (lend < tend ? lend : tend) : 0u;
-            cs[u] = base[u] < lim[u] ? *reinterpret_cast<const uint4 *>(d.cstep + base[u])
-                                     : make_uint4(GM_CS_INACTIVE, GM_CS_INACTIVE, GM_CS_INACTIVE, GM_CS_INACTIVE);
+            const uint32_t tend = tile_slot0(d, tile0, tI) + 64u * (hi - tile0 < GM_SW_TPC ? hi - tile0 : GM_SW_TPC);
+            const uint32_t lim = c < c_hi ? (lend < tend ? lend : tend) : 0u;
+            nocc[u] = lim > base ? (lim - base < 8u ? lim - base : 8u) : 0u; /* occupied slots of this lane */
+            cs[u] = nocc[u] ? *reinterpret_cast<const uint4 *>(d.cstep + base)
+                            : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
         }
 #pragma unroll
         for (int u = 0; u < GM_SW_UNROLL; u++) {
             const uint32_t c = c0 + u * (GM_BLOCK / 64);
             if (c >= c_hi) break;
-            const uint32_t v[4] = {cs[u].x, cs[u].y, cs[u].z, cs[u].w};
-            /* this lane's four slots: completed bits 0-3, INACTIVE bits 4-7 */
+            const uint32_t wd[4] = {cs[u].x, cs[u].y, cs[u].z, cs[u].w};
             uint32_t pack = 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (base[u] + q < lim[u]) {
-                    const uint32_t x = v[q];
-                    if (x == GM_CS_INACTIVE) pack |= 16u << q;
-                    else if (x == A.k) pack |= 1u << q;
-                    /* every cell is predicted before the sweep (when it is
-                     * infected, or by k_reanchor) or DEFERRED (above every step):
-                     * a past or missing prediction is an internal error */
-                    else if (x < A.k || x == GM_CS_UNKNOWN) unknown++;
-                }
+            for (int q = 0; q < 8; q++) {
+                const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+                const uint32_t in = (uint32_t)q < nocc[u] ? 1u : 0u;
+                pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
+                /* every cell is predicted before the sweep (when it is infected,
+                 * or by k_reanchor) or DEFERRED, NEVER or FAR: a step before k (a
+                 * code a horizon or more ahead) or a missing prediction is an
+                 * internal error */
+                unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
             }
-            const uint32_t tile0 = c * 4u;
+            const uint32_t tile0 = c * GM_SW_TPC;
             const bool isB = tile0 >= tI;
+            uint64_t myF = 0, myR = 0;
 #pragma unroll
-            for (uint32_t t = 0; t < 4; t++) {
-                if (tile0 + t >= hi) break;
-                /* slot 64t + lane of the chunk is component lane%4 of lane 16t + lane/4 */
-                const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(16u * t + (lane >> 2)), 64);
-                const uint64_t mf = __ballot((pk >> (lane & 3u)) & 1u);
-                const uint64_t mr = __ballot((pk >> (4u + (lane & 3u))) & 1u);
-                if (lane == t) {
-                    d.mF[tile0 + t] = mf;
-                    d.mR[tile0 + t] = mr;
-                }
-                if (isB) { cFB += (uint32_t)__popcll(mf); cRB += (uint32_t)__popcll(mr); }
-                else { cFI += (uint32_t)__popcll(mf); cRI += (uint32_t)__popcll(mr); }
+            for (uint32_t t = 0; t < GM_SW_TPC; t++) {
+                /* slot 64t + lane of the chunk is bit lane%8 of lane 8t + lane/8 */
+                const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(8u * t + (lane >> 3)), 64);
+                const uint64_t mf = __ballot((pk >> (lane & 7u)) & 1u);
+                const uint64_t mr = __ballot((pk >> (8u + (lane & 7u))) & 1u);
+                myF = lane == t ? mf : myF;
+                myR = lane == t ? mr : myR;
+                const uint32_t nf = (uint32_t)__popcll(mf), nr = (uint32_t)__popcll(mr);
+                cFB += isB ? nf : 0u;
+                cRB += isB ? nr : 0u;
+                cFI += isB ? 0u : nf;
+                cRI += isB ? 0u : nr;
+            }
+            if (lane < GM_SW_TPC && tile0 + lane < hi) {
+                d.mF[tile0 + lane] = myF;
+                d.mR[tile0 + lane] = myR;
             }
         }
     }
@@ -520,7 +536,7 @@ __global__ void __launch_bounds__(64) k_compact(GmDev d) {
     uint32_t tI, T;
     step_tiles(d.st->nI, d.st->nB, tI, T);
     uint32_t lo, hi;
-    part_of(T, GM_BLOCK / 64, lo, hi);
+    part_of(T, GM_SW_TPC, lo, hi);
     uint32_t run[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) run[c] = d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x];
@@ -553,7 +569,7 @@ __global__ void __launch_bounds__(64) k_recycle(GmDev d) {
 #pragma unroll
     for (int c = 0; c < 4; c++) run[c] = d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x];
     uint32_t lo, hi;
-    part_of(T, GM_BLOCK / 64, lo, hi);
+    part_of(T, GM_SW_TPC, lo, hi);
     for (uint32_t batch = lo; batch < hi; batch += 64) {
         if (run[BC_RI] >= limI && run[BC_RB] >= limB) return; /* nothing further is handed out */
         const uint32_t t = batch + lane;
@@ -584,8 +600,8 @@ static inline hipError_t mark(hipEvent_t *ev, int i, hipStream_t s) {
 
 /* grid of the sweep, compaction and recycling passes for lists of about nI, nB */
 static uint32_t sweep_blocks(uint32_t nI, uint32_t nB) {
-    const uint32_t tI = ((nI + 63) / 64 + 3u) & ~3u, T = tI + (nB + 63) / 64;
-    return grid_for(T, 4 * (GM_BLOCK / 64));
+    const uint32_t T = gm_tiles_inside(nI) + (nB + 63) / 64;
+    return grid_for(T, GM_SW_TPC * (GM_BLOCK / 64));
 }
 
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev) {
@@ -821,8 +837,8 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
     if (pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
     const uint32_t inf = c_code[r];
     double inc = 0.0, fin = 0.0;
-    uint32_t cs = GM_CS_DEFERRED;
-    if (!is_pending(Pn)) cs = predict_values(d, A, k + 1u, carry, Pn, g, inf, inc, fin);
+    gm_cs_t cs = (gm_cs_t)GM_CS_DEFERRED;
+    if (!is_pending(Pn)) cs = cs_code(d, predict_values(d, A, k + 1u, carry, Pn, g, inf, inc, fin), k + 1u);
     GmRec *rec = d.rec + gslot;
     rec_set_a(rec, carry, k + 1u, inf);
     rec_set_b(rec, inc, fin);
@@ -920,7 +936,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArg
          * so each run of equal ids among neighbouring lanes makes one atomic. */
         if (valid) {
             d.grid[q0.y] = d.ndefg + q0.w;
-            d.cstep[q0.x] = GM_CS_INACTIVE;
+            d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
         }
         const uint32_t g = q0.z, r = q0.w;
         const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(r, 1, 64);
@@ -1144,7 +1160,7 @@ __global__ void k_nuc_commit(GmDev d, const uint32_t *site, const double *nucP, 
         rec_set_b(r, 0.0, 0.0);
         rec_set_c(r, nucP[k], w, g);
         r->rxg = k;
-        d.cstep[gslot] = GM_CS_UNKNOWN;
+        d.cstep[gslot] = (gm_cs_t)GM_CS_UNKNOWN;
         const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
         clear_deformed(d, x, y, z);
     }
@@ -1296,7 +1312,7 @@ __global__ void __launch_bounds__(1024) k_scan_row(GmDev d, uint32_t nb) {
 /* Each ACTIVE record after `first` is copied, in slot order, to the scratch
  * array; one that the compacted block will not overwrite is then frozen. */
 __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t base, uint32_t first, uint32_t n,
-                                                           uint32_t k, GmRec *trec, uint32_t *tcs) {
+                                                           uint32_t k, GmRec *trec, gm_cs_t *tcs) {
     uint32_t lo, hi;
     chunk_of(n, lo, hi);
     uint32_t run = d.bcnt[blockIdx.x];
@@ -1318,7 +1334,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t ba
             dst[1] = q1;
             dst[2] = q2;
             dst[3] = q3;
-            tcs[o] = cs;
+            tcs[o] = (gm_cs_t)cs;
             if (c >= keep) {
                 double f = __hiloint2double((int)q0.y, (int)q0.x);
                 const double inc = __hiloint2double((int)q1.y, (int)q1.x);
@@ -1327,7 +1343,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t ba
                 rec_set_a(r, f, k, q0.w);
                 rec_set_b(r, 0.0, 0.0);
                 rec_set_c(r, P, q2.z, q2.w);
-                d.cstep[base + c] = GM_CS_NEVER;
+                d.cstep[base + c] = (gm_cs_t)GM_CS_NEVER;
             }
         }
         run += tot;
@@ -1335,7 +1351,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t ba
 }
 
 __global__ void __launch_bounds__(GM_BLOCK) k_defrag_scatter(GmDev d, uint32_t base, uint32_t first,
-                                                            const GmRec *trec, const uint32_t *tcs) {
+                                                            const GmRec *trec, const gm_cs_t *tcs) {
     const uint32_t n = d.st->dcount;
     for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
         const uint4 *src = reinterpret_cast<const uint4 *>(trec + i);
@@ -1350,7 +1366,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_scatter(GmDev d, uint32_t b
 }
 
 hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
-                                 GmRec *trec, uint32_t *tcs, hipStream_t s) {
+                                 GmRec *trec, gm_cs_t *tcs, hipStream_t s) {
     if (first + 1u >= nnever) return hipSuccess;
     const uint32_t n = nnever - first - 1u;
     const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
@@ -1463,7 +1479,7 @@ static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmSt
     const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
     st->xcells = xc;
     const uint64_t slots = (uint64_t)r.nI + r.nB;
-    const uint64_t tiles = (uint64_t)(((r.nI + 63) / 64 + 3u) & ~3u) + (r.nB + 63) / 64 + 4u;
+    const uint64_t tiles = (uint64_t)gm_tiles_inside(r.nI) + (r.nB + 63) / 64 + GM_SW_TPC;
     bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                 any_top || xc >= w.xstop;
     halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;

@@ -112,6 +112,50 @@ def test_launch_windows_match_at_every_boundary(require_gpu, seed, chunks):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("horizon,seed,chunks", [(16, 6, (5, 8, 3)), (17, 7, (8,))])
+def test_short_horizon_far_cells(require_gpu, monkeypatch, horizon, seed, chunks):
+    """Completion steps are stored modulo 2^15 and only up to a horizon ahead;
+    cells that complete later are FAR and every cell is predicted again at each
+    multiple of the horizon.  With the horizon lowered to a few steps most cells
+    are FAR for a while and the re-predictions happen every few windows: the
+    fronts still agree with the oracle wherever the host takes over."""
+    monkeypatch.setenv("SCORE_GM_HORIZON", str(horizon))
+    dom = synth.make_voronoi_domain(48, 40, 32, n_grains=9, n_nuclei=25, seed=seed)
+    o, g = Oracle(dom, ParamsSpec()), _gm(dom, ParamsSpec())
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+    assert g.result()["step"] > 4 * horizon
+
+
+def test_short_horizon_full_run_and_bounds(require_gpu, monkeypatch):
+    from score_amd.growth import GrowthMachine
+    monkeypatch.setenv("SCORE_GM_HORIZON", "16")
+    dom = synth.workload("voronoi100", seed=2, scale=0.5)
+    o = Oracle(dom, ParamsSpec())
+    o.solve_rxgrowth()
+    g = GrowthMachine(dom, ParamsSpec())
+    g.set_profiling(True)
+    g.solve_rxgrowth()
+    assert_final_equal(o, g)
+    # one re-prediction at each multiple of 16 besides the steps whose scalars changed
+    assert g.profile()["n_reanchor_steps"] >= g.result()["step"] // 16
+    g.close()
+    for bad in ("15", "16385", "x"):
+        monkeypatch.setenv("SCORE_GM_HORIZON", bad)
+        with pytest.raises(abi.ScoreGmError):
+            GrowthMachine(dom, ParamsSpec())
+
+
 def test_small_stepwise_sg(require_gpu):
     dom = synth.make_voronoi_domain(30, 30, 30, n_grains=8, n_nuclei=30, seed=5)
     prm = ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN)
