@@ -62,6 +62,8 @@ def load() -> C.CDLL:
         lib.oracle_disori_omega.restype = C.c_double
         lib.oracle_mobility_weight.argtypes = [C.POINTER(abi.Params), f64p, f64p]
         lib.oracle_mobility_weight.restype = C.c_double
+        lib.oracle_disori_quat.argtypes = [f64p, f64p, f64p]
+        lib.oracle_disori_quat.restype = None
         _lib = lib
     return _lib
 
@@ -87,6 +89,14 @@ def disori_omega(p, q) -> float:
     a, pa = _q(p)
     b, pb = _q(q)
     return load().oracle_disori_omega(pa, pb)
+
+
+def disori_quat(p, q) -> np.ndarray:
+    a, pa = _q(p)
+    b, pb = _q(q)
+    out = np.zeros(4)
+    load().oracle_disori_quat(pa, pb, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def mobility_weight(prm: ParamsSpec, qrx, qdef) -> float:

@@ -21,11 +21,11 @@
  * Parity of this file against the reference is pinned by tests/test_oracle_*.py
  * (closed forms, brute force over the cubic group, invariants, hand-derived
  * first steps, stable compaction, and the statistics of the reference's shipped
- * example/jmak run).  One part is parity unpinned: the bit-level values of the
- * mobility weight (oracle_mobility_weight), which pass through glibc acos/pow/exp and
- * for which the reference ships no golden vector; the tests pin its limits,
- * classes and monotonicity, and the library evaluates the same expressions
- * with the same glibc.
+ * example/jmak run).  The disorientation behind every mobility weight
+ * (oracle_disori_angle, oracle_disori_quat) is pinned bit for bit against the
+ * reference's own SCORE_Math.cpp, built into oracle/_ref (tests/test_ref_pins.py);
+ * the weight's outer expression is the reference's one-line formula on that
+ * angle with the same glibc exp/pow/cos, checked there and by its closed forms.
  */
 #include <math.h>
 #include <stdint.h>
@@ -181,7 +181,7 @@ double oracle_disori_angle(const double *p, const double *q) {
 }
 
 /* disori_fromoriquat_cubic_imm (SCORE_Math.cpp:1067-1131) */
-static void o_disori_quat(const double *p, const double *q, double *quat) {
+void oracle_disori_quat(const double *p, const double *q, double *quat) {
     double r[4], r0[6][4];
     o_misori(p, q, r);
     o_grimmer(r, r0);
@@ -207,7 +207,7 @@ double oracle_mobility_weight(const score_gm_params *p, const double *qrx, const
     }
     if (theta <= O_MAXDISORI_LAGB2HAGB) return -1.0;
     double qdis[4] = {1.0, 0.0, 0.0, 0.0};
-    o_disori_quat(qrx, qdef, qdis);
+    oracle_disori_quat(qrx, qdef, qdis);
     double maxDev40_111 = O_MAXDISORI_TO_40DEG111;
     double _sqrt3 = 1 / sqrt(3.0);
     double oneNinth = 1.0 / 9.0;

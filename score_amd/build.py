@@ -73,9 +73,17 @@ def build_io(out: str | None = None) -> str | None:
     return out
 
 
+REFERENCE = "/root/reference"
+
+
 def build_oracle() -> str:
+    """The oracle (test infrastructure); where the reference tree is present,
+    also its random-number and orientation code into oracle/_ref (the checker's
+    reference leg; the GPU box only uses the prebuilt file)."""
     odir = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-C", odir, "-B"])
+    _run(["make", "-s", "-C", odir, "-B", "liboracle.so"])
+    if os.path.isdir(os.path.join(REFERENCE, "src")):
+        _run(["make", "-s", "-C", odir, "-B", "ref", "REF=" + REFERENCE])
     return os.path.join(odir, "liboracle.so")
 
 
