@@ -1,6 +1,6 @@
 """Build the native parts in-tree.
 
-    python -m score_amd.build            # libscore_gm.so (gfx950) + libsynth.so
+    python -m score_amd.build            # libscore_gm.so (gfx950), libsynth.so, libscore_io.so, libscore_setup.so
     python -m score_amd.build --oracle   # also oracle/liboracle.so (test infrastructure)
 
 hipcc compiles device and host code of the growth machine for gfx950 only,
@@ -73,6 +73,19 @@ def build_io(out: str | None = None) -> str | None:
     return out
 
 
+def build_setup(out: str | None = None) -> str:
+    """The host setup library (UDS files, SCORE's random stream, cuboid structure,
+    CSR nucleation): g++ without -march and with -ffp-contract=off, as the
+    reference's GCC build rounds."""
+    out = out or os.path.join(HERE, "libscore_setup.so")
+    tmp = out + ".tmp"
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "-Wextra", "-I", INCLUDE,
+          os.path.join(CSRC, "score_setup.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 REFERENCE = "/root/reference"
 
 
@@ -91,6 +104,7 @@ def build_all(oracle: bool = True) -> None:
     build_synth()
     build_gm()
     build_io()
+    build_setup()
     if oracle:
         build_oracle()
 
@@ -98,7 +112,7 @@ def build_all(oracle: bool = True) -> None:
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true", help="also build the oracle (test infrastructure)")
-    ap.add_argument("--only", choices=["gm", "synth", "io", "oracle"], default=None)
+    ap.add_argument("--only", choices=["gm", "synth", "io", "setup", "oracle"], default=None)
     a = ap.parse_args(argv)
     if a.only == "gm":
         build_gm()
@@ -106,6 +120,8 @@ def main(argv: list[str] | None = None) -> int:
         build_synth()
     elif a.only == "io":
         build_io()
+    elif a.only == "setup":
+        build_setup()
     elif a.only == "oracle":
         build_oracle()
     else:

@@ -30,6 +30,7 @@
 
 #include "../../include/score_gm.h"
 #include "gm_internal.h"
+#include "score_orient.h"
 
 namespace {
 
@@ -38,7 +39,6 @@ const double kBoltzmann = 1.3806488e-23;
 const double kPi = 3.1415926535897932384626433832795;
 const double kLagbHagb = ((15.0) / (180.0) * (kPi));
 const double kMaxDev40_111 = ((10.0) / (180.0) * (kPi));
-const double kDoubleAccuracy = 5.0e-16;
 const double kInitialDeltaT = 1e-6;
 const double kInfinite = 1e36;
 const double kSmallHeatRate = 0.000277777777777777;
@@ -76,54 +76,8 @@ int hip_fail(hipError_t e, const char *where) {
 
 /* ---------------------------------------------------------------- orientation math */
 
-/* q^-1 p with the Hamilton product of multiply_quat_degraef (SCORE_Math.cpp:230-239, :564-572) */
-void misorientation(const double *p, const double *q, double *r) {
-    const double m0 = q[0], m1 = -1.0 * q[1], m2 = -1.0 * q[2], m3 = -1.0 * q[3];
-    r[0] = m0 * p[0] - m1 * p[1] - m2 * p[2] - m3 * p[3];
-    r[1] = p[0] * m1 + m0 * p[1] + (m2 * p[3] - m3 * p[2]);
-    r[2] = p[0] * m2 + m0 * p[2] + (-(m1 * p[3]) + m3 * p[1]);
-    r[3] = p[0] * m3 + m0 * p[3] + (m1 * p[2] - m2 * p[1]);
-}
-
-/* six Grimmer candidates (SCORE_Math.cpp:592-602) */
-void grimmer(const double *r, double (*c)[4]) {
-    const double a = r[0], b = r[1], cc = r[2], d = r[3];
-    const double fac = 1.0 / sqrt(2.0);
-    c[0][0] = (a + b) * fac; c[0][1] = (a - b) * fac; c[0][2] = (cc + d) * fac; c[0][3] = (cc - d) * fac;
-    c[1][0] = (a + cc) * fac; c[1][1] = (a - cc) * fac; c[1][2] = (b + d) * fac; c[1][3] = (b - d) * fac;
-    c[2][0] = (a + d) * fac; c[2][1] = (a - d) * fac; c[2][2] = (b + cc) * fac; c[2][3] = (b - cc) * fac;
-    c[3][0] = (a + b + cc + d) * 0.5; c[3][1] = (a + b - cc - d) * 0.5; c[3][2] = (a - b + cc - d) * 0.5; c[3][3] = (a - b - cc + d) * 0.5;
-    c[4][0] = (a + b + cc - d) * 0.5; c[4][1] = (a + b - cc + d) * 0.5; c[4][2] = (a - b + cc + d) * 0.5; c[4][3] = (a - b - cc - d) * 0.5;
-    c[5][0] = a; c[5][1] = b; c[5][2] = cc; c[5][3] = d;
-}
-
-/* disori_angle_oriquat_cubic_imm (SCORE_Math.cpp:558-622) */
-double disori_angle(const double *p, const double *q) {
-    double r[4], c[6][4];
-    misorientation(p, q, r);
-    grimmer(r, c);
-    double omega = 0.0;
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 4; j++)
-            if (fabs(c[i][j]) > omega) omega = fabs(c[i][j]);
-    if (omega < (1.0 - kDoubleAccuracy)) return 2.0 * acos(omega);
-    return 0.0;
-}
-
-/* disori_fromoriquat_cubic_imm (SCORE_Math.cpp:1067-1131) */
-void disori_quat(const double *p, const double *q, double *out) {
-    double r[4], c[6][4];
-    misorientation(p, q, r);
-    grimmer(r, c);
-    int mi = 0;
-    double mx = 0.0;
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 4; j++)
-            if (fabs(c[i][j]) > mx) { mx = fabs(c[i][j]); mi = i; }
-    double v[4] = {fabs(c[mi][0]), fabs(c[mi][1]), fabs(c[mi][2]), fabs(c[mi][3])};
-    std::sort(v, v + 4);
-    out[0] = v[3]; out[1] = v[2]; out[2] = v[1]; out[3] = v[0];
-}
+using score_orient::disori_angle;
+using score_orient::disori_quat;
 
 double rh_weight_of_theta(const score_gm_params &p, double theta) {
     return 1.0 - (p.rh_cut * exp(-1.0 * p.rh_trans * pow((theta / kLagbHagb), p.rh_exponent)));

@@ -1,0 +1,259 @@
+"""SCORE's input deck and per-domain setup on the host, through
+libscore_setup.so (include/score_setup.h).
+
+    ens = Ensemble.from_uds("SCORE.Input.uds")
+    dom, prm = ens.domain(jobid=0)            # SCORE's default seeding
+    GrowthMachine(dom, prm).solve_rxgrowth()
+
+A domain is what SCORE holds after solve_REPLACE_CA_STRUCTURE for solitary unit
+`jobid`: cuboid deformed grains picked from the deck's DeformedGrainsPool and
+site-saturated CSR nuclei with Shoemake orientations, all drawn from SCORE's
+Mersenne Twister with the domain's seed (caHdl::init_cahdlprng,
+SCORE_Kernel.cpp:5729-5745).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+from . import abi
+from .synth import DomainArrays, ParamsSpec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscore_setup.so")
+DEFAULT_SEED = -46356  # SCORE_SETUP_DEFAULT_SEED
+ABI_VERSION = 1
+
+ERR = {-1: "ERR_ARG", -2: "ERR_FILE", -3: "ERR_FORMAT", -4: "ERR_RANGE", -5: "ERR_UNSUPPORTED", -6: "ERR_OOM"}
+ERR_ARG, ERR_FILE, ERR_FORMAT, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6
+
+
+class SetupError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{ERR.get(code, code)}: {what}")
+        self.code = code
+
+
+class Mt(C.Structure):
+    _fields_ = [("mt", C.c_uint32 * 624), ("mti", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class EnsembleInfo(C.Structure):
+    _fields_ = [("n_domains", C.c_uint32), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+                ("cellsize", C.c_double), ("defg_mean", C.c_double * 3), ("n_nuclei", C.c_uint32),
+                ("n_world_ori", C.c_uint32), ("n_world_defg", C.c_uint32), ("n_world_rxg", C.c_uint32)]
+
+
+class DomainInfo(C.Structure):
+    _fields_ = [("local_seed", C.c_int64), ("offset", C.c_double * 3), ("ngr", C.c_uint32 * 3),
+                ("n_defg", C.c_uint32), ("nuclei_desired", C.c_uint32), ("nuclei_placed", C.c_uint32),
+                ("collisions", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run `python -m score_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        vp, f64p, u32p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+        sig = {
+            "score_setup_abi_version": ([], C.c_int),
+            "score_setup_last_error": ([], C.c_char_p),
+            "score_mt_init": ([C.POINTER(Mt), C.c_int64], C.c_int),
+            "score_mt_next": ([C.POINTER(Mt)], C.c_double),
+            "score_mt_shoemake": ([C.POINTER(Mt), f64p], None),
+            "score_uds_open": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "score_uds_close": ([vp], None),
+            "score_uds_real": ([vp, C.c_char_p, C.c_char_p, f64p], C.c_int),
+            "score_uds_int": ([vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
+            "score_uds_shape": ([vp, C.c_char_p, u32p, u32p], C.c_int),
+            "score_uds_table_real": ([vp, C.c_char_p, C.c_uint32, C.c_uint32, f64p], C.c_int),
+            "score_setup_ensemble_create": ([vp, C.POINTER(vp)], C.c_int),
+            "score_setup_ensemble_get_info": ([vp, C.POINTER(EnsembleInfo)], C.c_int),
+            "score_setup_world_quat": ([vp, C.c_uint32, f64p], C.c_int),
+            "score_setup_world_defg": ([vp, C.c_uint32, u32p, f64p], C.c_int),
+            "score_setup_ensemble_free": ([vp], None),
+            "score_setup_domain_create": ([vp, C.c_uint32, C.c_int64, C.POINTER(vp)], C.c_int),
+            "score_setup_domain_view": ([vp, C.POINTER(abi.Domain), C.POINTER(abi.Params)], C.c_int),
+            "score_setup_domain_get_info": ([vp, C.POINTER(DomainInfo)], C.c_int),
+            "score_setup_domain_free": ([vp], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        if lib.score_setup_abi_version() != ABI_VERSION:
+            raise RuntimeError("libscore_setup.so ABI version mismatch: rebuild")
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SetupError(rc, load().score_setup_last_error().decode())
+
+
+class MersenneTwister:
+    """randomClass's Mersenne Twister after init(seed) (SCORE_Random.cpp:49-76, :400-458)"""
+
+    def __init__(self, seed: int):
+        self._g = Mt()
+        _check(load().score_mt_init(C.byref(self._g), seed))
+
+    def next(self) -> float:
+        return load().score_mt_next(C.byref(self._g))
+
+    def draws(self, n: int) -> np.ndarray:
+        return np.array([self.next() for _ in range(n)])
+
+    def shoemake(self) -> np.ndarray:
+        q = np.zeros(4)
+        load().score_mt_shoemake(C.byref(self._g), q.ctypes.data_as(C.POINTER(C.c_double)))
+        return q
+
+
+class UdsFile:
+    """A SCORE .uds input deck (io::readDataBlock and its accessors, SCORE_Io.cpp:265-432)."""
+
+    def __init__(self, path: str):
+        self._h = C.c_void_p()
+        _check(load().score_uds_open(os.fsencode(path), C.byref(self._h)))
+
+    def real(self, block: str, key: str) -> float:
+        x = C.c_double()
+        _check(load().score_uds_real(self._h, block.encode(), key.encode(), C.byref(x)))
+        return x.value
+
+    def int(self, block: str, key: str) -> int:
+        x = C.c_int64()
+        _check(load().score_uds_int(self._h, block.encode(), key.encode(), C.byref(x)))
+        return x.value
+
+    def shape(self, block: str) -> tuple[int, int]:
+        r, c = C.c_uint32(), C.c_uint32()
+        _check(load().score_uds_shape(self._h, block.encode(), C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def table_real(self, block: str, row: int, col: int) -> float:
+        x = C.c_double()
+        _check(load().score_uds_table_real(self._h, block.encode(), row, col, C.byref(x)))
+        return x.value
+
+    def close(self) -> None:
+        if self._h:
+            load().score_uds_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclasses.dataclass
+class DomainSetup:
+    """What the setup drew for one domain (score_setup_domain_info)."""
+    local_seed: int
+    offset: tuple
+    ngr: tuple
+    n_defg: int
+    nuclei_desired: int
+    nuclei_placed: int
+    collisions: int
+
+
+class Ensemble:
+    """The settings of the growth path and the world grain pools of one deck."""
+
+    def __init__(self, uds: UdsFile):
+        self.uds = uds
+        self._h = C.c_void_p()
+        _check(load().score_setup_ensemble_create(uds._h, C.byref(self._h)))
+        info = EnsembleInfo()
+        _check(load().score_setup_ensemble_get_info(self._h, C.byref(info)))
+        self.info = info
+
+    @classmethod
+    def from_uds(cls, path: str) -> "Ensemble":
+        return cls(UdsFile(path))
+
+    def world_quat(self, i: int) -> np.ndarray:
+        q = np.zeros(4)
+        _check(load().score_setup_world_quat(self._h, i, q.ctypes.data_as(C.POINTER(C.c_double))))
+        return q
+
+    def world_defg(self, g: int) -> tuple[int, float]:
+        o, r = C.c_uint32(), C.c_double()
+        _check(load().score_setup_world_defg(self._h, g, C.byref(o), C.byref(r)))
+        return o.value, r.value
+
+    def domain(self, jobid: int = 0, seed: int = DEFAULT_SEED) -> tuple[DomainArrays, ParamsSpec, DomainSetup]:
+        """Solitary unit `jobid` as (DomainArrays, ParamsSpec, what was drawn).
+        `seed` is SCORE's ensemble seed: the default gives domain j the seed
+        -(j+1)-10; an explicit one is shared by every domain."""
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.score_setup_domain_create(self._h, jobid, seed, C.byref(h)))
+        try:
+            d, p, info = abi.Domain(), abi.Params(), DomainInfo()
+            _check(lib.score_setup_domain_view(h, C.byref(d), C.byref(p)))
+            _check(lib.score_setup_domain_get_info(h, C.byref(info)))
+            N = d.nx * d.ny * d.nz
+            arr = lambda ptr, n, dt: np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)  # noqa: E731
+            dom = DomainArrays(nx=d.nx, ny=d.ny, nz=d.nz, cellsize=d.cellsize,
+                               cellgrid=arr(d.cellgrid, N, np.uint32),
+                               ori_quat=arr(d.ori_quat, 4 * d.n_ori, np.float64).reshape(-1, 4),
+                               defg_ori=arr(d.defg_ori, d.n_defg, np.uint32),
+                               defg_rho=arr(d.defg_rho, d.n_defg, np.float64),
+                               rxg_ori=arr(d.rxg_ori, d.n_rxg, np.uint32),
+                               rxg_site=arr(d.rxg_site, d.n_rxg, np.uint32))
+            prm = self._params(p)
+            drawn = DomainSetup(local_seed=info.local_seed, offset=tuple(info.offset), ngr=tuple(info.ngr),
+                                n_defg=info.n_defg, nuclei_desired=info.nuclei_desired,
+                                nuclei_placed=info.nuclei_placed, collisions=info.collisions)
+            return dom, prm, drawn
+        finally:
+            lib.score_setup_domain_free(h)
+
+    def _params(self, p: abi.Params) -> ParamsSpec:
+        """ParamsSpec whose to_c() gives the same doubles: the enthalpies are the
+        deck's eV values (ParamsSpec multiplies by echarge as SCORE does)."""
+        u = self.uds
+        B = "BoundaryMobilityModel"
+        ts = np.ctypeslib.as_array(p.sched_time, shape=(p.n_sched,)).copy()
+        tT = np.ctypeslib.as_array(p.sched_temp, shape=(p.n_sched,)).copy()
+        dfr = tuple(np.ctypeslib.as_array(p.defrag_at, shape=(p.n_defrag,)).tolist()) if p.n_defrag else ()
+        return ParamsSpec(mobility_model=p.mobility_model,
+                          rh_hagb_m0=p.rh_hagb_m0, rh_hagb_hact_ev=u.real(B, "RHModelHAGBHact"),
+                          rh_cut=p.rh_cut, rh_trans=p.rh_trans, rh_exponent=p.rh_exponent,
+                          lagb_m0=p.lagb_m0, lagb_hact_ev=u.real(B, "LAGBHact"),
+                          hagb_m0=p.hagb_m0, hagb_hact_ev=u.real(B, "HAGBHact"),
+                          gs_m0=p.gs_m0, gs_hact_ev=u.real(B, "GSHact"),
+                          G0=p.G0, b0=p.b0, sched_time=tuple(ts.tolist()), sched_temp=tuple(tT.tolist()),
+                          maxfill=p.maxfill, xmax=p.xmax, tmax=p.tmax, nmax=p.nmax,
+                          rel_cell_caching=p.rel_cell_caching, rel_cell_recaching=p.rel_cell_recaching,
+                          defrag_at=dfr)
+
+    def close(self) -> None:
+        if self._h:
+            load().score_setup_ensemble_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def domain_from_uds(path: str, jobid: int = 0, seed: int = DEFAULT_SEED):
+    """(DomainArrays, ParamsSpec, DomainSetup) of solitary unit `jobid` of a deck."""
+    return Ensemble.from_uds(path).domain(jobid, seed)

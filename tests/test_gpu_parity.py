@@ -156,6 +156,20 @@ def test_short_horizon_full_run_and_bounds(require_gpu, monkeypatch):
             GrowthMachine(dom, ParamsSpec())
 
 
+@pytest.mark.parametrize("jobid", [0, 2])
+def test_domain_from_uds_deck(require_gpu, jobid):
+    """A domain as SCORE sets it up from an input deck (cuboid grains picked by
+    its Mersenne Twister, CSR nuclei with Shoemake orientations, the deck's
+    schedule and defragmentation points), grown on the GPU and by the oracle."""
+    import os
+    from score_amd import domain_setup as ds
+    deck = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cuboid_fe.uds")
+    dom, prm, _ = ds.domain_from_uds(deck, jobid=jobid)
+    assert len(prm.defrag_at) == 2
+    o, g = run_both(dom, prm)
+    assert_final_equal(o, g)
+
+
 def test_small_stepwise_sg(require_gpu):
     dom = synth.make_voronoi_domain(30, 30, 30, n_grains=8, n_nuclei=30, seed=5)
     prm = ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN)
