@@ -47,16 +47,16 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
     (a completed cell reads its 26 neighbours' cell-grid entries, 104 B, and its
     record, 30 B; a valid attempt is a 4 B atomic; a new record is 30 B plus its
     4 B cell-grid write; retirement writes 4 B of grid and 8 B of counts).  Sweep
-    kernels use what this design's sweep must touch: a 4 B completion step per
-    occupied slot, 16 B of masks per 64-slot tile, 4 B per list entry, and on the
-    steps whose scalars changed 100 B per re-anchored cell (the 64 B record read,
-    its 32 B first sector and 4 B completion step written)."""
+    kernels use what this design's sweep must touch: a 2 B completion-step code
+    per occupied slot, 16 B of masks per 64-slot tile, 4 B per list entry, and on
+    the steps whose scalars changed 100 B per re-anchored cell (the 64 B record
+    read, its 32 B first sector written, its 2 B code read and written)."""
     full, att, new = pf["n_full"], pf["n_attempts"], pf["n_new"]
     scanned, tiles = pf["slots_scanned"], (pf["slots_scanned"] + 63) // 64
     table = {
-        "k_reanchor": (4 * scanned + 100 * pf["n_reanchored"],
-                       "4 B per occupied slot + 100 B per re-anchored cell (record 64 read, 32 written, step 4)"),
-        "k_sweep": (4 * scanned + 16 * tiles, "4 B per occupied slot + 16 B per tile"),
+        "k_reanchor": (100 * pf["n_reanchored"],
+                       "100 B per re-anchored cell (record 64 read, 32 written, code 2 read and 2 written)"),
+        "k_sweep": (2 * scanned + 16 * tiles, "2 B per occupied slot + 16 B per tile"),
         "k_compact": (16 * tiles + 4 * full, "16 B per tile + 4 B per FullRX entry"),
         "k_recycle": (4 * pf["n_recycled"], "4 B per recycled slot handed out"),
         "k_propose": (220 * full + 4 * att, "220 B per completed cell (26 cell-grid entries 104, record 64 read, "
