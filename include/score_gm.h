@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 6
+#define SCORE_GM_ABI_VERSION 7
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -70,7 +70,7 @@ typedef struct score_gm_domain {
 /* Physics and integrator settings (caHdl::init_parameter, SCORE_Kernel.cpp:5747-5880). */
 typedef struct score_gm_params {
     int32_t mobility_model;       /* SCORE_GM_MOB_* */
-    int32_t zener_mode;           /* 0 = no dispersoid drag (only mode implemented) */
+    int32_t zener_mode;           /* dispersoid drag: 0 none, 1 constant (first f/r of the table), 2 f/r linear in time */
     int32_t recovery_mode;        /* 0 = no recovery (only mode implemented) */
     int32_t reserved0;
     /* Rollett-Holm (MobilityModel 2) */
@@ -101,6 +101,16 @@ typedef struct score_gm_params {
     uint32_t n_defrag;
     uint32_t reserved2;
     const double *defrag_at;
+    /* Zener drag p_z = zener_alpha * zener_gamma * f/r (get_zener, update_microchemistry,
+     * get_dtmax_instantslope_zener; SCORE_Kernel.cpp:5370-5456, :5639-5666): the table
+     * EvoMicrochemistry of times (s) and f/r (1/m); mode 2 needs at least two rows,
+     * strictly increasing in time from t = 0 */
+    double zener_alpha;           /* ZenerAlpha */
+    double zener_gamma;           /* ZenerGamma, J/m^2 */
+    uint32_t n_disp;
+    uint32_t reserved3;
+    const double *disp_time;
+    const double *disp_fr;
 } score_gm_params;
 
 /* One row per executed growth step (the values SCORE's master section holds,

@@ -45,6 +45,8 @@
 #define O_INFINITE (1e36)
 #define O_SMALL_HEATRATE (0.000277777777777777)
 #define O_SMALL_HEAT (1.0)
+#define O_SMALL_DRAGGINGRATE (0.027777777)
+#define O_SMALL_ZENERFORCE (100.0)
 #define O_RHOMAX_WELLANNEALED (1e8)
 #define O_DEFAULT_PMAX (0.0)
 #define O_DEFAULT_SMALL_NUMBER (1e-6)
@@ -112,6 +114,8 @@ typedef struct oracle {
     uint32_t *defcnt, *rxcnt;
     score_gm_params p;
     double *sched_t, *sched_T, *defrag_at;
+    double *disp_t, *disp_fr;  /* EvoMicrochemistry: time, f/r */
+    double fr;                 /* myDragData.fr */
     double tprocessingend;
     o_list in, bd;
     /* caHdl scalars */
@@ -268,6 +272,52 @@ static double o_dt_temp(const oracle *o) {
 }
 
 /* get_dtmax_instantslope_cells (SCORE_Kernel.cpp:5519-5558) */
+/* update_microchemistry (SCORE_Kernel.cpp:5370-5405): f/r constant (first entry)
+ * or linear in time; past the table's end it keeps its last value */
+static void o_update_microchemistry(oracle *o) {
+    if (o->p.zener_mode == 0) {
+        o->fr = 0.0;
+        o->pzmin = 0.0;
+        return;
+    }
+    if (o->p.zener_mode == 1) {
+        o->fr = o->disp_fr[0];
+        o->pzmin = o->p.zener_alpha * o->p.zener_gamma * o->fr;
+        return;
+    }
+    uint32_t i = 0, n = o->p.n_disp;
+    while (i < n && o->disp_t[i] < o->t) i++;
+    if (i == 0) o->fr = 0.0; /* t <= 0: the reference reads before its table here */
+    else if (i < n) {
+        double dfrdt = ((o->disp_fr[i] - o->disp_fr[i - 1]) / (o->disp_t[i] - o->disp_t[i - 1]));
+        o->fr = o->disp_fr[i - 1] + dfrdt * (o->t - o->disp_t[i - 1]);
+    }
+    o->pzmin = o->p.zener_alpha * o->p.zener_gamma * o->fr;
+}
+
+/* get_zener (SCORE_Kernel.cpp:5448-5456) */
+static double o_get_zener(const oracle *o) {
+    if (o->p.zener_mode == 0) return 0.0;
+    return (1.0 * o->p.zener_alpha * o->p.zener_gamma * o->fr);
+}
+
+/* get_dtmax_instantslope_zener (SCORE_Kernel.cpp:5639-5666) */
+static double o_dt_zener(const oracle *o) {
+    if (o->p.zener_mode != 2) return O_INFINITE;
+    uint32_t i = 0, n = o->p.n_disp;
+    double dtmax = O_INITIAL_DELTAT;
+    while (i < n && o->disp_t[i] < o->t) i++;
+    if (i == 0) return dtmax;
+    if (i < n) {
+        double dpzdt = o->p.zener_alpha * o->p.zener_gamma *
+                       (fabs(o->disp_fr[i] - o->disp_fr[i - 1]) / (o->disp_t[i] - o->disp_t[i - 1]));
+        if (fabs(dpzdt) <= O_SMALL_DRAGGINGRATE) dpzdt = O_SMALL_DRAGGINGRATE;
+        dtmax = (O_SMALL_ZENERFORCE / dpzdt);
+        return dtmax;
+    }
+    return O_INFINITE;
+}
+
 static double o_dt_cells(const oracle *o) {
     double kT = O_KBOLTZMAN * o->T;
     if (o->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) {
@@ -283,12 +333,14 @@ static double o_dt_cells(const oracle *o) {
     return ((o->p.maxfill * o->cellsize) / v_max);
 }
 
-/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery, Zener and
- * nucleation rules return INFINITE for the modes implemented here. */
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery and nucleation
+ * rules return INFINITE for the modes implemented here. */
 static double o_dt_minimum(const oracle *o) {
     double mn = O_INFINITE;
     double d = o_dt_temp(o);
     if (d < mn) mn = d;
+    double dz = o_dt_zener(o);
+    if (dz < mn) mn = dz;
     double dm = o_dt_cells(o);
     if (dm < mn) mn = dm;
     return mn;
@@ -380,7 +432,7 @@ static int o_sweep(oracle *o, o_list *L) {
             if (o->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) m = e->P * o->mRHHAGB;
             else if (e->P < 0.0) m = o->mLAGB;
             else m = (e->P * o->mGS) + ((1.0 - e->P) * o->mHAGB);
-            double p = (o->Gbhalfsq * (o->rho[e->defg] - O_RHO_RX)) - 0.0;
+            double p = (o->Gbhalfsq * (o->rho[e->defg] - O_RHO_RX)) - o_get_zener(o);
             if (p > O_DOUBLE_ACCURACY) {
                 double v = m * p;
                 v *= o_shape[e->inf];
@@ -502,7 +554,16 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
     if (d->cellsize <= 0.0) return SCORE_GM_ERR_ARG;
     if (p->mobility_model != SCORE_GM_MOB_ROLLETTHOLM && p->mobility_model != SCORE_GM_MOB_SEBALDGOTTSTEIN)
         return SCORE_GM_ERR_ARG;
-    if (p->zener_mode != 0 || p->recovery_mode != 0) return SCORE_GM_ERR_UNSUPPORTED;
+    if (p->zener_mode < 0 || p->zener_mode > 2) return SCORE_GM_ERR_ARG;
+    if (p->zener_mode != 0) {
+        if (p->n_disp < (p->zener_mode == 2 ? 2u : 1u) || !p->disp_time || !p->disp_fr) return SCORE_GM_ERR_ARG;
+        if (p->zener_mode == 2) {
+            if (p->disp_time[0] != 0.0) return SCORE_GM_ERR_ARG;
+            for (uint32_t i = 1; i < p->n_disp; i++)
+                if (!(p->disp_time[i] > p->disp_time[i - 1])) return SCORE_GM_ERR_ARG;
+        }
+    }
+    if (p->recovery_mode != 0) return SCORE_GM_ERR_UNSUPPORTED;
     if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
     if (p->n_defrag && !p->defrag_at) return SCORE_GM_ERR_ARG;
     uint64_t N = (uint64_t)d->nx * d->ny * d->nz;
@@ -519,6 +580,7 @@ void oracle_destroy(oracle *o) {
     if (!o) return;
     free(o->grid); free(o->defq); free(o->rxq); free(o->rho); free(o->site);
     free(o->defcnt); free(o->rxcnt); free(o->sched_t); free(o->sched_T); free(o->defrag_at);
+    free(o->disp_t); free(o->disp_fr);
     o_list_free(&o->in); o_list_free(&o->bd);
     free(o->traj);
     free(o);
@@ -547,8 +609,10 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o->sched_t = (double *)malloc(sizeof(double) * p->n_sched);
     o->sched_T = (double *)malloc(sizeof(double) * p->n_sched);
     o->defrag_at = (double *)malloc(sizeof(double) * (p->n_defrag + 1));
+    o->disp_t = (double *)malloc(sizeof(double) * (p->n_disp + 1));
+    o->disp_fr = (double *)malloc(sizeof(double) * (p->n_disp + 1));
     if (!o->grid || !o->defq || !o->rxq || !o->rho || !o->site || !o->defcnt || !o->rxcnt || !o->sched_t ||
-        !o->sched_T || !o->defrag_at) {
+        !o->sched_T || !o->defrag_at || !o->disp_t || !o->disp_fr) {
         oracle_destroy(o);
         return SCORE_GM_ERR_OOM;
     }
@@ -567,6 +631,13 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o->p.sched_time = o->sched_t;
     o->p.sched_temp = o->sched_T;
     o->p.defrag_at = o->defrag_at;
+    if (p->zener_mode != 0) {
+        memcpy(o->disp_t, p->disp_time, sizeof(double) * p->n_disp);
+        memcpy(o->disp_fr, p->disp_fr, sizeof(double) * p->n_disp);
+    }
+    o->p.disp_time = o->disp_t;
+    o->p.disp_fr = o->disp_fr;
+    o->fr = 0.0;
     /* init_processing (SCORE_Kernel.cpp:5888-5901) */
     o->tprocessingend = 0.0;
     for (uint32_t i = 0; i < p->n_sched; i++)
@@ -684,7 +755,7 @@ int oracle_step(oracle *o) {
     o->t += o->dt;
     o_update_temperature(o);
     o_update_atomistic(o);
-    o->pzmin = 0.0;
+    o_update_microchemistry(o);
     o_update_mobilities(o);
     o->dt = o_dt_minimum(o);
     o->step++;

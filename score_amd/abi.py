@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 OK = 0
 ERR_ARG = -1
@@ -95,6 +95,9 @@ class Params(C.Structure):
         ("rel_cell_caching", C.c_double), ("rel_cell_recaching", C.c_double),
         ("n_defrag", C.c_uint32), ("reserved2", C.c_uint32),
         ("defrag_at", _f64p),
+        ("zener_alpha", C.c_double), ("zener_gamma", C.c_double),
+        ("n_disp", C.c_uint32), ("reserved3", C.c_uint32),
+        ("disp_time", _f64p), ("disp_fr", _f64p),
     ]
 
 

@@ -43,6 +43,8 @@ const double kInitialDeltaT = 1e-6;
 const double kInfinite = 1e36;
 const double kSmallHeatRate = 0.000277777777777777;
 const double kSmallHeat = 1.0;
+const double kSmallDraggingRate = 0.027777777; /* SMALL_DRAGGINGRATE, SCORE_Defs_Physics.h:84 */
+const double kSmallZenerForce = 100.0;         /* SMALL_ZENERFORCE, :87 */
 const double kRhoWellAnnealed = 1e8;
 const double kDefaultPmax = 0.0;
 const double kDefaultSmallNumber = 1e-6;
@@ -149,6 +151,8 @@ struct score_gm_handle {
     /* caHdl scalars */
     double t = 0.0, X = 0.0, Xcells = 0.0, dt = 0.0, T = 0.0, G = 0.0, b = 0.0, Gbhalfsq = 0.0;
     double mLAGB = 0.0, mHAGB = 0.0, mGS = 0.0, mRHHAGB = 0.0, pzmin = 0.0, rhomax = 0.0;
+    double fr = 0.0; /* dispersoid f/r (myDragData.fr) */
+    std::vector<double> disp_t, disp_fr;
     double Pmax = kDefaultPmax, regPmax = kDefaultPmax;
     uint32_t step = 0;
     uint32_t nI = 0, nB = 0;
@@ -239,7 +243,18 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
         return fail(SCORE_GM_ERR_ARG, "grain ids exceed CA_GRAINIDS_MAXIMUM");
     if (prm->mobility_model != SCORE_GM_MOB_ROLLETTHOLM && prm->mobility_model != SCORE_GM_MOB_SEBALDGOTTSTEIN)
         return fail(SCORE_GM_ERR_ARG, "unknown mobility model");
-    if (prm->zener_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "dispersoid drag is not implemented");
+    if (prm->zener_mode < 0 || prm->zener_mode > 2) return fail(SCORE_GM_ERR_ARG, "unknown Zener drag mode");
+    if (prm->zener_mode != 0) {
+        if (prm->n_disp < (prm->zener_mode == 2 ? 2u : 1u) || !prm->disp_time || !prm->disp_fr)
+            return fail(SCORE_GM_ERR_ARG, "Zener drag needs its f/r table (two rows for the time-dependent mode)");
+        if (prm->zener_mode == 2) {
+            /* SCORE reads the entry before the first one for times up to disp_time[0] */
+            if (prm->disp_time[0] != 0.0) return fail(SCORE_GM_ERR_ARG, "time-dependent drag table must start at t = 0");
+            for (uint32_t i = 1; i < prm->n_disp; i++)
+                if (!(prm->disp_time[i] > prm->disp_time[i - 1]))
+                    return fail(SCORE_GM_ERR_ARG, "drag table times must increase strictly");
+        }
+    }
     if (prm->recovery_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "recovery is not implemented");
     if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
@@ -303,6 +318,57 @@ double dt_temp(const score_gm_handle *h) {
     return kInfinite;
 }
 
+/* update_microchemistry (SCORE_Kernel.cpp:5370-5405): f/r of the dispersoids,
+ * constant (the first entry) or linear in time; past the table's end f/r keeps
+ * its last value.  SCORE's prologue does not call it, so the first step runs
+ * without drag. */
+void update_microchemistry(score_gm_handle *h) {
+    const int mode = h->p.zener_mode;
+    if (mode == 0) {
+        h->fr = 0.0;
+        h->pzmin = 0.0;
+        return;
+    }
+    if (mode == 1) {
+        h->fr = h->disp_fr[0];
+        h->pzmin = h->p.zener_alpha * h->p.zener_gamma * h->fr;
+        return;
+    }
+    const size_t n = h->disp_t.size();
+    size_t i = 0;
+    while (i < n && h->disp_t[i] < h->t) i++;
+    if (i == 0) h->fr = 0.0; /* t <= 0 (a negative time step): SCORE reads before its table here */
+    else if (i < n) {
+        const double dfrdt = ((h->disp_fr[i] - h->disp_fr[i - 1]) / (h->disp_t[i] - h->disp_t[i - 1]));
+        h->fr = h->disp_fr[i - 1] + dfrdt * (h->t - h->disp_t[i - 1]);
+    }
+    h->pzmin = h->p.zener_alpha * h->p.zener_gamma * h->fr;
+}
+
+/* get_zener (SCORE_Kernel.cpp:5448-5456) */
+double zener_force(const score_gm_handle *h) {
+    if (h->p.zener_mode == 0) return 0.0;
+    return (1.0 * h->p.zener_alpha * h->p.zener_gamma * h->fr);
+}
+
+/* get_dtmax_instantslope_zener (SCORE_Kernel.cpp:5639-5666) */
+double dt_zener(const score_gm_handle *h) {
+    if (h->p.zener_mode != 2) return kInfinite;
+    const size_t n = h->disp_t.size();
+    size_t i = 0;
+    double dtmax = kInitialDeltaT;
+    while (i < n && h->disp_t[i] < h->t) i++;
+    if (i == 0) return dtmax;
+    if (i < n) {
+        double dpzdt = h->p.zener_alpha * h->p.zener_gamma *
+                       (fabs(h->disp_fr[i] - h->disp_fr[i - 1]) / (h->disp_t[i] - h->disp_t[i - 1]));
+        if (fabs(dpzdt) <= kSmallDraggingRate) dpzdt = kSmallDraggingRate;
+        dtmax = (kSmallZenerForce / dpzdt);
+        return dtmax;
+    }
+    return kInfinite;
+}
+
 /* get_dtmax_instantslope_cells (SCORE_Kernel.cpp:5519-5558) */
 double dt_cells(const score_gm_handle *h) {
     const double kT = kBoltzmann * h->T;
@@ -319,12 +385,14 @@ double dt_cells(const score_gm_handle *h) {
     return ((h->p.maxfill * h->cellsize) / v_max);
 }
 
-/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); recovery, drag and nucleation
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery and nucleation
  * rules are INFINITE for the modes implemented */
 double dt_minimum(const score_gm_handle *h) {
     double mn = kInfinite;
     const double a = dt_temp(h);
     if (a < mn) mn = a;
+    const double z = dt_zener(h);
+    if (z < mn) mn = z;
     const double c = dt_cells(h);
     if (c < mn) mn = c;
     return mn;
@@ -516,7 +584,7 @@ GmSweepParams sweep_params(const score_gm_handle *h) {
     memset(&sp, 0, sizeof(sp));
     sp.model = h->p.mobility_model;
     sp.mRH = h->mRHHAGB; sp.mLAGB = h->mLAGB; sp.mHAGB = h->mHAGB; sp.mGS = h->mGS;
-    sp.Gbhalfsq = h->Gbhalfsq; sp.zener = 0.0; sp.dt = h->dt; sp.inv_dx = h->inv_dx;
+    sp.Gbhalfsq = h->Gbhalfsq; sp.zener = zener_force(h); sp.dt = h->dt; sp.inv_dx = h->inv_dx;
     return sp;
 }
 
@@ -525,7 +593,7 @@ void advance_scalars(score_gm_handle *h) {
     h->t += h->dt;
     update_temperature(h);
     update_atomistic(h);
-    h->pzmin = 0.0;
+    update_microchemistry(h);
     update_mobilities(h);
     h->dt = dt_minimum(h);
 }
@@ -546,7 +614,7 @@ uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &s
     uint32_t w = std::min<uint32_t>(wmax, GM_WMAX);
     if (imin - 1.0 < (double)w) w = imin >= 2.0 ? (uint32_t)(imin - 1.0) : 1u;
     if (w <= 1) return 1;
-    const double t0 = h->t, T0 = h->T, G0 = h->G, b0 = h->b, gb0 = h->Gbhalfsq, dt0 = h->dt, pz0 = h->pzmin;
+    const double t0 = h->t, T0 = h->T, G0 = h->G, b0 = h->b, gb0 = h->Gbhalfsq, dt0 = h->dt, pz0 = h->pzmin, fr0 = h->fr;
     const double m0 = h->mLAGB, m1 = h->mHAGB, m2 = h->mGS, m3 = h->mRHHAGB;
     const uint32_t step0 = h->step;
     uint32_t n = 1;
@@ -557,7 +625,7 @@ uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &s
         if (done(h) || !same_scalars(sweep_params(h), sp) || h->step % h->d.horizon == 0) break;
         n++;
     }
-    h->t = t0; h->T = T0; h->G = G0; h->b = b0; h->Gbhalfsq = gb0; h->dt = dt0; h->pzmin = pz0;
+    h->t = t0; h->T = T0; h->G = G0; h->b = b0; h->Gbhalfsq = gb0; h->dt = dt0; h->pzmin = pz0; h->fr = fr0;
     h->mLAGB = m0; h->mHAGB = m1; h->mGS = m2; h->mRHHAGB = m3;
     h->step = step0;
     return n;
@@ -842,6 +910,12 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->p.sched_temp = h->sched_T.data();
     if (prm->n_defrag) h->defrag_at.assign(prm->defrag_at, prm->defrag_at + prm->n_defrag);
     h->p.defrag_at = h->defrag_at.data();
+    if (prm->zener_mode != 0) {
+        h->disp_t.assign(prm->disp_time, prm->disp_time + prm->n_disp);
+        h->disp_fr.assign(prm->disp_fr, prm->disp_fr + prm->n_disp);
+    }
+    h->p.disp_time = h->disp_t.data();
+    h->p.disp_fr = h->disp_fr.data();
     h->cellsize = dom->cellsize;
     h->inv_dx = (1.0 / dom->cellsize);
     /* init_processing (SCORE_Kernel.cpp:5888-5901) */

@@ -282,6 +282,10 @@ struct score_setup_ensemble {
     int32_t model = 0;
     double lagb_m0 = 0, lagb_h = 0, hagb_m0 = 0, hagb_h = 0, gs_m0 = 0, gs_h = 0;
     double rh_m0 = 0, rh_h = 0, rh_cut = 0, rh_trans = 0, rh_exp = 0;
+    /* DragParameter, EvoMicrochemistry */
+    int32_t zener_mode = 0;
+    double zener_alpha = 0, zener_gamma = 0;
+    std::vector<double> disp_t, disp_fr;
     /* AnnealingSchedule */
     std::vector<double> ttime, ttemp;
     /* HeuristicRXFrontListDefragmentation: X, ascending */
@@ -465,7 +469,7 @@ int read_physics(const score_uds *u, score_setup_ensemble *e) {
         (e->rh_m0 <= 0.0 || e->rh_h <= 0.0 || e->rh_cut <= 0.0 || e->rh_trans <= 0.0 || e->rh_exp <= 0.0))
         return fail(SCORE_SETUP_ERR_RANGE, "Unrealistic mobility model parameterization");
     /* recovery and Zener drag (init_parameter_recovery, _drag, SCORE_Kernel.cpp:595-665):
-     * the growth machine implements neither */
+     * no recovery model is implemented; drag is constant (1) or linear in time (2) */
     const char *R = "RecoveryParameter";
     if ((rc = uds_int(u, R, "RecoveryConsider", &v))) return rc;
     const long recovery = v;
@@ -477,13 +481,27 @@ int read_physics(const score_uds *u, score_setup_ensemble *e) {
         if ((rc = uds_real(u, R, k, &x))) return rc;
     if (recovery >= 1 && recovery <= 3) return fail(SCORE_SETUP_ERR_UNSUPPORTED, "RecoveryConsider (recovery models) is not supported");
     const char *D = "DragParameter";
-    if ((rc = uds_int(u, D, "ZenerConsider", &v)) || (rc = uds_real(u, D, "ZenerAlpha", &x)) || (rc = uds_real(u, D, "ZenerGamma", &x)))
+    if ((rc = uds_int(u, D, "ZenerConsider", &v)) || (rc = uds_real(u, D, "ZenerAlpha", &e->zener_alpha)) ||
+        (rc = uds_real(u, D, "ZenerGamma", &e->zener_gamma)))
         return rc;
-    if (v == 1 || v == 2) return fail(SCORE_SETUP_ERR_UNSUPPORTED, "ZenerConsider (Zener drag) is not supported");
+    e->zener_mode = (v == 1 || v == 2) ? (int32_t)v : 0;
     const UdsBlock *b;
     if ((rc = block_or_fail(u, "EvoMicrochemistry", &b))) return rc;
-    for (size_t t = 0; t < b->lines.size(); t++)
-        if ((rc = table_real(*b, t, 0, &x)) || (rc = table_real(*b, t, 1, &x))) return rc;
+    for (size_t t = 0; t < b->lines.size(); t++) {
+        double ti, fri;
+        if ((rc = table_real(*b, t, 0, &ti)) || (rc = table_real(*b, t, 1, &fri))) return rc;
+        e->disp_t.push_back(ti);
+        e->disp_fr.push_back(fri);
+    }
+    if (e->zener_mode != 0 && e->disp_t.empty()) return fail(SCORE_SETUP_ERR_RANGE, "EvoMicrochemistry is empty");
+    if (e->zener_mode == 2) {
+        /* SCORE reads before the table's first entry for times up to its first time */
+        bool ok = e->disp_t.size() >= 2 && e->disp_t[0] == 0.0;
+        for (size_t t = 1; ok && t < e->disp_t.size(); t++) ok = e->disp_t[t] > e->disp_t[t - 1];
+        if (!ok)
+            return fail(SCORE_SETUP_ERR_UNSUPPORTED,
+                        "time-dependent Zener drag needs an EvoMicrochemistry table from t = 0 with increasing times");
+    }
     return 0;
 }
 
@@ -587,7 +605,7 @@ struct score_setup_domain {
     std::vector<uint32_t> defg_ori, rxg_ori, rxg_site;
     std::vector<double> defg_rho;
     score_gm_params prm{};
-    std::vector<double> sched_t, sched_T, defrag_at;
+    std::vector<double> sched_t, sched_T, defrag_at, disp_t, disp_fr;
 };
 
 extern "C" {
@@ -862,7 +880,6 @@ int score_setup_domain_create(const score_setup_ensemble *e, uint32_t jobid, int
     score_gm_params &p = s->prm;
     memset(&p, 0, sizeof(p));
     p.mobility_model = e->model;
-    p.zener_mode = 0;
     p.recovery_mode = 0;
     p.rh_hagb_m0 = e->rh_m0;
     p.rh_hagb_hact = e->rh_h;
@@ -895,6 +912,15 @@ int score_setup_domain_create(const score_setup_ensemble *e, uint32_t jobid, int
     }
     p.n_defrag = (uint32_t)s->defrag_at.size();
     p.defrag_at = s->defrag_at.empty() ? nullptr : s->defrag_at.data();
+    /* caHdl::init_zenerdrag (SCORE_Kernel.cpp:5904-5911) */
+    p.zener_mode = e->zener_mode;
+    p.zener_alpha = e->zener_alpha;
+    p.zener_gamma = e->zener_gamma;
+    s->disp_t = e->disp_t;
+    s->disp_fr = e->disp_fr;
+    p.n_disp = e->zener_mode ? (uint32_t)s->disp_t.size() : 0u;
+    p.disp_time = s->disp_t.empty() ? nullptr : s->disp_t.data();
+    p.disp_fr = s->disp_fr.empty() ? nullptr : s->disp_fr.data();
     *out = s;
     return 0;
 }

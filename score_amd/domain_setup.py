@@ -231,6 +231,8 @@ class Ensemble:
         ts = np.ctypeslib.as_array(p.sched_time, shape=(p.n_sched,)).copy()
         tT = np.ctypeslib.as_array(p.sched_temp, shape=(p.n_sched,)).copy()
         dfr = tuple(np.ctypeslib.as_array(p.defrag_at, shape=(p.n_defrag,)).tolist()) if p.n_defrag else ()
+        dt_ = tuple(np.ctypeslib.as_array(p.disp_time, shape=(p.n_disp,)).tolist()) if p.n_disp else ()
+        dfr_ = tuple(np.ctypeslib.as_array(p.disp_fr, shape=(p.n_disp,)).tolist()) if p.n_disp else ()
         return ParamsSpec(mobility_model=p.mobility_model,
                           rh_hagb_m0=p.rh_hagb_m0, rh_hagb_hact_ev=u.real(B, "RHModelHAGBHact"),
                           rh_cut=p.rh_cut, rh_trans=p.rh_trans, rh_exponent=p.rh_exponent,
@@ -240,7 +242,8 @@ class Ensemble:
                           G0=p.G0, b0=p.b0, sched_time=tuple(ts.tolist()), sched_temp=tuple(tT.tolist()),
                           maxfill=p.maxfill, xmax=p.xmax, tmax=p.tmax, nmax=p.nmax,
                           rel_cell_caching=p.rel_cell_caching, rel_cell_recaching=p.rel_cell_recaching,
-                          defrag_at=dfr)
+                          defrag_at=dfr, zener_mode=p.zener_mode, zener_alpha=p.zener_alpha,
+                          zener_gamma=p.zener_gamma, disp_time=dt_, disp_fr=dfr_)
 
     def close(self) -> None:
         if self._h:

@@ -178,13 +178,20 @@ class ParamsSpec:
     rel_cell_caching: float = 0.2
     rel_cell_recaching: float = 1.0
     defrag_at: tuple = ()
+    zener_mode: int = 0                 # 0 none, 1 constant, 2 time-dependent
+    zener_alpha: float = 1.5
+    zener_gamma: float = 0.324          # J/m^2
+    disp_time: tuple = ()               # s
+    disp_fr: tuple = ()                 # f/r, 1/m
 
     def to_c(self):
         keep = [np.ascontiguousarray(self.sched_time, np.float64), np.ascontiguousarray(self.sched_temp, np.float64),
-                np.ascontiguousarray(self.defrag_at if len(self.defrag_at) else [0.0], np.float64)]
+                np.ascontiguousarray(self.defrag_at if len(self.defrag_at) else [0.0], np.float64),
+                np.ascontiguousarray(self.disp_time if len(self.disp_time) else [0.0], np.float64),
+                np.ascontiguousarray(self.disp_fr if len(self.disp_fr) else [0.0], np.float64)]
         p = abi.Params()
         p.mobility_model = self.mobility_model
-        p.zener_mode = 0
+        p.zener_mode = self.zener_mode
         p.recovery_mode = 0
         p.rh_hagb_m0 = self.rh_hagb_m0
         p.rh_hagb_hact = self.rh_hagb_hact_ev * ECHARGE
@@ -200,6 +207,10 @@ class ParamsSpec:
         p.rel_cell_caching, p.rel_cell_recaching = self.rel_cell_caching, self.rel_cell_recaching
         p.n_defrag = len(self.defrag_at)
         p.defrag_at = _ptr(keep[2], C.c_double)
+        p.zener_alpha, p.zener_gamma = self.zener_alpha, self.zener_gamma
+        p.n_disp = len(self.disp_time)
+        p.disp_time = _ptr(keep[3], C.c_double)
+        p.disp_fr = _ptr(keep[4], C.c_double)
         return p, keep
 
 

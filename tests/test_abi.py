@@ -107,7 +107,8 @@ def test_invalid_arguments_rejected_before_device_use():
     with pytest.raises(abi.ScoreGmError) as e:
         GrowthMachine(dom)
     assert e.value.code == abi.ERR_ARG
-    # thresholds announced but not given; then a drag model the library does not have
+    # thresholds announced but not given; drag without its table or with a table
+    # that does not start at t = 0; a recovery model the library does not have
     lib = abi.load()
     dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
     cd, keep_d = dom.to_c()
@@ -117,6 +118,16 @@ def test_invalid_arguments_rejected_before_device_use():
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     cp, keep_p = synth.ParamsSpec().to_c()
     cp.zener_mode = 1
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec(zener_mode=2, disp_time=(10.0, 20.0), disp_fr=(1e5, 2e5)).to_c()
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec(zener_mode=2, disp_time=(0.0, 0.0), disp_fr=(1e5, 2e5)).to_c()
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec().to_c()
+    cp.zener_mode = 3
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec().to_c()
+    cp.recovery_mode = 1
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_UNSUPPORTED
     assert not h.value
 

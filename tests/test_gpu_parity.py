@@ -170,6 +170,35 @@ def test_domain_from_uds_deck(require_gpu, jobid):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("mode,disp", [(1, ((0.0,), (6.0e4,))), (2, ((0.0, 2000.0, 20000.0), (0.0, 1.0e5, 2.0e4)))])
+def test_zener_drag_stepwise(require_gpu, mode, disp):
+    """Dispersoid drag p_z = alpha*gamma*f/r, constant or linear in time: the first
+    step runs without it, the time step follows the drag rule while f/r changes
+    (every step re-anchors), and p_z lowers every increment."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=7, n_nuclei=18, seed=21)
+    prm = ParamsSpec(zener_mode=mode, zener_alpha=1.5, zener_gamma=0.324, disp_time=disp[0], disp_fr=disp[1])
+    o, g = run_both(dom, prm, stepwise=True)
+    assert_final_equal(o, g)
+    assert g.result()["completed"]
+
+
+def test_zener_drag_in_launch_windows(require_gpu):
+    dom = synth.make_voronoi_domain(48, 40, 32, n_grains=9, n_nuclei=25, seed=8)
+    prm = ParamsSpec(zener_mode=1, zener_alpha=1.5, zener_gamma=0.324, disp_time=(0.0,), disp_fr=(5.0e4,))
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        do, dg = o.step(7), g.step(7)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+
+
 def test_small_stepwise_sg(require_gpu):
     dom = synth.make_voronoi_domain(30, 30, 30, n_grains=8, n_nuclei=30, seed=5)
     prm = ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN)

@@ -114,6 +114,66 @@ def test_second_dt_and_increment_closed_form():
         assert inc[k] == pytest.approx(0.1 * shape[cls] * (1e15 - 1e11) / 1e15, rel=1e-9)
 
 
+def test_zener_drag_constant_closed_form():
+    """constant drag p_z = alpha*gamma*f/r (get_zener, SCORE_Kernel.cpp:5448-5456): the
+    first step runs without it (SCORE's prologue does not update the microchemistry),
+    then the time step is maxfill*dx/(m*(Gb^2/2*rhomax - p_z)) and a face cell advances
+    by maxfill*(Gb^2/2*(rho - 1e11) - p_z)/(Gb^2/2*rho - p_z) per step"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    fr0 = 5.0e5
+    prm = ParamsSpec(zener_mode=1, zener_alpha=1.5, zener_gamma=0.324, disp_time=(0.0,), disp_fr=(fr0,))
+    o, o0 = Oracle(dom, prm), Oracle(dom)
+    o.step(1)
+    o0.step(1)
+    assert np.array_equal(o.front(abi.LIST_INSIDE)["rxfrac"], o0.front(abi.LIST_INSIDE)["rxfrac"])
+    before = o.front(abi.LIST_INSIDE)["rxfrac"].copy()
+    T = 1000.0
+    G = 6.92e10 * (-1.38e-9 * T ** 3 + 1.21e-6 * T ** 2 - 4.58e-4 * T + 1.0)
+    gb = 0.5 * G * 2.48e-10 ** 2
+    pz = 1.5 * 0.324 * fr0
+    m = 0.1 * math.exp(-2.8 * EC / (KB * T))
+    assert o.dt() == pytest.approx(0.1 * 0.5e-6 / (m * (gb * 1e15 - pz)), rel=1e-12)
+    o.step(1)
+    inc = o.front(abi.LIST_INSIDE)["rxfrac"][1:27] - before[1:27]
+    shape = {1: 1.0, 2: 0.7957, 3: 0.6494}
+    for k, ((dx, dy, dz), code) in enumerate(ORDER):
+        cls = abs(dx) + abs(dy) + abs(dz)
+        assert inc[k] == pytest.approx(0.1 * shape[cls] * (gb * (1e15 - 1e11) - pz) / (gb * 1e15 - pz), rel=1e-9)
+    o.close()
+    o0.close()
+
+
+def test_zener_drag_time_dependent_dt_rule():
+    """f/r linear in time (update_microchemistry, SCORE_Kernel.cpp:5370-5405): the time
+    step is SMALL_ZENERFORCE / (alpha*gamma*|d(f/r)/dt|) while inside the table
+    (get_dtmax_instantslope_zener, :5639-5666); past its end f/r keeps its last
+    interpolated value and the drag rule is INFINITE"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    a, g, slope = 1.5, 0.324, 1.0e6 / 100.0
+    prm = ParamsSpec(zener_mode=2, zener_alpha=a, zener_gamma=g, disp_time=(0.0, 0.05), disp_fr=(0.0, 0.05 * slope))
+    o = Oracle(dom, prm)
+    o.step(4)
+    tr = o.trajectory()
+    want = 100.0 / (a * g * (abs(0.05 * slope - 0.0) / (0.05 - 0.0)))
+    # step 0 ran with the initial 1e-6 s; then the drag rule sets the step while t < 0.05
+    assert tr["dt"][0] == 1e-6 and all(d == want for d, t in zip(tr["dt"][1:], tr["t"][1:]) if t < 0.05)
+    # past the table: the cells rule with the last f/r reached inside it
+    while o.trajectory()["t"][-1] < 0.05:
+        o.step(1)
+    tr = o.trajectory()
+    t_in = max(t for t in tr["t"] if t <= 0.05)
+    fr_last = 0.0 + ((0.05 * slope - 0.0) / (0.05 - 0.0)) * (t_in - 0.0)
+    T = 1000.0
+    G = 6.92e10 * (-1.38e-9 * T ** 3 + 1.21e-6 * T ** 2 - 4.58e-4 * T + 1.0)
+    gb = 0.5 * G * 2.48e-10 ** 2
+    m = 0.1 * math.exp(-2.8 * EC / (KB * T))
+    o.step(1)
+    assert o.dt() == pytest.approx(0.1 * 0.5e-6 / (m * (gb * 1e15 - a * g * fr_last)), rel=1e-9)
+    o.close()
+
+
 def test_recycled_slot_reused_first():
     """slot 0 (retired nucleus) is the first recycled slot: the next infection takes it"""
     n = 20

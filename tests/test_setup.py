@@ -230,7 +230,7 @@ def test_params_from_deck(tmp_path):
     assert lib.score_setup_domain_view(h, C.byref(d), C.byref(p)) == 0
     q, keep = prm.to_c()
     for f, _ in abi.Params._fields_:
-        if f in ("sched_time", "sched_temp", "defrag_at"):
+        if f in ("sched_time", "sched_temp", "defrag_at", "disp_time", "disp_fr"):
             continue
         assert getattr(p, f) == getattr(q, f), f
     lib.score_setup_domain_free(h)
@@ -240,13 +240,18 @@ def test_params_from_deck(tmp_path):
                                                             ("RuntimeControl", "OnTheFlyDefragmentation"): "0"})))
     _, p2, _ = e2.domain(0)
     assert p2.maxfill == 0.1 and p2.rel_cell_caching == 0.10 and p2.defrag_at == ()
+    # Zener drag from the deck: alpha, gamma and the f/r table (float columns)
+    assert prm.zener_mode == 0 and prm.disp_time == ()
+    e3 = ds.Ensemble.from_uds(write(tmp_path, mk.deck_text({("DragParameter", "ZenerConsider"): "2"})))
+    _, p3, _ = e3.domain(0)
+    assert (p3.zener_mode, p3.zener_alpha, p3.zener_gamma) == (2, 1.5, 0.324)
+    assert p3.disp_time == (0.0, f32(360000.0)) and p3.disp_fr == (0.0, 0.0)
 
 
 @pytest.mark.parametrize("override,code", [
     ((("NucleationModeling", "CSRNucleation"), "2"), ds.ERR_UNSUPPORTED),
     ((("NucleationModeling", "GBNucleation"), "1"), ds.ERR_UNSUPPORTED),
     ((("RecoveryParameter", "RecoveryConsider"), "1"), ds.ERR_UNSUPPORTED),
-    ((("DragParameter", "ZenerConsider"), "1"), ds.ERR_UNSUPPORTED),
     ((("EnsembleDefinition", "DefStructureSynthesis"), "4"), ds.ERR_UNSUPPORTED),
     ((("RuntimeControl", "XMAX"), "1.5"), ds.ERR_RANGE),
     ((("RuntimeControl", "NMAX"), "0"), ds.ERR_RANGE),
