@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 7
+#define SCORE_GM_ABI_VERSION 8
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -71,7 +71,7 @@ typedef struct score_gm_domain {
 typedef struct score_gm_params {
     int32_t mobility_model;       /* SCORE_GM_MOB_* */
     int32_t zener_mode;           /* dispersoid drag: 0 none, 1 constant (first f/r of the table), 2 f/r linear in time */
-    int32_t recovery_mode;        /* 0 = no recovery (only mode implemented) */
+    int32_t recovery_mode;        /* 0 none, 1 Nes vacancy-core diffusion, 2 Nes solute drag, 3 Michalak-Paxton */
     int32_t reserved0;
     /* Rollett-Holm (MobilityModel 2) */
     double rh_hagb_m0;            /* m^4/(J s) */
@@ -111,6 +111,18 @@ typedef struct score_gm_params {
     uint32_t reserved3;
     const double *disp_time;
     const double *disp_fr;
+    /* recovery of the deformed grains' dislocation density after every step
+     * (update_deformedsubstructure, SCORE_Kernel.cpp:5408-5437; SCORE_RecoveryModels.cpp:50-125)
+     * and its time-step rule (get_dtmax_instantslope_rho, SCORE_Kernel.cpp:5594-5636):
+     * mode 1 grows the network radius 1/sqrt(rho) of every grain, mode 2 (as in SCORE)
+     * changes no rho but limits the step, mode 3 scales rho0 by a fixed function of time */
+    double rec_vacancy_d0;        /* RecoveryVacancyDiffPreexp */
+    double rec_vacancy_hact;      /* RecoveryVacancyDiffHact, J */
+    double rec_solute_d0;         /* RecoverySoluteDiffPreexp */
+    double rec_solute_hact;       /* RecoverySoluteDiffHact, J */
+    double rec_solute_ls;         /* RecoverySoluteLsPropFactor */
+    double rec_solute_conc;       /* RecoverySoluteConcentration */
+    double rec_alpha3, rec_kappa2, rec_c3, rec_c5;   /* RecoveryParameterAlpha3, Kappa2, C3, C5 */
 } score_gm_params;
 
 /* One row per executed growth step (the values SCORE's master section holds,

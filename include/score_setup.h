@@ -27,7 +27,7 @@
  *
  * Supported: DefStructureSynthesis 1 (cuboid grains), CSRNucleation 1
  * (csr_enforced), no grain-boundary or clustered nucleation, site saturation,
- * no recovery, Zener drag constant or linear in time, no EBSD input; other settings return
+ * the three recovery models, Zener drag constant or linear in time, no EBSD input; other settings return
  * SCORE_SETUP_ERR_UNSUPPORTED.  All functions return 0 or a negative
  * SCORE_SETUP_ERR_* code; score_setup_last_error() describes the last failure
  * of the calling thread.  Objects are immutable once created and may be shared

@@ -47,6 +47,8 @@
 #define O_SMALL_HEAT (1.0)
 #define O_SMALL_DRAGGINGRATE (0.027777777)
 #define O_SMALL_ZENERFORCE (100.0)
+#define O_SMALL_RECOVERYRATE (2.77e10)
+#define O_SMALL_RHO (1.0e11)
 #define O_RHOMAX_WELLANNEALED (1e8)
 #define O_DEFAULT_PMAX (0.0)
 #define O_DEFAULT_SMALL_NUMBER (1e-6)
@@ -109,7 +111,7 @@ typedef struct oracle {
     double cellsize, inv_dx;
     uint32_t *grid;
     uint32_t n_defg, n_rxg;
-    double *defq, *rxq, *rho;
+    double *defq, *rxq, *rho, *rho0;
     uint32_t *site;
     uint32_t *defcnt, *rxcnt;
     score_gm_params p;
@@ -318,6 +320,57 @@ static double o_dt_zener(const oracle *o) {
     return O_INFINITE;
 }
 
+/* update_deformedsubstructure (SCORE_Kernel.cpp:5408-5437) with the recovery
+ * models of SCORE_RecoveryModels.cpp: Nes vacancy-core diffusion (:50-80) grows
+ * r = rho^-1/2 of every grain over the step just done; solute drag (:83-86)
+ * changes nothing; Michalak-Paxton (:89-125) scales rho0 by a function of t */
+static void o_update_deformedsubstructure(oracle *o) {
+    if (o->p.recovery_mode == 1) {
+        double B = O_PI * o->p.rec_c3 * o->p.rec_c5 * o->p.rec_vacancy_d0 *
+                   exp(-1.0 * o->p.rec_vacancy_hact / (O_KBOLTZMAN * o->T));
+        B /= o->p.rec_kappa2;
+        double exponent = (o->b * o->b) / (O_KBOLTZMAN * o->T);
+        double expterm = exp(o->p.rec_alpha3 * o->p.rec_kappa2 * o->G * o->b * exponent);
+        double currdt = o->dt;
+        for (uint32_t dg = 0; dg < o->n_defg; dg++) {
+            double r_t = 1.0 / pow(o->rho[dg], 0.5);
+            double Bx = (B / r_t) * expterm;
+            double r_tdt = (Bx * currdt) + r_t;
+            o->rho[dg] = 1.0 / (r_tdt * r_tdt);
+        }
+    } else if (o->p.recovery_mode == 3) {
+        double f1 = 0.25, f2 = 0.75, tau2 = 0.3605, tau5 = 5.2824;
+        double currtime_minutes = o->t / 60.0;
+        double factor = f1 * pow((1 + (currtime_minutes / tau2)), -1.0) + f2 * pow((1 + (currtime_minutes / tau5)), -0.5);
+        for (uint32_t dg = 0; dg < o->n_defg; dg++) o->rho[dg] = o->rho0[dg] * (factor * factor);
+    }
+}
+
+/* get_dtmax_instantslope_rho (SCORE_Kernel.cpp:5594-5636) */
+static double o_dt_rho(const oracle *o) {
+    double drdtmax, dtmax;
+    if (o->p.recovery_mode == 1) {
+        double Bx = O_PI * o->p.rec_c3 * o->p.rec_c5 * o->p.rec_vacancy_d0 *
+                    exp(-1.0 * o->p.rec_vacancy_hact / (O_KBOLTZMAN * o->T));
+        Bx = Bx / (o->p.rec_kappa2 * (1.0 / pow(o->rhomax, 0.5)));
+        double exponent = (o->b * o->b) / (O_KBOLTZMAN * o->T);
+        drdtmax = Bx * exp(o->p.rec_alpha3 * o->p.rec_kappa2 * o->G * o->b * exponent);
+        if (drdtmax < (1.0 / pow(O_SMALL_RECOVERYRATE, 0.5))) drdtmax = (1.0 / pow(O_SMALL_RECOVERYRATE, 0.5));
+        dtmax = (1.0 / pow(O_SMALL_RHO, 0.5)) / drdtmax;
+        return dtmax;
+    }
+    if (o->p.recovery_mode == 2) {
+        double Bsa = o->p.rec_c3 * o->b * o->p.rec_solute_d0 * exp(-1.0 * o->p.rec_solute_hact / (O_KBOLTZMAN * o->T));
+        double exponent = (o->b * o->b) / (O_KBOLTZMAN * o->T);
+        double ls = o->p.rec_solute_ls * pow(o->p.rec_solute_conc, (-2.0 / 3.0));
+        drdtmax = Bsa * exp(o->p.rec_alpha3 * ls * o->G * o->b * exponent / (1.0 / (o->rhomax * o->rhomax)));
+        if (drdtmax < (1.0 / pow(O_SMALL_RECOVERYRATE, 0.5))) drdtmax = (1.0 / pow(O_SMALL_RECOVERYRATE, 0.5));
+        dtmax = (1.0 / pow(O_SMALL_RHO, 0.5)) / drdtmax;
+        return dtmax;
+    }
+    return O_INFINITE;
+}
+
 static double o_dt_cells(const oracle *o) {
     double kT = O_KBOLTZMAN * o->T;
     if (o->p.mobility_model == SCORE_GM_MOB_ROLLETTHOLM) {
@@ -333,12 +386,14 @@ static double o_dt_cells(const oracle *o) {
     return ((o->p.maxfill * o->cellsize) / v_max);
 }
 
-/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery and nucleation
- * rules return INFINITE for the modes implemented here. */
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the nucleation rule returns
+ * INFINITE for site saturation. */
 static double o_dt_minimum(const oracle *o) {
     double mn = O_INFINITE;
     double d = o_dt_temp(o);
     if (d < mn) mn = d;
+    double dr = o_dt_rho(o);
+    if (dr < mn) mn = dr;
     double dz = o_dt_zener(o);
     if (dz < mn) mn = dz;
     double dm = o_dt_cells(o);
@@ -563,7 +618,7 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
                 if (!(p->disp_time[i] > p->disp_time[i - 1])) return SCORE_GM_ERR_ARG;
         }
     }
-    if (p->recovery_mode != 0) return SCORE_GM_ERR_UNSUPPORTED;
+    if (p->recovery_mode < 0 || p->recovery_mode > 3) return SCORE_GM_ERR_ARG;
     if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
     if (p->n_defrag && !p->defrag_at) return SCORE_GM_ERR_ARG;
     uint64_t N = (uint64_t)d->nx * d->ny * d->nz;
@@ -578,7 +633,7 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
 
 void oracle_destroy(oracle *o) {
     if (!o) return;
-    free(o->grid); free(o->defq); free(o->rxq); free(o->rho); free(o->site);
+    free(o->grid); free(o->defq); free(o->rxq); free(o->rho); free(o->rho0); free(o->site);
     free(o->defcnt); free(o->rxcnt); free(o->sched_t); free(o->sched_T); free(o->defrag_at);
     free(o->disp_t); free(o->disp_fr);
     o_list_free(&o->in); o_list_free(&o->bd);
@@ -603,6 +658,7 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o->defq = (double *)malloc(sizeof(double) * 4 * (size_t)o->n_defg);
     o->rxq = (double *)malloc(sizeof(double) * 4 * (size_t)(o->n_rxg + 1));
     o->rho = (double *)malloc(sizeof(double) * o->n_defg);
+    o->rho0 = (double *)malloc(sizeof(double) * o->n_defg);
     o->site = (uint32_t *)malloc(sizeof(uint32_t) * (o->n_rxg + 1));
     o->defcnt = (uint32_t *)calloc(o->n_defg, sizeof(uint32_t));
     o->rxcnt = (uint32_t *)calloc(o->n_rxg + 1, sizeof(uint32_t));
@@ -611,7 +667,7 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o->defrag_at = (double *)malloc(sizeof(double) * (p->n_defrag + 1));
     o->disp_t = (double *)malloc(sizeof(double) * (p->n_disp + 1));
     o->disp_fr = (double *)malloc(sizeof(double) * (p->n_disp + 1));
-    if (!o->grid || !o->defq || !o->rxq || !o->rho || !o->site || !o->defcnt || !o->rxcnt || !o->sched_t ||
+    if (!o->grid || !o->defq || !o->rxq || !o->rho || !o->rho0 || !o->site || !o->defcnt || !o->rxcnt || !o->sched_t ||
         !o->sched_T || !o->defrag_at || !o->disp_t || !o->disp_fr) {
         oracle_destroy(o);
         return SCORE_GM_ERR_OOM;
@@ -620,6 +676,7 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     for (uint32_t i = 0; i < o->n_defg; i++) {
         memcpy(&o->defq[4 * (size_t)i], &d->ori_quat[4 * (size_t)d->defg_ori[i]], 4 * sizeof(double));
         o->rho[i] = d->defg_rho[i];
+        o->rho0[i] = d->defg_rho[i];
     }
     for (uint32_t i = 0; i < o->n_rxg; i++) {
         memcpy(&o->rxq[4 * (size_t)i], &d->ori_quat[4 * (size_t)d->rxg_ori[i]], 4 * sizeof(double));
@@ -757,6 +814,7 @@ int oracle_step(oracle *o) {
     o_update_atomistic(o);
     o_update_microchemistry(o);
     o_update_mobilities(o);
+    o_update_deformedsubstructure(o);
     o->dt = o_dt_minimum(o);
     o->step++;
     if ((rc = o_push_traj(o, &r))) { o->err = rc; return rc; }

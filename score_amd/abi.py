@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 OK = 0
 ERR_ARG = -1
@@ -98,6 +98,10 @@ class Params(C.Structure):
         ("zener_alpha", C.c_double), ("zener_gamma", C.c_double),
         ("n_disp", C.c_uint32), ("reserved3", C.c_uint32),
         ("disp_time", _f64p), ("disp_fr", _f64p),
+        ("rec_vacancy_d0", C.c_double), ("rec_vacancy_hact", C.c_double),
+        ("rec_solute_d0", C.c_double), ("rec_solute_hact", C.c_double),
+        ("rec_solute_ls", C.c_double), ("rec_solute_conc", C.c_double),
+        ("rec_alpha3", C.c_double), ("rec_kappa2", C.c_double), ("rec_c3", C.c_double), ("rec_c5", C.c_double),
     ]
 
 

@@ -44,6 +44,8 @@ const double kInfinite = 1e36;
 const double kSmallHeatRate = 0.000277777777777777;
 const double kSmallHeat = 1.0;
 const double kSmallDraggingRate = 0.027777777; /* SMALL_DRAGGINGRATE, SCORE_Defs_Physics.h:84 */
+const double kSmallRecoveryRate = 2.77e10;     /* SMALL_RECOVERYRATE, :83 */
+const double kSmallRho = 1.0e11;               /* SMALL_RHO, :86 */
 const double kSmallZenerForce = 100.0;         /* SMALL_ZENERFORCE, :87 */
 const double kRhoWellAnnealed = 1e8;
 const double kDefaultPmax = 0.0;
@@ -181,6 +183,8 @@ struct score_gm_handle {
     bool force_reanchor = false; /* records moved: every cell is predicted again */
     /* deformed grains by descending rho and whether they still own cells */
     std::vector<double> rho_host;
+    std::vector<double> rho0_host;     /* mydefgpool[].rho0 (Michalak-Paxton recovery) */
+    bool rho_changed = false;          /* recovery changed rho: upload and re-anchor */
     std::vector<uint32_t> rho_order;
     std::vector<uint8_t> alive;
     size_t rho_top = 0;
@@ -255,7 +259,7 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
                     return fail(SCORE_GM_ERR_ARG, "drag table times must increase strictly");
         }
     }
-    if (prm->recovery_mode != 0) return fail(SCORE_GM_ERR_UNSUPPORTED, "recovery is not implemented");
+    if (prm->recovery_mode < 0 || prm->recovery_mode > 3) return fail(SCORE_GM_ERR_ARG, "unknown recovery model");
     if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
         return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");
@@ -369,6 +373,62 @@ double dt_zener(const score_gm_handle *h) {
     return kInfinite;
 }
 
+/* update_deformedsubstructure (SCORE_Kernel.cpp:5408-5437): the recovery models
+ * of SCORE_RecoveryModels.cpp, after the mobilities and before the new time step;
+ * mode 1 uses the step just done (nes_networkgrowthmodel_vacancycorediff, :50-80),
+ * mode 2 changes nothing (nes_networkgrowthmodel_solutedrag, :83-86), mode 3
+ * scales rho0 (nes_michalakpaxton, :89-125) */
+void update_deformedsubstructure(score_gm_handle *h) {
+    const int mode = h->p.recovery_mode;
+    if (mode == 1) {
+        double B = score_orient::kPi * h->p.rec_c3 * h->p.rec_c5 * h->p.rec_vacancy_d0 *
+                   exp(-1.0 * h->p.rec_vacancy_hact / (kBoltzmann * h->T));
+        B /= h->p.rec_kappa2;
+        const double exponent = (h->b * h->b) / (kBoltzmann * h->T);
+        const double expterm = exp(h->p.rec_alpha3 * h->p.rec_kappa2 * h->G * h->b * exponent);
+        const double currdt = h->dt;
+        for (double &rho : h->rho_host) {
+            const double r_t = 1.0 / pow(rho, 0.5);
+            const double Bx = (B / r_t) * expterm;
+            const double r_tdt = (Bx * currdt) + r_t;
+            rho = 1.0 / (r_tdt * r_tdt);
+        }
+        h->rho_changed = true;
+    } else if (mode == 3) {
+        const double f1 = 0.25, f2 = 0.75, tau2 = 0.3605, tau5 = 5.2824;
+        const double currtime_minutes = h->t / 60.0;
+        const double factor = f1 * pow((1 + (currtime_minutes / tau2)), -1.0) + f2 * pow((1 + (currtime_minutes / tau5)), -0.5);
+        for (size_t g = 0; g < h->rho_host.size(); g++) h->rho_host[g] = h->rho0_host[g] * (factor * factor);
+        h->rho_changed = true;
+    }
+}
+
+/* get_dtmax_instantslope_rho (SCORE_Kernel.cpp:5594-5636) */
+double dt_rho(const score_gm_handle *h) {
+    const int mode = h->p.recovery_mode;
+    double drdtmax, dtmax;
+    if (mode == 1) {
+        double Bx = score_orient::kPi * h->p.rec_c3 * h->p.rec_c5 * h->p.rec_vacancy_d0 *
+                    exp(-1.0 * h->p.rec_vacancy_hact / (kBoltzmann * h->T));
+        Bx = Bx / (h->p.rec_kappa2 * (1.0 / pow(h->rhomax, 0.5)));
+        const double exponent = (h->b * h->b) / (kBoltzmann * h->T);
+        drdtmax = Bx * exp(h->p.rec_alpha3 * h->p.rec_kappa2 * h->G * h->b * exponent);
+        if (drdtmax < (1.0 / pow(kSmallRecoveryRate, 0.5))) drdtmax = (1.0 / pow(kSmallRecoveryRate, 0.5));
+        dtmax = (1.0 / pow(kSmallRho, 0.5)) / drdtmax;
+        return dtmax;
+    }
+    if (mode == 2) {
+        const double Bsa = h->p.rec_c3 * h->b * h->p.rec_solute_d0 * exp(-1.0 * h->p.rec_solute_hact / (kBoltzmann * h->T));
+        const double exponent = (h->b * h->b) / (kBoltzmann * h->T);
+        const double ls = h->p.rec_solute_ls * pow(h->p.rec_solute_conc, (-2.0 / 3.0));
+        drdtmax = Bsa * exp(h->p.rec_alpha3 * ls * h->G * h->b * exponent / (1.0 / (h->rhomax * h->rhomax)));
+        if (drdtmax < (1.0 / pow(kSmallRecoveryRate, 0.5))) drdtmax = (1.0 / pow(kSmallRecoveryRate, 0.5));
+        dtmax = (1.0 / pow(kSmallRho, 0.5)) / drdtmax;
+        return dtmax;
+    }
+    return kInfinite;
+}
+
 /* get_dtmax_instantslope_cells (SCORE_Kernel.cpp:5519-5558) */
 double dt_cells(const score_gm_handle *h) {
     const double kT = kBoltzmann * h->T;
@@ -385,12 +445,14 @@ double dt_cells(const score_gm_handle *h) {
     return ((h->p.maxfill * h->cellsize) / v_max);
 }
 
-/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the recovery and nucleation
- * rules are INFINITE for the modes implemented */
+/* get_dtmax_minimum (SCORE_Kernel.cpp:5687-5723); the nucleation rule is
+ * INFINITE for site saturation */
 double dt_minimum(const score_gm_handle *h) {
     double mn = kInfinite;
     const double a = dt_temp(h);
     if (a < mn) mn = a;
+    const double r = dt_rho(h);
+    if (r < mn) mn = r;
     const double z = dt_zener(h);
     if (z < mn) mn = z;
     const double c = dt_cells(h);
@@ -595,6 +657,7 @@ void advance_scalars(score_gm_handle *h) {
     update_atomistic(h);
     update_microchemistry(h);
     update_mobilities(h);
+    update_deformedsubstructure(h);
     h->dt = dt_minimum(h);
 }
 
@@ -606,6 +669,8 @@ void advance_scalars(score_gm_handle *h) {
  * qualifies: its time step does not depend on the weights of new pairs. */
 uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &sp) {
     if (h->prof || wmax <= 1 || h->p.mobility_model != SCORE_GM_MOB_ROLLETTHOLM) return 1;
+    /* recovery that changes rho re-anchors every step */
+    if (h->p.recovery_mode == 1 || h->p.recovery_mode == 3) return 1;
     /* a new cell starts at most at 0.0682 times the largest increment and takes
      * at least imin additions to reach 1.0; the increment is at most maxfill
      * because the time step is bounded by maxfill*dx/vmax and P <= 1 */
@@ -803,6 +868,12 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
                 h->rho_top = 0;
             }
         }
+        if (h->p.recovery_mode != 0) {
+            /* update_mydgoptimize_rho (SCORE_RecoveryModels.cpp:128-135): re-sorted by the current rho */
+            std::stable_sort(h->rho_order.begin(), h->rho_order.end(),
+                             [&](uint32_t a, uint32_t b) { return h->rho_host[a] > h->rho_host[b]; });
+            h->rho_top = 0;
+        }
         update_rhomax(h);
         r.sv = sv;
         r.nfull_inside = row.nfullI;
@@ -833,6 +904,15 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         ad.nB = h->nB;
         if ((e = gm_launch_predict_deferred(d, ad, st.ndeferred, h->stream)) != hipSuccess)
             return h->err = hip_fail(e, "deferred predictions");
+    }
+    if (h->rho_changed) {
+        /* recovered dislocation densities for the next step (after the
+         * window's own predictions); every cell's increment changes, so it re-anchors */
+        if ((e = hipMemcpyAsync(d.rho, h->rho_host.data(), sizeof(double) * d.ndefg, hipMemcpyHostToDevice,
+                                h->stream)) != hipSuccess)
+            return h->err = hip_fail(e, "recovered dislocation densities");
+        h->rho_changed = false;
+        h->force_reanchor = true;
     }
     *nsteps = nexec;
     h->windows++;
@@ -1064,6 +1144,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
 
     /* solve_RXGROWTH prologue (SCORE_Kernel.cpp:6501-6530): scalars and the first dt */
     h->rho_host.assign(dom->defg_rho, dom->defg_rho + d.ndefg);
+    h->rho0_host = h->rho_host;
     h->rho_order.resize(d.ndefg);
     h->alive.resize(d.ndefg);
     for (uint32_t g = 0; g < d.ndefg; g++) {

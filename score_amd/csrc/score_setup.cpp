@@ -282,6 +282,10 @@ struct score_setup_ensemble {
     int32_t model = 0;
     double lagb_m0 = 0, lagb_h = 0, hagb_m0 = 0, hagb_h = 0, gs_m0 = 0, gs_h = 0;
     double rh_m0 = 0, rh_h = 0, rh_cut = 0, rh_trans = 0, rh_exp = 0;
+    /* RecoveryParameter (enthalpies in J) */
+    int32_t recovery_mode = 0;
+    double rec_vd0 = 0, rec_vh = 0, rec_sd0 = 0, rec_sh = 0, rec_ls = 0, rec_conc = 0;
+    double rec_alpha3 = 0, rec_kappa2 = 0, rec_c3 = 0, rec_c5 = 0;
     /* DragParameter, EvoMicrochemistry */
     int32_t zener_mode = 0;
     double zener_alpha = 0, zener_gamma = 0;
@@ -469,17 +473,28 @@ int read_physics(const score_uds *u, score_setup_ensemble *e) {
         (e->rh_m0 <= 0.0 || e->rh_h <= 0.0 || e->rh_cut <= 0.0 || e->rh_trans <= 0.0 || e->rh_exp <= 0.0))
         return fail(SCORE_SETUP_ERR_RANGE, "Unrealistic mobility model parameterization");
     /* recovery and Zener drag (init_parameter_recovery, _drag, SCORE_Kernel.cpp:595-665):
-     * no recovery model is implemented; drag is constant (1) or linear in time (2) */
+     * RecoveryConsider 1-3 selects a recovery model, any other value none; drag is
+     * constant (1) or linear in time (2) */
     const char *R = "RecoveryParameter";
     if ((rc = uds_int(u, R, "RecoveryConsider", &v))) return rc;
-    const long recovery = v;
-    const char *rkeys[] = {"RecoveryVacancyDiffGeometry", "RecoveryVacancyDiffPreexp", "RecoveryVacancyDiffHact",
-                           "RecoverySoluteDiffPreexp", "RecoverySoluteDiffHact", "RecoverySoluteLsPropFactor",
-                           "RecoverySoluteConcentration", "RecoveryParameterAlpha3", "RecoveryParameterKappa2",
-                           "RecoveryParameterC3", "RecoveryParameterC4", "RecoveryParameterC5"};
-    for (const char *k : rkeys)
-        if ((rc = uds_real(u, R, k, &x))) return rc;
-    if (recovery >= 1 && recovery <= 3) return fail(SCORE_SETUP_ERR_UNSUPPORTED, "RecoveryConsider (recovery models) is not supported");
+    e->recovery_mode = (v >= 1 && v <= 3) ? (int32_t)v : 0;
+    double vgeo, c4;
+    if ((rc = uds_real(u, R, "RecoveryVacancyDiffGeometry", &vgeo)) ||
+        (rc = uds_real(u, R, "RecoveryVacancyDiffPreexp", &e->rec_vd0)) ||
+        (rc = uds_real(u, R, "RecoveryVacancyDiffHact", &x)))
+        return rc;
+    e->rec_vh = x * kEcharge;
+    if ((rc = uds_real(u, R, "RecoverySoluteDiffPreexp", &e->rec_sd0)) ||
+        (rc = uds_real(u, R, "RecoverySoluteDiffHact", &x)))
+        return rc;
+    e->rec_sh = x * kEcharge;
+    if ((rc = uds_real(u, R, "RecoverySoluteLsPropFactor", &e->rec_ls)) ||
+        (rc = uds_real(u, R, "RecoverySoluteConcentration", &e->rec_conc)) ||
+        (rc = uds_real(u, R, "RecoveryParameterAlpha3", &e->rec_alpha3)) ||
+        (rc = uds_real(u, R, "RecoveryParameterKappa2", &e->rec_kappa2)) ||
+        (rc = uds_real(u, R, "RecoveryParameterC3", &e->rec_c3)) || (rc = uds_real(u, R, "RecoveryParameterC4", &c4)) ||
+        (rc = uds_real(u, R, "RecoveryParameterC5", &e->rec_c5)))
+        return rc;
     const char *D = "DragParameter";
     if ((rc = uds_int(u, D, "ZenerConsider", &v)) || (rc = uds_real(u, D, "ZenerAlpha", &e->zener_alpha)) ||
         (rc = uds_real(u, D, "ZenerGamma", &e->zener_gamma)))
@@ -880,7 +895,18 @@ int score_setup_domain_create(const score_setup_ensemble *e, uint32_t jobid, int
     score_gm_params &p = s->prm;
     memset(&p, 0, sizeof(p));
     p.mobility_model = e->model;
-    p.recovery_mode = 0;
+    /* caHdl::init_parameter copies the recovery model (SCORE_Kernel.cpp:5835-5847) */
+    p.recovery_mode = e->recovery_mode;
+    p.rec_vacancy_d0 = e->rec_vd0;
+    p.rec_vacancy_hact = e->rec_vh;
+    p.rec_solute_d0 = e->rec_sd0;
+    p.rec_solute_hact = e->rec_sh;
+    p.rec_solute_ls = e->rec_ls;
+    p.rec_solute_conc = e->rec_conc;
+    p.rec_alpha3 = e->rec_alpha3;
+    p.rec_kappa2 = e->rec_kappa2;
+    p.rec_c3 = e->rec_c3;
+    p.rec_c5 = e->rec_c5;
     p.rh_hagb_m0 = e->rh_m0;
     p.rh_hagb_hact = e->rh_h;
     p.rh_cut = e->rh_cut;

@@ -228,6 +228,7 @@ class Ensemble:
         deck's eV values (ParamsSpec multiplies by echarge as SCORE does)."""
         u = self.uds
         B = "BoundaryMobilityModel"
+        R = "RecoveryParameter"
         ts = np.ctypeslib.as_array(p.sched_time, shape=(p.n_sched,)).copy()
         tT = np.ctypeslib.as_array(p.sched_temp, shape=(p.n_sched,)).copy()
         dfr = tuple(np.ctypeslib.as_array(p.defrag_at, shape=(p.n_defrag,)).tolist()) if p.n_defrag else ()
@@ -243,7 +244,12 @@ class Ensemble:
                           maxfill=p.maxfill, xmax=p.xmax, tmax=p.tmax, nmax=p.nmax,
                           rel_cell_caching=p.rel_cell_caching, rel_cell_recaching=p.rel_cell_recaching,
                           defrag_at=dfr, zener_mode=p.zener_mode, zener_alpha=p.zener_alpha,
-                          zener_gamma=p.zener_gamma, disp_time=dt_, disp_fr=dfr_)
+                          zener_gamma=p.zener_gamma, disp_time=dt_, disp_fr=dfr_,
+                          recovery_mode=p.recovery_mode, rec_vacancy_d0=p.rec_vacancy_d0,
+                          rec_vacancy_hact_ev=u.real(R, "RecoveryVacancyDiffHact"),
+                          rec_solute_d0=p.rec_solute_d0, rec_solute_hact_ev=u.real(R, "RecoverySoluteDiffHact"),
+                          rec_solute_ls=p.rec_solute_ls, rec_solute_conc=p.rec_solute_conc,
+                          rec_alpha3=p.rec_alpha3, rec_kappa2=p.rec_kappa2, rec_c3=p.rec_c3, rec_c5=p.rec_c5)
 
     def close(self) -> None:
         if self._h:

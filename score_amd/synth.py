@@ -183,6 +183,17 @@ class ParamsSpec:
     zener_gamma: float = 0.324          # J/m^2
     disp_time: tuple = ()               # s
     disp_fr: tuple = ()                 # f/r, 1/m
+    recovery_mode: int = 0              # 0 none, 1 Nes vacancy-core diffusion, 2 Nes solute drag, 3 Michalak-Paxton
+    rec_vacancy_d0: float = 1.0
+    rec_vacancy_hact_ev: float = 2.9
+    rec_solute_d0: float = 1.0
+    rec_solute_hact_ev: float = 1.6
+    rec_solute_ls: float = 1.0
+    rec_solute_conc: float = 1.0
+    rec_alpha3: float = 1.0
+    rec_kappa2: float = 0.1
+    rec_c3: float = 1.0
+    rec_c5: float = 0.65
 
     def to_c(self):
         keep = [np.ascontiguousarray(self.sched_time, np.float64), np.ascontiguousarray(self.sched_temp, np.float64),
@@ -192,7 +203,7 @@ class ParamsSpec:
         p = abi.Params()
         p.mobility_model = self.mobility_model
         p.zener_mode = self.zener_mode
-        p.recovery_mode = 0
+        p.recovery_mode = self.recovery_mode
         p.rh_hagb_m0 = self.rh_hagb_m0
         p.rh_hagb_hact = self.rh_hagb_hact_ev * ECHARGE
         p.rh_cut, p.rh_trans, p.rh_exponent = self.rh_cut, self.rh_trans, self.rh_exponent
@@ -211,6 +222,10 @@ class ParamsSpec:
         p.n_disp = len(self.disp_time)
         p.disp_time = _ptr(keep[3], C.c_double)
         p.disp_fr = _ptr(keep[4], C.c_double)
+        p.rec_vacancy_d0, p.rec_vacancy_hact = self.rec_vacancy_d0, self.rec_vacancy_hact_ev * ECHARGE
+        p.rec_solute_d0, p.rec_solute_hact = self.rec_solute_d0, self.rec_solute_hact_ev * ECHARGE
+        p.rec_solute_ls, p.rec_solute_conc = self.rec_solute_ls, self.rec_solute_conc
+        p.rec_alpha3, p.rec_kappa2, p.rec_c3, p.rec_c5 = self.rec_alpha3, self.rec_kappa2, self.rec_c3, self.rec_c5
         return p, keep
 
 

@@ -108,7 +108,7 @@ def test_invalid_arguments_rejected_before_device_use():
         GrowthMachine(dom)
     assert e.value.code == abi.ERR_ARG
     # thresholds announced but not given; drag without its table or with a table
-    # that does not start at t = 0; a recovery model the library does not have
+    # that does not start at t = 0; a recovery model SCORE does not have
     lib = abi.load()
     dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
     cd, keep_d = dom.to_c()
@@ -127,8 +127,8 @@ def test_invalid_arguments_rejected_before_device_use():
     cp.zener_mode = 3
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     cp, keep_p = synth.ParamsSpec().to_c()
-    cp.recovery_mode = 1
-    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_UNSUPPORTED
+    cp.recovery_mode = 4
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     assert not h.value
 
 

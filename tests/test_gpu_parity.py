@@ -199,6 +199,39 @@ def test_zener_drag_in_launch_windows(require_gpu):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("kw", [
+    dict(recovery_mode=1, rec_vacancy_d0=2e-6, rh_hagb_m0=20.0),     # Nes vacancy core diffusion, both dt rules
+    dict(recovery_mode=3, rh_hagb_m0=40.0),                          # Michalak-Paxton
+    dict(recovery_mode=2, rec_alpha3=0.0, rh_hagb_m0=8.0),           # solute drag: its dt rule only
+    dict(recovery_mode=2, nmax=60),                                  # solute drag overflowing: dt = 0 until NMAX
+], ids=["vacancy", "michalak_paxton", "solute", "solute_dt0"])
+def test_recovery_stepwise(require_gpu, kw):
+    """Recovery of the deformed grains' dislocation density after every step (every
+    step re-anchors while rho changes) and the recovery time-step rules."""
+    dom = synth.make_voronoi_domain(30, 28, 26, n_grains=7, n_nuclei=18, seed=21)
+    prm = ParamsSpec(**kw)
+    o, g = run_both(dom, prm, stepwise=True)
+    assert_final_equal(o, g)
+    assert g.result()["completed"]
+
+
+def test_recovery_in_launch_windows(require_gpu):
+    """solute drag keeps rho and the time step constant, so launch windows stay long"""
+    dom = synth.make_voronoi_domain(48, 40, 32, n_grains=9, n_nuclei=25, seed=8)
+    prm = ParamsSpec(recovery_mode=2, rec_alpha3=0.0, rh_hagb_m0=8.0)
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        do, dg = o.step(7), g.step(7)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+    assert g.result()["windows"] < len(g.trajectory()) // 2
+
+
 def test_small_stepwise_sg(require_gpu):
     dom = synth.make_voronoi_domain(30, 30, 30, n_grains=8, n_nuclei=30, seed=5)
     prm = ParamsSpec(mobility_model=abi.MOB_SEBALDGOTTSTEIN)

@@ -174,6 +174,118 @@ def test_zener_drag_time_dependent_dt_rule():
     o.close()
 
 
+def _atomistic(T=1000.0):
+    G = 6.92e10 * (-1.38e-9 * T ** 3 + 1.21e-6 * T ** 2 - 4.58e-4 * T + 1.0)
+    return G, 2.48e-10, 0.5 * G * 2.48e-10 ** 2
+
+
+def _mp_factor(t):
+    """Michalak-Paxton recovery of SCORE_RecoveryModels.cpp:89-125: two relaxations
+    with time constants 0.3605 min and 5.2824 min, weights 1/4 and 3/4"""
+    return 0.25 / (1.0 + t / (0.3605 * 60.0)) + 0.75 / math.sqrt(1.0 + t / (5.2824 * 60.0))
+
+
+def test_recovery_michalak_paxton_closed_form():
+    """rho = rho0 * factor(t)^2 after each step (update_deformedsubstructure,
+    SCORE_Kernel.cpp:5408-5437), applied after the master section: rhomax of step j is
+    rho0 * factor(t_j)^2, the time step lags by one step (the cells rule with that
+    rhomax, no recovery rule for this model), and a face cell of step j advances by
+    maxfill * (rho_j - 1e11) / rhomax_(j-1)"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    o = Oracle(dom, ParamsSpec(recovery_mode=3, rh_hagb_m0=40.0))
+    o.step(2)
+    before = o.front(abi.LIST_INSIDE)["rxfrac"].copy()
+    o.step(1)
+    after = o.front(abi.LIST_INSIDE)["rxfrac"]
+    o.step(5)
+    tr = o.trajectory()
+    assert tr["rhomax"][0] == 1e15
+    for j in range(1, len(tr)):
+        assert tr["rhomax"][j] == pytest.approx(1e15 * _mp_factor(tr["t"][j]) ** 2, rel=1e-13)
+    G, b, gb = _atomistic()
+    m = 40.0 * math.exp(-2.8 * EC / (KB * 1000.0))
+    for j in range(1, len(tr)):
+        assert tr["dt"][j] == pytest.approx(0.1 * 0.5e-6 / (m * gb * tr["rhomax"][j - 1]), rel=1e-12)
+    # the recovery is visible: rho fell by a fifth within the first steps
+    assert tr["rhomax"][3] < 0.99 * tr["rhomax"][1]
+    inc = after[1:27] - before[1:27]
+    shape = {1: 1.0, 2: 0.7957, 3: 0.6494}
+    for k, ((dx, dy, dz), code) in enumerate(ORDER):
+        cls = abs(dx) + abs(dy) + abs(dz)
+        assert inc[k] == pytest.approx(0.1 * shape[cls] * (tr["rhomax"][2] - 1e11) / tr["rhomax"][1], rel=1e-9)
+    o.close()
+
+
+def _nes_drdt(D0, rho, T=1000.0, c3=1.0, c5=0.65, kappa2=0.1, alpha3=1.0, H=2.9 * EC):
+    """dr/dt of the Nes vacancy-core-diffusion network growth, r = rho^-1/2:
+    pi C3 C5 D0 exp(-H/kT) / (kappa2 r) * exp(alpha3 kappa2 G b^3 / kT)
+    (SCORE_RecoveryModels.cpp:50-80)"""
+    G, b, _ = _atomistic(T)
+    r = rho ** -0.5
+    return math.pi * c3 * c5 * D0 * math.exp(-H / (KB * T)) / (kappa2 * r) * math.exp(alpha3 * kappa2 * G * b ** 3 / (KB * T))
+
+
+def test_recovery_vacancy_core_diffusion_euler_step_and_dt_rule():
+    """r = rho^-1/2 of every grain takes one forward-Euler step dr = dr/dt * dt with the
+    time step just done; the next time step is the smaller of the cells rule and
+    (1/sqrt(SMALL_RHO)) / max(dr/dt at rhomax, 1/sqrt(SMALL_RECOVERYRATE))
+    (get_dtmax_instantslope_rho, SCORE_Kernel.cpp:5594-5636)"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    D0 = 2e-5
+    o = Oracle(dom, ParamsSpec(recovery_mode=1, rec_vacancy_d0=D0, rh_hagb_m0=20.0))
+    o.step(40)
+    tr = o.trajectory()
+    G, b, gb = _atomistic()
+    m = 20.0 * math.exp(-2.8 * EC / (KB * 1000.0))
+    floor = math.sqrt(2.77e10 / 1e11)
+    used = set()
+    for j in range(len(tr) - 1):
+        r = tr["rhomax"][j] ** -0.5 + _nes_drdt(D0, tr["rhomax"][j]) * tr["dt"][j]
+        assert tr["rhomax"][j + 1] == pytest.approx(r ** -2, rel=1e-12)
+        cells = 0.1 * 0.5e-6 / (m * gb * tr["rhomax"][j])
+        rule = 1e11 ** -0.5 / max(_nes_drdt(D0, tr["rhomax"][j]), 2.77e10 ** -0.5)
+        assert rule == pytest.approx(floor, rel=1e-12)
+        assert tr["dt"][j + 1] == pytest.approx(min(cells, rule), rel=1e-12)
+        used.add(cells < rule)
+    assert used == {True, False}           # both rules set the step in this run
+    assert tr["rhomax"][-1] < 0.8e15
+    o.close()
+    # a fast recovery: the rule's rate is above its floor and sets the second step
+    o = Oracle(dom, ParamsSpec(recovery_mode=1, rec_vacancy_d0=1.0))
+    o.step(3)
+    tr = o.trajectory()
+    assert _nes_drdt(1.0, tr["rhomax"][0]) > 2.77e10 ** -0.5
+    assert tr["dt"][1] == pytest.approx(1e11 ** -0.5 / _nes_drdt(1.0, tr["rhomax"][0]), rel=1e-12)
+    o.close()
+
+
+def test_recovery_solute_drag_dt_rule():
+    """the solute-drag model changes no rho; its rule divides 1/sqrt(SMALL_RHO) by
+    C3 b D0 exp(-H/kT) exp(alpha3 ls G b^3 / kT * rhomax^2), ls = LsPropFactor c^(-2/3),
+    floored at 1/sqrt(SMALL_RECOVERYRATE).  With alpha3 > 0 the exponent overflows
+    to infinity for any realistic rhomax, the time step is 0 from the start and the
+    domain never grows (SCORE_Kernel.cpp:5618-5630)"""
+    n = 20
+    dom = one_grain(n, [lin(10, 10, 10, n, n)])
+    G, b, gb = _atomistic()
+    for D0, want in ((1.0, math.sqrt(2.77e10 / 1e11)),
+                     (1e14, 1e11 ** -0.5 / (1.0 * b * 1e14 * math.exp(-1.6 * EC / (KB * 1000.0))))):
+        o = Oracle(dom, ParamsSpec(recovery_mode=2, rec_alpha3=0.0, rec_solute_d0=D0))
+        o.step(4)
+        tr = o.trajectory()
+        assert np.all(tr["rhomax"] == 1e15)
+        # the first step is the initial 1e-6 s of the temperature rule at t = 0
+        assert tr["dt"][0] == 1e-6 and np.all(tr["dt"][1:] == pytest.approx(want, rel=1e-12))
+        o.close()
+    o = Oracle(dom, ParamsSpec(recovery_mode=2, nmax=30))
+    r = o.solve_rxgrowth()
+    tr = o.trajectory()
+    assert r["t"] == 0.0 and np.all(tr["dt"] == 0.0) and r["step"] == 31 and r["xcells"] == 1
+    o.close()
+
+
 def test_recycled_slot_reused_first():
     """slot 0 (retired nucleus) is the first recycled slot: the next infection takes it"""
     n = 20

@@ -246,12 +246,33 @@ def test_params_from_deck(tmp_path):
     _, p3, _ = e3.domain(0)
     assert (p3.zener_mode, p3.zener_alpha, p3.zener_gamma) == (2, 1.5, 0.324)
     assert p3.disp_time == (0.0, f32(360000.0)) and p3.disp_fr == (0.0, 0.0)
+    # recovery from the deck: RecoveryConsider 1-3 selects a model, any other value
+    # none; activation enthalpies in eV times echarge, as for the mobilities
+    assert prm.recovery_mode == 0
+    for mode, want in (("1", 1), ("2", 2), ("3", 3), ("7", 0)):
+        e4 = ds.Ensemble.from_uds(write(tmp_path, mk.deck_text({("RecoveryParameter", "RecoveryConsider"): mode,
+                                                                ("RecoveryParameter", "RecoveryVacancyDiffPreexp"): "2.5e-5",
+                                                                ("RecoveryParameter", "RecoverySoluteConcentration"): "0.01"})))
+        _, p4, _ = e4.domain(0)
+        assert p4.recovery_mode == want
+        assert (p4.rec_vacancy_d0, p4.rec_vacancy_hact_ev, p4.rec_solute_d0, p4.rec_solute_hact_ev) == (2.5e-5, 2.9, 1.0, 1.6)
+        assert (p4.rec_solute_ls, p4.rec_solute_conc, p4.rec_alpha3, p4.rec_kappa2, p4.rec_c3, p4.rec_c5) == \
+            (1.0, 0.01, 1.0, 0.1, 1.0, 0.65)
+        h = C.c_void_p()
+        assert lib.score_setup_domain_create(e4._h, 0, ds.DEFAULT_SEED, C.byref(h)) == 0
+        dv, pv = abi.Domain(), abi.Params()
+        assert lib.score_setup_domain_view(h, C.byref(dv), C.byref(pv)) == 0
+        q, keep = p4.to_c()
+        for f in ("recovery_mode", "rec_vacancy_d0", "rec_vacancy_hact", "rec_solute_d0", "rec_solute_hact",
+                  "rec_solute_ls", "rec_solute_conc", "rec_alpha3", "rec_kappa2", "rec_c3", "rec_c5"):
+            assert getattr(pv, f) == getattr(q, f), f
+        lib.score_setup_domain_free(h)
+        e4.close()
 
 
 @pytest.mark.parametrize("override,code", [
     ((("NucleationModeling", "CSRNucleation"), "2"), ds.ERR_UNSUPPORTED),
     ((("NucleationModeling", "GBNucleation"), "1"), ds.ERR_UNSUPPORTED),
-    ((("RecoveryParameter", "RecoveryConsider"), "1"), ds.ERR_UNSUPPORTED),
     ((("EnsembleDefinition", "DefStructureSynthesis"), "4"), ds.ERR_UNSUPPORTED),
     ((("RuntimeControl", "XMAX"), "1.5"), ds.ERR_RANGE),
     ((("RuntimeControl", "NMAX"), "0"), ds.ERR_RANGE),
