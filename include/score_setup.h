@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define SCORE_SETUP_ABI_VERSION 1
+#define SCORE_SETUP_ABI_VERSION 2
 
 #define SCORE_SETUP_OK 0
 #define SCORE_SETUP_ERR_ARG -1          /* invalid argument */
@@ -107,6 +107,7 @@ typedef struct score_setup_ensemble_info {
     uint32_t n_world_ori;         /* orientations after merging */
     uint32_t n_world_defg;        /* DeformedGrainsPool rows */
     uint32_t n_world_rxg;         /* RXGrainsPool rows */
+    uint32_t n_ideal;             /* IdealComponents rows (texture classes 1..n_ideal; 0 is RANDOM) */
 } score_setup_ensemble_info;
 
 /* all settings of the growth path from a UDS file (ensembleHdl::init_parameter_*) */
@@ -116,6 +117,13 @@ int score_setup_ensemble_get_info(const score_setup_ensemble *e, score_setup_ens
 int score_setup_world_quat(const score_setup_ensemble *e, uint32_t i, double q[4]);
 int score_setup_world_defg(const score_setup_ensemble *e, uint32_t g, uint32_t *ori, double *rho);
 void score_setup_ensemble_free(score_setup_ensemble *e);
+/* ideal texture component i (init_parameter_idealtexture, SCORE_Kernel.cpp:668-712):
+ * Bunge angles and scatter in radians, its quaternion */
+int score_setup_ideal(const score_setup_ensemble *e, uint32_t i, double bunge[3], double q[4], double *scatter);
+/* the texture class of an orientation (ens_get_closest_standardlage,
+ * SCORE_OrientationHdl.cpp:51-77): 1 + the closest ideal component within its
+ * scatter (the last of equally close ones), or 0 (RANDOM) */
+int score_setup_closest_ideal(const score_setup_ensemble *e, const double q[4], uint32_t *cls);
 
 /* ---- one domain ----------------------------------------------------------------- */
 

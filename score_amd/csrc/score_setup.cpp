@@ -38,6 +38,7 @@ constexpr double kRhoRecrystallized = (1.0e11);                              /* 
 constexpr double kResolutionSO3Grid = ((0.5) / (180.0) * (kPiDefs));        /* :102 */
 constexpr double kBungeMin = ((0.0) / (180.0) * (kPiDefs));                 /* :106 */
 constexpr double kBungeMax = ((360.0) / (180.0) * (kPiDefs));               /* :107 */
+constexpr double kMaxFcc = (1.099);                                          /* MAX_FCC, :101 */
 constexpr double kDefaultMaxFill = (0.1), kMinFillIn = (0.01), kMaxFillIn = (0.2);      /* :123-125 */
 constexpr double kDefaultRelCell = (0.10), kDefaultTransRelCell = (1.0);                /* :127-128 */
 constexpr double kDefaultTmax = (360000.0);                                  /* :132 */
@@ -54,6 +55,7 @@ constexpr long kIntegerRangeMax = 2147483647;                                /* 
 constexpr long kMaxNcas = 1048576;                                           /* :161 */
 constexpr long kMaxDeformedGrains = 65536;                                   /* :162 */
 constexpr long kMaxRxGrains = 1048576;                                       /* :164 */
+constexpr size_t kMaxInputDataLen = 4096;                                    /* DEFAULT_MAX_INPUTDATALEN, :165 */
 constexpr int kBufsize = 1024;                                               /* BUFSIZE, SCORE_Io.h:62 */
 constexpr int64_t kDefaultSeed = SCORE_SETUP_DEFAULT_SEED;
 
@@ -265,6 +267,11 @@ struct WorldDefg {
     uint32_t ori;
     double rho0, rho;
 };
+struct Ideal {
+    double bunge[3];  /* rad */
+    double q[4];
+    double scatter;   /* rad */
+};
 
 struct score_setup_ensemble {
     /* RuntimeControl */
@@ -290,6 +297,8 @@ struct score_setup_ensemble {
     int32_t zener_mode = 0;
     double zener_alpha = 0, zener_gamma = 0;
     std::vector<double> disp_t, disp_fr;
+    /* IdealComponents */
+    std::vector<Ideal> ideals;
     /* AnnealingSchedule */
     std::vector<double> ttime, ttemp;
     /* HeuristicRXFrontListDefragmentation: X, ascending */
@@ -520,6 +529,47 @@ int read_physics(const score_uds *u, score_setup_ensemble *e) {
     return 0;
 }
 
+int read_ideals(const score_uds *u, score_setup_ensemble *e) {
+    /* init_parameter_idealtexture (SCORE_Kernel.cpp:668-712): name, Bunge angles
+     * and scatter in degrees (float columns 2-5) */
+    const UdsBlock *b;
+    int rc = block_or_fail(u, "IdealComponents", &b);
+    if (rc) return rc;
+    if (b->lines.size() > kMaxInputDataLen) return fail(SCORE_SETUP_ERR_RANGE, "too many ideal components");
+    for (size_t ii = 0; ii < b->lines.size(); ii++) {
+        Ideal c;
+        double raw;
+        for (int a = 0; a < 3; a++) {
+            if ((rc = table_real(*b, ii, 1 + a, &raw))) return rc;
+            c.bunge[a] = deg2rad(raw);
+        }
+        if ((rc = table_real(*b, ii, 4, &raw))) return rc;
+        c.scatter = deg2rad(raw);
+        if (c.bunge[0] < kBungeMin || c.bunge[0] > kBungeMax || c.bunge[1] < kBungeMin || c.bunge[1] > kBungeMax ||
+            c.bunge[2] < kBungeMin || c.bunge[2] > kBungeMax)
+            return fail(SCORE_SETUP_ERR_RANGE, "Invalid Euler angles in ideal orientation defintion");
+        score_orient::euler2quat(c.bunge, c.q);
+        e->ideals.push_back(c);
+    }
+    return 0;
+}
+
+/* ens_get_closest_standardlage (SCORE_OrientationHdl.cpp:51-77): the ideal
+ * component within its scatter that is closest (the last of equally close
+ * ones), numbered from 1; 0 is RANDOM */
+uint32_t closest_ideal(const score_setup_ensemble *e, const double *q) {
+    double closest = kMaxFcc;
+    uint32_t best = UINT32_MAX;
+    for (uint32_t cand = 0; cand < e->ideals.size(); cand++) {
+        const double disori = score_orient::disori_angle(q, e->ideals[cand].q);
+        if (disori <= e->ideals[cand].scatter && disori <= closest) {
+            closest = disori;
+            best = cand;
+        }
+    }
+    return best == UINT32_MAX ? 0u : best + 1u;
+}
+
 int read_schedule(const score_uds *u, score_setup_ensemble *e) {
     /* init_parameter_processing (SCORE_Kernel.cpp:716-754): time in s and
      * temperature in degrees Celsius, float columns, strictly increasing time */
@@ -724,7 +774,7 @@ int score_setup_ensemble_create(const score_uds *u, score_setup_ensemble **out) 
     int rc;
     /* the order of SCORE's init_parameter_* calls (SCORE_Kernel.cpp:1550-1612) */
     if ((rc = read_runtime(u, e)) || (rc = read_ensemble(u, e)) || (rc = read_nucleation(u, e)) ||
-        (rc = read_physics(u, e)) || (rc = read_schedule(u, e)) || (rc = read_pools(u, e)) ||
+        (rc = read_physics(u, e)) || (rc = read_ideals(u, e)) || (rc = read_schedule(u, e)) || (rc = read_pools(u, e)) ||
         (rc = read_defrag(u, e))) {
         delete e;
         return rc;
@@ -746,6 +796,22 @@ int score_setup_ensemble_get_info(const score_setup_ensemble *e, score_setup_ens
     out->n_world_ori = (uint32_t)e->oris.size();
     out->n_world_defg = (uint32_t)e->defg.size();
     out->n_world_rxg = e->n_rxg;
+    out->n_ideal = (uint32_t)e->ideals.size();
+    return 0;
+}
+
+int score_setup_ideal(const score_setup_ensemble *e, uint32_t i, double bunge[3], double q[4], double *scatter) {
+    if (!e || !bunge || !q || !scatter || i >= e->ideals.size())
+        return fail(SCORE_SETUP_ERR_ARG, "ideal component index out of range");
+    memcpy(bunge, e->ideals[i].bunge, 3 * sizeof(double));
+    memcpy(q, e->ideals[i].q, 4 * sizeof(double));
+    *scatter = e->ideals[i].scatter;
+    return 0;
+}
+
+int score_setup_closest_ideal(const score_setup_ensemble *e, const double q[4], uint32_t *cls) {
+    if (!e || !q || !cls) return fail(SCORE_SETUP_ERR_ARG, "null argument");
+    *cls = closest_ideal(e, q);
     return 0;
 }
 

@@ -25,7 +25,7 @@ from .synth import DomainArrays, ParamsSpec
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_setup.so")
 DEFAULT_SEED = -46356  # SCORE_SETUP_DEFAULT_SEED
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 ERR = {-1: "ERR_ARG", -2: "ERR_FILE", -3: "ERR_FORMAT", -4: "ERR_RANGE", -5: "ERR_UNSUPPORTED", -6: "ERR_OOM"}
 ERR_ARG, ERR_FILE, ERR_FORMAT, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6
@@ -44,7 +44,8 @@ class Mt(C.Structure):
 class EnsembleInfo(C.Structure):
     _fields_ = [("n_domains", C.c_uint32), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
                 ("cellsize", C.c_double), ("defg_mean", C.c_double * 3), ("n_nuclei", C.c_uint32),
-                ("n_world_ori", C.c_uint32), ("n_world_defg", C.c_uint32), ("n_world_rxg", C.c_uint32)]
+                ("n_world_ori", C.c_uint32), ("n_world_defg", C.c_uint32), ("n_world_rxg", C.c_uint32),
+                ("n_ideal", C.c_uint32)]
 
 
 class DomainInfo(C.Structure):
@@ -80,6 +81,8 @@ def load() -> C.CDLL:
             "score_setup_world_quat": ([vp, C.c_uint32, f64p], C.c_int),
             "score_setup_world_defg": ([vp, C.c_uint32, u32p, f64p], C.c_int),
             "score_setup_ensemble_free": ([vp], None),
+            "score_setup_ideal": ([vp, C.c_uint32, f64p, f64p, f64p], C.c_int),
+            "score_setup_closest_ideal": ([vp, f64p, u32p], C.c_int),
             "score_setup_domain_create": ([vp, C.c_uint32, C.c_int64, C.POINTER(vp)], C.c_int),
             "score_setup_domain_view": ([vp, C.POINTER(abi.Domain), C.POINTER(abi.Params)], C.c_int),
             "score_setup_domain_get_info": ([vp, C.POINTER(DomainInfo)], C.c_int),
@@ -194,6 +197,31 @@ class Ensemble:
         o, r = C.c_uint32(), C.c_double()
         _check(load().score_setup_world_defg(self._h, g, C.byref(o), C.byref(r)))
         return o.value, r.value
+
+    def ideals(self) -> list[tuple[tuple, np.ndarray, float]]:
+        """the deck's ideal texture components: (Bunge angles in rad, quaternion,
+        scatter in rad); texture class i + 1 is component i, class 0 is RANDOM"""
+        out = []
+        for i in range(self.info.n_ideal):
+            b, q, s = np.zeros(3), np.zeros(4), C.c_double()
+            _check(load().score_setup_ideal(self._h, i, b.ctypes.data_as(C.POINTER(C.c_double)),
+                                            q.ctypes.data_as(C.POINTER(C.c_double)), C.byref(s)))
+            out.append((tuple(b.tolist()), q, s.value))
+        return out
+
+    def closest_ideal(self, q) -> int:
+        """texture class of an orientation (ens_get_closest_standardlage)"""
+        qa = np.ascontiguousarray(q, np.float64)
+        c = C.c_uint32()
+        _check(load().score_setup_closest_ideal(self._h, qa.ctypes.data_as(C.POINTER(C.c_double)), C.byref(c)))
+        return c.value
+
+    def texture_classes(self, dom: DomainArrays) -> np.ndarray:
+        """the texture class of every grain of a domain, deformed grains first, then
+        nuclei: closestideal of its orientation in the domain's pool"""
+        ori = np.concatenate([np.asarray(dom.defg_ori), np.asarray(dom.rxg_ori)])
+        per_ori = np.array([self.closest_ideal(q) for q in np.asarray(dom.ori_quat)], np.uint32)
+        return per_ori[ori]
 
     def domain(self, jobid: int = 0, seed: int = DEFAULT_SEED) -> tuple[DomainArrays, ParamsSpec, DomainSetup]:
         """Solitary unit `jobid` as (DomainArrays, ParamsSpec, what was drawn).

@@ -1,7 +1,7 @@
-"""Ensemble post-processing of growth runs: rediscretized kinetics and the final
-grain-size distribution, as SCORE's solitary-unit analysis computes them
-(ensembleHdl::postprocess_*, SCORE_SolitaryUnitAveraging.cpp:51-93, :546-672,
-:809-1063) from each domain's grain-evolution log (caHdl::log_grainevo,
+"""Ensemble post-processing of growth runs: rediscretized kinetics, macrotexture
+and the final grain-size distribution, as SCORE's solitary-unit analysis
+computes them (ensembleHdl::postprocess_*, SCORE_SolitaryUnitAveraging.cpp:51-93,
+:546-1063) from each domain's grain-evolution log (caHdl::log_grainevo,
 SCORE_OnTheFlyCharacterization.cpp:478-530).
 
 A domain's log holds, at each log point, the time and the cell count of every
@@ -10,7 +10,10 @@ the first step whose Xcells reaches each output threshold, before t advances
 (SCORE_OnTheFlyCharacterization.cpp:646-651).  From the growth machine's
 trajectory the same points give the time and Xcells; the deformed grains then
 hold N - Xcells cells in all, which is all the kinetics needs, and the nuclei's
-final counts give the final grain sizes.
+final counts give the final grain sizes.  The macrotexture needs every grain's
+count at every log point: grain_log_by_steps records them by stepping a growth
+machine (or the oracle) one step at a time and reading its grain counts where
+SCORE's master section logs.
 """
 from __future__ import annotations
 
@@ -35,6 +38,7 @@ class GrainLog:
     tsimend: float
     tincub: np.ndarray | None = None
     ideal: np.ndarray | None = None
+    classes: np.ndarray | None = None   # texture class of every row (closestideal; 0 = RANDOM)
 
     def interp_cell_count(self, gid: int, when: float) -> float:
         """caHdl::get_interpCellCount (SCORE_SolitaryUnitAveraging.cpp:1304-1363)"""
@@ -76,6 +80,32 @@ def grain_log_from_trajectory(traj: np.ndarray, N: int, defg_counts, rxg_counts,
     return GrainLog(N=N, ndefg=1, times=np.asarray(times), counts=counts, final_counts=final, tsimend=tend)
 
 
+def grain_log_by_steps(machine, N: int, xlog, classes=None) -> GrainLog:
+    """A domain's full log (log_grainevo): steps `machine` (a GrowthMachine or the
+    oracle) to the end of its run one step at a time; at the first step whose
+    Xcells reaches each threshold of `xlog` (in cells, ascending; one per step) the
+    cell count of every grain, deformed grains first, is recorded with the step's
+    start time, as SCORE's master section does before t advances
+    (log_OUTPUT_Diehl, SCORE_OnTheFlyCharacterization.cpp:646-651)."""
+    times, cols = [], []
+    k = 0
+    done = bool(machine.result()["completed"])
+    while not done:
+        t0 = float(machine.result()["t"])
+        done = machine.step(1)
+        if k < len(xlog) and float(machine.result()["xcells"]) >= xlog[k]:
+            dg, rg = machine.counts()
+            times.append(t0)
+            cols.append(np.concatenate([dg, rg]).astype(np.float64))
+            k += 1
+    dg, rg = machine.counts()
+    final = np.concatenate([dg, rg]).astype(np.float64)
+    counts = np.stack(cols, axis=1) if cols else np.zeros((len(final), 1))
+    return GrainLog(N=N, ndefg=len(dg), times=np.asarray(times), counts=counts, final_counts=final,
+                    tsimend=float(machine.result()["t"]),
+                    classes=None if classes is None else np.asarray(classes, np.int64))
+
+
 def rediscr_times(tmin: float, tmax: float, nslots: int) -> np.ndarray:
     """postprocess_initrediscrtimes: equidistant slots, each the end of its interval"""
     dt = (tmax - tmin) / float(nslots - 1)
@@ -115,6 +145,39 @@ def write_kinetics_csv(path: str, rows, vref: float) -> None:
         for n, t, v, x, lnt, lnx in rows:
             f.write(f"{n};{t:g};{v:g};{x:g};" + ("" if lnt is None else f"{lnt:g}") + ";" +
                     ("" if lnx is None else f"{lnx:g}") + "\n")
+
+
+def macrotexture(logs, times, nclasses: int) -> tuple[np.ndarray, float]:
+    """postprocess_rediscr_macrotexture (SCORE_SolitaryUnitAveraging.cpp:673-806):
+    the interpolated volume of every grain (deformed, then nuclei) summed into its
+    texture class at each time, per domain and then over the ensemble; rows are
+    times, columns classes (0 = RANDOM).  Also the total box volume."""
+    ens = np.zeros((len(times), nclasses))
+    vref = 0.0
+    for lg in logs:
+        if lg.classes is None:
+            raise ValueError("the log has no texture classes")
+        vref = vref + float(lg.N)
+        dom = np.zeros((len(times), nclasses))
+        for g in range(len(lg.final_counts)):
+            c = int(lg.classes[g])
+            for n, t in enumerate(times):
+                dom[n, c] += lg.interp_cell_count(g, t)
+        ens += dom
+    return ens, vref
+
+
+def write_macrotexture_csv(path: str, times, vol, ideals_bunge, vref: float) -> None:
+    """SCORE.<simid>.Rediscretized.Macrotexture.csv: `ideals_bunge` holds each ideal
+    component's Bunge angles in rad, printed in degrees (RAD2DEG, SCORE_Math.h:58)"""
+    deg = lambda r: (r) / (3.1415926535897932384626433832795) * (180.0)
+    with open(path, "w") as f:
+        f.write("Step;RediscrTime/s;RANDOM-phi1-PHI-phi2")
+        for i, b in enumerate(ideals_bunge):
+            f.write(f";{i}-{deg(b[0]):g}-{deg(b[1]):g}-{deg(b[2]):g}")
+        f.write(f";SumRediscrVolume={vref:g}\n")
+        for n, t in enumerate(times):
+            f.write(f"{n};{float(t):g}" + "".join(f";{float(v):g}" for v in vol[n]) + ";\n")
 
 
 def final_gsd(logs, now: float) -> dict:

@@ -62,7 +62,8 @@ def pool_rows(seed: int = 5, n: int = 24):
 
 
 def deck_text(overrides: dict | None = None, pool=None, schedule=((0.0, 726.85), (360000.0, 726.85)),
-              defrag=(0.30, 0.60), rxpool=((0.0, 30.0, 45.0, 0.0), (55.0, 55.0, 45.0, 0.0))) -> str:
+              defrag=(0.30, 0.60), rxpool=((0.0, 30.0, 45.0, 0.0), (55.0, 55.0, 45.0, 0.0)),
+              ideals=(("Cube", 0.0, 0.0, 0.0, 10.0),)) -> str:
     blocks = {k: dict(v) for k, v in DEFAULTS.items()}
     for (blk, key), val in (overrides or {}).items():
         blocks[blk][key] = val
@@ -73,8 +74,10 @@ def deck_text(overrides: dict | None = None, pool=None, schedule=((0.0, 726.85),
             out.append(f'"{k}"\t\t\t"{v}"\t\t// {name}\n')
         out.append("\n")
     out.append("|| EvoMicrochemistry\t\t|| *(ff)\t\t|| time, fr\n0.0\t\t0.00E+0\n360000.0\t\t0.00E+0\n\n")
-    out.append("|| IdealComponents\t\t|| *(sffff)\t\t|| Name, Bunge1, Bunge2, Bunge3, scatter\n"
-               '"Cube"\t\t0.0\t\t0.0\t\t0.0\t\t10.0\n\n')
+    out.append("|| IdealComponents\t\t|| *(sffff)\t\t|| Name, Bunge1, Bunge2, Bunge3, scatter\n")
+    for name, b1, b2, b3, sc in ideals:
+        out.append(f'"{name}"\t\t{b1}\t\t{b2}\t\t{b3}\t\t{sc}\n')
+    out.append("\n")
     out.append("|| AnnealingSchedule\t\t|| *(ff)\t\t|| time, temperature\n")
     for t, T in schedule:
         out.append(f"{t}\t\t{T}\n")

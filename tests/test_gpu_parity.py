@@ -497,3 +497,29 @@ def test_profile_counters(require_gpu):
     assert pf["n_reanchor_steps"] >= 1 and pf["n_predicted"] > 0
     assert all(v >= 0.0 for v in pf["ms_kernel"].values())
     assert pf["ms_step"] >= max(pf["ms_kernel"].values())
+
+
+def test_per_grain_log_and_macrotexture_from_deck(require_gpu):
+    """A deck domain stepped to the end of its run, the cell count of every grain
+    read where SCORE logs its grain evolution: the same log, and so the same
+    rediscretized macrotexture, from the GPU and from the oracle."""
+    import os
+    from score_amd import domain_setup as ds
+    from score_amd import postprocess as pp
+    deck = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cuboid_fe.uds")
+    e = ds.Ensemble.from_uds(deck)
+    dom, prm, _ = e.domain(1)
+    classes = e.texture_classes(dom)
+    assert len(classes) == dom.n_defg + dom.n_rxg and set(classes.tolist()) <= set(range(1 + e.info.n_ideal))
+    xlog = [0.0] + [0.02 * i * dom.N for i in range(1, 51)]
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    lo = pp.grain_log_by_steps(o, dom.N, xlog, classes)
+    lgpu = pp.grain_log_by_steps(g, dom.N, xlog, classes)
+    assert lo.times.view(np.uint64).tolist() == lgpu.times.view(np.uint64).tolist()
+    assert np.array_equal(lo.counts, lgpu.counts) and np.array_equal(lo.final_counts, lgpu.final_counts)
+    assert lo.tsimend == lgpu.tsimend
+    times = pp.rediscr_times(0.0, lo.tsimend, 100)
+    vo, _ = pp.macrotexture([lo], times, 1 + e.info.n_ideal)
+    vg, _ = pp.macrotexture([lgpu], times, 1 + e.info.n_ideal)
+    assert np.array_equal(vo, vg)
+    assert_final_equal(o, g)
