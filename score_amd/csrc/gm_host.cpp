@@ -144,7 +144,8 @@ struct score_gm_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     GmDev d{};
-    GmStats *hst = nullptr; /* pinned mirror of d.st */
+    GmStats *hst = nullptr; /* pinned mirror of d.st (mapped, coherent), written by k_stats_out */
+    GmStats *hst_dev = nullptr; /* its device address */
     score_gm_params p{};
     std::vector<double> sched_t, sched_T, defrag_at;
     std::vector<double> defq, rxq;
@@ -172,8 +173,8 @@ struct score_gm_handle {
     score_gm_profile profc{};
     /* staging for pending P */
     std::vector<unsigned long long> pend_keys;
-    double *hpend_vals = nullptr; /* pinned */
-    double *d_pend_vals = nullptr;
+    double *hpend_vals = nullptr; /* pinned, mapped and coherent: k_resolve_pending reads it */
+    double *hpend_dev = nullptr;  /* its device address */
     uint32_t pend_vals_cap = 0;
     /* scalars of the previous sweep: a change forces re-anchoring of rxFrac */
     GmSweepParams last_sp{};
@@ -192,6 +193,40 @@ struct score_gm_handle {
 };
 
 namespace {
+
+/* Waits for the stream by polling it: a window takes well under a millisecond,
+ * and a blocking wait adds the wake-up of a sleeping thread to every window. */
+hipError_t stream_wait(hipStream_t s) {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e != hipErrorNotReady) return e;
+        __builtin_ia32_pause();
+    }
+}
+
+/* Pinned host memory that kernels read or write directly: mapped into the
+ * device's address space and coherent (not cached by the device), so that
+ * what a kernel stores is in host memory when the stream has been synchronised
+ * and what the host stores before a launch is what the kernel reads. */
+int pinned_mapped(score_gm_handle *h, void **host, void **dev, size_t bytes) {
+    (void)h;
+    *host = *dev = nullptr;
+    hipError_t e = hipHostMalloc(host, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) {
+        e = hipHostGetDevicePointer(dev, *host, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(*host);
+            *host = nullptr;
+        }
+    }
+    if (e != hipSuccess) {
+        *host = *dev = nullptr;
+        (void)hipGetLastError();
+        g_last_error = std::string("pinned host memory: ") + hipGetErrorString(e);
+        return SCORE_GM_ERR_OOM;
+    }
+    return 0;
+}
 
 /* Device memory comes from the device's stream-ordered pool on the handle's
  * stream: a free does not wait for the whole device (as hipFree does), so
@@ -220,14 +255,15 @@ void free_device(score_gm_handle *h) {
     void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
-                    d.scan_bsum, d.st, h->d_pend_vals};
+                    d.scan_bsum, d.st};
     for (void *q : ptrs) dfree(h, q);
     memset(&h->d, 0, sizeof(h->d));
-    h->d_pend_vals = nullptr;
     if (h->hst) (void)hipHostFree(h->hst);
     h->hst = nullptr;
+    h->hst_dev = nullptr;
     if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
     h->hpend_vals = nullptr;
+    h->hpend_dev = nullptr;
     for (hipEvent_t &ev : h->ev)
         if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -528,19 +564,14 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
         keys = h->pend_keys.data();
     }
     if (npend > h->pend_vals_cap) {
-        /* both staging buffers are idle here: the stream was synchronised after the step */
+        /* the staging buffer is idle here: the stream was synchronised after the step */
         const uint32_t nc = std::max<uint32_t>(std::max<uint32_t>(npend, 2 * h->pend_vals_cap), 1024);
-        dfree(h, h->d_pend_vals);
-        h->d_pend_vals = nullptr;
         if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
         h->hpend_vals = nullptr;
+        h->hpend_dev = nullptr;
         h->pend_vals_cap = 0;
-        int rc = dalloc(h, &h->d_pend_vals, nc);
+        int rc = pinned_mapped(h, (void **)&h->hpend_vals, (void **)&h->hpend_dev, sizeof(double) * nc);
         if (rc) return rc;
-        if (hipHostMalloc((void **)&h->hpend_vals, sizeof(double) * nc, hipHostMallocDefault) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(SCORE_GM_ERR_OOM, "pinned P staging");
-        }
         h->pend_vals_cap = nc;
     }
     for (uint32_t i = 0; i < npend; i++) {
@@ -550,10 +581,10 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
         h->hpend_vals[i] = P;
         if (P >= h->regPmax) h->regPmax = P;
     }
-    /* stream-ordered: the values reach the cache before the next step's kernels
-     * read it, and the pinned buffer is not rewritten before the next step's sync */
-    HIPCHK(hipMemcpyAsync(h->d_pend_vals, h->hpend_vals, sizeof(double) * npend, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(gm_launch_resolve_pending(d, h->d_pend_vals, npend, h->stream));
+    /* stream-ordered: the kernel reads the values from the mapped buffer and
+     * stores them into the cache before the next step's kernels read it; the
+     * buffer is not rewritten before the next window's sync */
+    HIPCHK(gm_launch_resolve_pending(d, h->hpend_dev, npend, h->stream));
     h->hcount += npend;
     return 0;
 }
@@ -605,12 +636,12 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     /* the first INACTIVE slot; the reference's search runs on past nnever while
      * it meets ACTIVE records left behind by an earlier defragmentation */
     HIPCHK(gm_launch_defrag_first_inactive(d, base, 0, nnever, s));
-    HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+    HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
     HIPCHK(hipStreamSynchronize(s));
     uint32_t first = h->hst->dfirst;
     if (first >= nnever && nnever < cap) {
         HIPCHK(gm_launch_defrag_first_inactive(d, base, nnever, cap, s));
-        HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
         HIPCHK(hipStreamSynchronize(s));
         first = h->hst->dfirst;
     }
@@ -628,7 +659,7 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     }
     hipError_t e;
     if ((e = gm_launch_defrag_move(d, base, first, nnever, h->step, trec, tcs, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = gm_launch_stats_out(d, h->hst_dev, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess) {
         dfree(h, trec);
         dfree(h, tcs);
@@ -782,13 +813,14 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
             (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
             return h->err = hip_fail(e, "growth step");
     }
-    if ((e = hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+    if ((e = gm_launch_stats_out(d, h->hst_dev, h->stream)) != hipSuccess || (e = stream_wait(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
     h->last_sp = sp;
     h->have_last_sp = true;
     h->force_reanchor = false;
-    GmStats st = *h->hst;
+    /* read in place: only the rows and inline ids the window filled are touched,
+     * and nothing rewrites the mirror before the next window's end */
+    const GmStats &st = *h->hst;
     const uint32_t nexec = st.nexec;
     if (st.err) return h->err = step_error(st.err);
     if (nexec < 1 || nexec > W) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: window step count");
@@ -825,6 +857,24 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     /* the window's new pairs, evaluated with glibc and stored in the device cache */
     if ((rc = resolve_pending(h, st))) return h->err = rc;
     if (st.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
+    {
+        const GmStepRow &last = st.rows[nexec - 1];
+        h->nI = last.nI;
+        h->nB = last.nB;
+    }
+    if ((rc = grow_pcache(h))) return h->err = rc;
+    if (st.ndeferred) {
+        /* cells that waited for a weight: predicted under the window's scalars at
+         * the step after the window, queued before the host's master section so
+         * that the device works while the host does */
+        GmSweepArgs ad = a;
+        ad.sp = sp;
+        ad.k = h->step + nexec;
+        ad.nI = h->nI;
+        ad.nB = h->nB;
+        if ((e = gm_launch_predict_deferred(d, ad, st.ndeferred, h->stream)) != hipSuccess)
+            return h->err = hip_fail(e, "deferred predictions");
+    }
     /* master section of every step (SCORE_Kernel.cpp:6688-6742) */
     uint32_t prev_slots = slots0;
     for (uint32_t j = 0; j < nexec; j++) {
@@ -851,8 +901,9 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
                 const uint32_t *ids = st.zeroed_ids;
                 if (st.nzeroed > GM_ZEROED_INLINE) {
                     h->zbuf.resize(st.nzeroed);
-                    if ((e = hipMemcpy(h->zbuf.data(), d.zlist, sizeof(uint32_t) * std::min(st.nzeroed, d.ndefg),
-                                       hipMemcpyDeviceToHost)) != hipSuccess)
+                    if ((e = hipMemcpyAsync(h->zbuf.data(), d.zlist, sizeof(uint32_t) * std::min(st.nzeroed, d.ndefg),
+                                            hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+                        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
                         return h->err = hip_fail(e, "exhausted grains");
                     ids = h->zbuf.data();
                 }
@@ -861,8 +912,9 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
             if (st.cwrap) {
                 /* a count went below zero and wrapped: cellcount > 0 again, as in SCORE */
                 std::vector<uint32_t> cnt(d.ndefg);
-                if ((e = hipMemcpy(cnt.data(), d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost)) !=
-                    hipSuccess)
+                if ((e = hipMemcpyAsync(cnt.data(), d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost,
+                                        h->stream)) != hipSuccess ||
+                    (e = hipStreamSynchronize(h->stream)) != hipSuccess)
                     return h->err = hip_fail(e, "deformed grain counts");
                 for (uint32_t g = 0; g < d.ndefg; g++) h->alive[g] = cnt[g] > 0 ? 1 : 0;
                 h->rho_top = 0;
@@ -890,20 +942,6 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         h->traj.push_back(r);
         if (j + 1 < nexec && !same_scalars(sweep_params(h), sp))
             return h->err = fail(SCORE_GM_ERR_STATE, "internal error: scalars changed inside a launch window");
-    }
-    const GmStepRow &last = st.rows[nexec - 1];
-    h->nI = last.nI;
-    h->nB = last.nB;
-    if ((rc = grow_pcache(h))) return h->err = rc;
-    if (st.ndeferred) {
-        /* cells that waited for a weight: predicted under the window's scalars */
-        GmSweepArgs ad = a;
-        ad.sp = sp;
-        ad.k = h->step;
-        ad.nI = h->nI;
-        ad.nB = h->nB;
-        if ((e = gm_launch_predict_deferred(d, ad, st.ndeferred, h->stream)) != hipSuccess)
-            return h->err = hip_fail(e, "deferred predictions");
     }
     if (h->rho_changed) {
         /* recovered dislocation densities for the next step (after the
@@ -1093,11 +1131,10 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return rc;
     }
-    if (hipHostMalloc((void **)&h->hst, sizeof(GmStats), hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
+    if ((rc = pinned_mapped(h, (void **)&h->hst, (void **)&h->hst_dev, sizeof(GmStats)))) {
         free_device(h);
         delete h;
-        return fail(SCORE_GM_ERR_OOM, "pinned stats");
+        return rc;
     }
     uint32_t *d_site = nullptr;
     double *d_nucP = nullptr;
@@ -1124,7 +1161,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
             HIPCHK(hipMemcpyAsync(d_nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
         }
         HIPCHK(gm_launch_nucleate(d, d_site, d_nucP, d.nrxg, s));
-        HIPCHK(hipMemcpyAsync(h->hst, d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
         HIPCHK(hipStreamSynchronize(s));
         return 0;
     };
@@ -1309,7 +1346,7 @@ int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, ui
     if (rc) return rc;
     auto run = [&]() -> int {
         HIPCHK(gm_launch_status(h->d, h->nI, h->nB, h->step, rxthreshold, buf, buf + m, s));
-        HIPCHK(hipMemcpyAsync(h->hst, h->d.st, sizeof(GmStats), hipMemcpyDeviceToHost, s));
+        HIPCHK(gm_launch_stats_out(h->d, h->hst_dev, s));
         HIPCHK(hipStreamSynchronize(s));
         const uint32_t cnt = h->hst->dcount;
         *n = cnt;

@@ -69,8 +69,8 @@ static inline __host__ __device__ uint32_t gm_tiles_inside(uint32_t nI) {
 }
 #define GM_MAX_BLOCKS 8192        /* grid of the partitioned passes (scanned by one 1024-thread block) */
 #define GM_SCAN_BLOCKS 1024       /* fixed grid of the device-wide scans (nucleation) */
-#define GM_ZEROED_INLINE 62       /* exhausted grain ids returned with the per-step stats */
-#define GM_PEND_INLINE 64         /* new P-cache keys returned with the per-step stats */
+#define GM_ZEROED_INLINE 1022     /* exhausted grain ids returned with the per-step stats */
+#define GM_PEND_INLINE 2048       /* new P-cache keys returned with the per-step stats */
 #define GM_NK 9                   /* kernels per step (SCORE_GM_K_* order) */
 #define GM_WMAX 8                 /* steps per launch window */
 
@@ -114,7 +114,7 @@ struct GmStepRow {
     uint32_t pad[2];
 };
 
-struct GmStats {
+struct alignas(8) GmStats {
     uint32_t nI, nB;          /* nextSlotNeverActive of both lists */
     uint32_t sv;              /* ACTIVE slots seen by the sweep */
     uint32_t nfull;           /* parents (completed cells), Inside then Border */
@@ -233,6 +233,7 @@ hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s)
 hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, uint32_t ndeferred, hipStream_t s);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
+hipError_t gm_launch_stats_out(const GmDev &d, GmStats *out, hipStream_t s);
 /* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */
 struct GmFrontOut {
     uint8_t *act, *inf;

@@ -1201,6 +1201,18 @@ hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_
     return hipGetLastError();
 }
 
+/* the stats into the host's pinned mirror (mapped, coherent), word by word:
+ * cheaper than a copy command at the end of every window */
+__global__ void __launch_bounds__(GM_BLOCK) k_stats_out(GmDev d, uint32_t *out) {
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(d.st);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(GmStats) / 4); i += GM_BLOCK) out[i] = in[i];
+}
+
+hipError_t gm_launch_stats_out(const GmDev &d, GmStats *out, hipStream_t s) {
+    k_stats_out<<<1, GM_BLOCK, 0, s>>>(d, reinterpret_cast<uint32_t *>(out));
+    return hipGetLastError();
+}
+
 /* SCORE's per-slot view before step k: activity, rxFrac (replayed from the
  * anchor; INACTIVE slots hold 0.0 as the reference leaves them), and the record
  * fields, with pending weights replaced by their cached values in the record
