@@ -1202,10 +1202,19 @@ hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_
 }
 
 /* the stats into the host's pinned mirror (mapped, coherent), word by word:
- * cheaper than a copy command at the end of every window */
+ * cheaper than a copy command at the end of every window.  Only the filled part
+ * of the inline id and key arrays crosses the bus; the host reads no more. */
 __global__ void __launch_bounds__(GM_BLOCK) k_stats_out(GmDev d, uint32_t *out) {
     const uint32_t *in = reinterpret_cast<const uint32_t *>(d.st);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(GmStats) / 4); i += GM_BLOCK) out[i] = in[i];
+    const GmStats *st = d.st;
+    const uint32_t nz = st->nzeroed < GM_ZEROED_INLINE ? st->nzeroed : GM_ZEROED_INLINE;
+    const uint32_t np = st->npend < GM_PEND_INLINE ? st->npend : GM_PEND_INLINE;
+    const uint32_t z0 = offsetof(GmStats, zeroed_ids) / 4, p0 = offsetof(GmStats, pend_keys) / 4;
+    const uint32_t t0 = offsetof(GmStats, kcur) / 4, end = sizeof(GmStats) / 4;
+    for (uint32_t i = threadIdx.x; i < end; i += GM_BLOCK) {
+        const bool skip = (i >= z0 + nz && i < p0) || (i >= p0 + 2u * np && i < t0);
+        if (!skip) out[i] = in[i];
+    }
 }
 
 hipError_t gm_launch_stats_out(const GmDev &d, GmStats *out, hipStream_t s) {
