@@ -194,16 +194,6 @@ struct score_gm_handle {
 
 namespace {
 
-/* Waits for the stream by polling it: a window takes well under a millisecond,
- * and a blocking wait adds the wake-up of a sleeping thread to every window. */
-hipError_t stream_wait(hipStream_t s) {
-    for (;;) {
-        const hipError_t e = hipStreamQuery(s);
-        if (e != hipErrorNotReady) return e;
-        __builtin_ia32_pause();
-    }
-}
-
 /* Pinned host memory that kernels read or write directly: mapped into the
  * device's address space and coherent (not cached by the device), so that
  * what a kernel stores is in host memory when the stream has been synchronised
@@ -813,7 +803,8 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
             (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
             return h->err = hip_fail(e, "growth step");
     }
-    if ((e = gm_launch_stats_out(d, h->hst_dev, h->stream)) != hipSuccess || (e = stream_wait(h->stream)) != hipSuccess)
+    if ((e = gm_launch_stats_out(d, h->hst_dev, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
     h->last_sp = sp;
     h->have_last_sp = true;
