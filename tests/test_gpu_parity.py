@@ -523,3 +523,57 @@ def test_per_grain_log_and_macrotexture_from_deck(require_gpu):
     vg, _ = pp.macrotexture([lgpu], times, 1 + e.info.n_ideal)
     assert np.array_equal(vo, vg)
     assert_final_equal(o, g)
+
+
+def _mixed_case(seed):
+    """A small domain with settings drawn from a seeded generator: either
+    mobility model, a temperature ramp or none, any recovery model, drag
+    constant, time-dependent or none, defragmentation thresholds or none, and
+    a mobility prefactor over 2.5 decades (some draws stall below X = 0.01 and
+    run to NMAX)."""
+    rng = np.random.default_rng(1000 + seed)
+    nx, ny, nz = (int(v) for v in rng.integers(18, 40, 3))
+    dom = synth.make_voronoi_domain(nx, ny, nz, n_grains=int(rng.integers(2, 12)),
+                                    n_nuclei=int(rng.integers(3, 30)), seed=int(seed))
+    kw = {"mobility_model": abi.MOB_SEBALDGOTTSTEIN if rng.random() < 0.3 else abi.MOB_ROLLETTHOLM}
+    if rng.random() < 0.5:
+        T0, T1 = float(rng.uniform(950, 1000)), float(rng.uniform(1000, 1060))
+        kw["sched_time"], kw["sched_temp"] = (0.0, float(rng.uniform(200, 3000)), 360000.0), (T0, T1, T1)
+    rec = int(rng.integers(0, 4))
+    if rec == 1:
+        kw.update(recovery_mode=1, rec_vacancy_d0=float(10 ** rng.uniform(-7, -5)))
+    elif rec == 2:
+        kw.update(recovery_mode=2, rec_alpha3=0.0)
+    elif rec == 3:
+        kw.update(recovery_mode=3)
+    z = int(rng.integers(0, 3))
+    if z == 1:
+        kw.update(zener_mode=1, zener_alpha=1.5, zener_gamma=0.324, disp_time=(0.0,),
+                  disp_fr=(float(rng.uniform(1e4, 8e4)),))
+    elif z == 2:
+        kw.update(zener_mode=2, zener_alpha=1.5, zener_gamma=0.324, disp_time=(0.0, 500.0, 5000.0),
+                  disp_fr=(0.0, 5e4, 2e4))
+    if rng.random() < 0.4:
+        kw["defrag_at"] = tuple(sorted(float(x) * dom.N for x in rng.uniform(0.05, 0.9, 2)))
+    kw["rh_hagb_m0"] = float(10 ** rng.uniform(-1, 1.5))
+    kw["nmax"] = 3000
+    chunks = tuple(int(c) for c in rng.integers(1, 9, 4))
+    return dom, ParamsSpec(**kw), chunks
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_mixed_settings_in_launch_windows(require_gpu, seed):
+    dom, prm, chunks = _mixed_case(seed)
+    print(f"mixed case {seed}: {dom.nx}x{dom.ny}x{dom.nz} model={prm.mobility_model} recovery={prm.recovery_mode} "
+          f"zener={prm.zener_mode} defrag={len(prm.defrag_at)} chunks={chunks}", flush=True)
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
