@@ -73,13 +73,17 @@ GM_HD double gm_radd_round(double &f, double inc, double budget) {
     const double R = inc - qd * u;
     const double F = f * inv_u;
     const double twoR = R + R;
-    const bool tie = twoR == u;
     /* D*u is the exact change of one addition inside the binade */
-    double D = twoR < u ? qd : qd + 1.0;
-    if (tie) D = qd + (gm_odd(qd) ? 1.0 : 0.0);
-    if (tie && gm_odd(F)) { /* one hardware step makes f/u even */
-        f = f + inc;
-        return 1.0;
+    double D;
+    if (twoR == u) { /* a tie (rare): rounding to even decides */
+        /* F = f/u lies in [2^52, 2^53), so its parity is the last mantissa bit of f */
+        if (gm_dbits(f) & 1u) { /* one hardware step makes f/u even */
+            f = f + inc;
+            return 1.0;
+        }
+        D = qd + (gm_odd(qd) ? 1.0 : 0.0);
+    } else {
+        D = twoR < u ? qd : qd + 1.0;
     }
     if (D == 0.0) return 0.0; /* the increment is absorbed by rounding */
     /* addition i of the run keeps its exact sum below 2^(e+1) when
