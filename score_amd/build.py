@@ -91,8 +91,8 @@ REFERENCE = "/root/reference"
 
 def build_oracle() -> str:
     """The oracle (test infrastructure); where the reference tree is present,
-    also its random-number and orientation code into oracle/_ref (the checker's
-    reference leg; the GPU box only uses the prebuilt file)."""
+    also its random-number and orientation code into oracle/_ref (machines
+    without the reference tree use the file built here)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "-B", "liboracle.so"])
     if os.path.isdir(os.path.join(REFERENCE, "src")):
