@@ -33,7 +33,7 @@ extern "C" {
 #define SCORE_GM_ERR_OOM           -2  /* device or host allocation failed */
 #define SCORE_GM_ERR_HIP           -3  /* HIP runtime error */
 #define SCORE_GM_ERR_CAPACITY      -4  /* a fixed-size front list overflowed (SCORE would write out of bounds) */
-#define SCORE_GM_ERR_UNSUPPORTED   -5  /* option not implemented by this build */
+#define SCORE_GM_ERR_UNSUPPORTED   -5  /* option this library does not implement */
 #define SCORE_GM_ERR_STATE         -6  /* call not valid in the handle's current state */
 #define SCORE_GM_ERR_NODEVICE      -7  /* no usable gfx950 device */
 

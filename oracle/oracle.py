@@ -2,7 +2,7 @@
 
 The oracle is a serial CPU restatement of SCORE's growth machine with
 OMP_NUM_THREADS=1 semantics.  tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py use it as the checker; the library never does.
+CPU-baseline leg of bench.py compare the library with it; the library never uses it.
 """
 from __future__ import annotations
 

@@ -5,8 +5,8 @@
  * (caHdl::solve_RXGROWTH, SCORE_Kernel.cpp:6497-6792) with the semantics of a
  * run with OMP_NUM_THREADS=1: one region spanning the whole domain, an Inside
  * and a Border front list, recycle-then-append slot assignment, first-come
- * infection in list order and fixed direction order.  It is the checker for the
- * GPU library; it is never linked into the library, and only tests/, the smoke
+ * infection in list order and fixed direction order.  The GPU library is tested
+ * against it; it is never linked into the library, and only tests/, the smoke
  * entry point and the CPU-baseline leg of bench.py load it.
  *
  * Each function names the reference lines it follows.  It consumes the same
