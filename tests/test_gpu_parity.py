@@ -577,3 +577,25 @@ def test_mixed_settings_in_launch_windows(require_gpu, seed):
             break
         k += 1
     assert_final_equal(o, g)
+
+
+def test_hand_built_pin_cases(require_gpu):
+    """The two hand-built domains that pin the oracle bit for bit against the
+    reference's expressions (an Inside and a Border nucleus sharing nine target
+    cells; one nucleus under parameters at which every alternative association
+    of the increment and time-step chain changes bits), run stepwise on the GPU
+    to completion and compared slot by slot."""
+    from test_oracle_order_pins import DX, H_EV, M0, MAXFILL, RHO, T_K, lin, qaxis
+    n = 12
+    N = n ** 3
+    q = [qaxis([0, 0, 1], 0.0), qaxis([0, 0, 1], 45.0), qaxis([1, 1, 1], 60.0)]
+    dom = synth.DomainArrays(n, n, n, 0.5e-6, np.zeros(N, np.uint32), np.array(q), np.array([0], np.uint32),
+                             np.array([1e15]), np.array([1, 2], np.uint32),
+                             np.array([lin(2, 5, 5, n, n), lin(0, 5, 5, n, n)], np.uint32))
+    o, g = run_both(dom, ParamsSpec(xmax=1.0), stepwise=True)
+    assert_final_equal(o, g)
+    dom = synth.DomainArrays(n, n, n, DX, np.zeros(N, np.uint32), np.array(q[:2]), np.array([0], np.uint32),
+                             np.array([RHO]), np.array([1], np.uint32), np.array([lin(6, 6, 6, n, n)], np.uint32))
+    prm = ParamsSpec(rh_hagb_m0=M0, rh_hagb_hact_ev=H_EV, sched_temp=(T_K, T_K), maxfill=MAXFILL, xmax=1.0)
+    o, g = run_both(dom, prm, stepwise=True)
+    assert_final_equal(o, g)
