@@ -56,6 +56,7 @@
 #define O_MAXDISORI_TO_40DEG111 ((10.0) / (180.0) * (O_PI))
 #define O_FULLRXCACHING_ATT_FACTOR (1.5)
 #define O_FULLRECYCLING_ATT_FACTOR (3.0)
+#define O_DEFAULT_NMAX (1000000)   /* DEFAULT_NMAX, SCORE_Defs_Physics.h:133 */
 #define O_FGEOFACE (1.0000)
 #define O_FGEOEDGE (0.7957)
 #define O_FGEODIAG (0.6494)
@@ -620,6 +621,8 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
     }
     if (p->recovery_mode < 0 || p->recovery_mode > 3) return SCORE_GM_ERR_ARG;
     if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
+    /* the range SCORE accepts for NMAX (SCORE_Kernel.cpp:274) */
+    if (p->nmax < 1 || p->nmax > O_DEFAULT_NMAX) return SCORE_GM_ERR_ARG;
     if (p->n_defrag && !p->defrag_at) return SCORE_GM_ERR_ARG;
     uint64_t N = (uint64_t)d->nx * d->ny * d->nz;
     for (uint32_t i = 0; i < d->n_defg; i++)
@@ -764,10 +767,11 @@ static int o_push_traj(oracle *o, const score_gm_traj_row *r) {
     return 0;
 }
 
-/* termination test (SCORE_Kernel.cpp:6619) */
+/* termination test (SCORE_Kernel.cpp:6619); the loop itself runs at most
+ * DEFAULT_NMAX iterations (:6615) */
 static int o_done(const oracle *o) {
     return (o->Xcells >= (double)o->N) || (o->X > o->p.xmax) || ((int64_t)o->step > o->p.nmax) ||
-           (o->t > o->tprocessingend) || (o->t > o->p.tmax);
+           (o->t > o->tprocessingend) || (o->t > o->p.tmax) || ((int64_t)o->step >= O_DEFAULT_NMAX);
 }
 
 /* one iteration of the main loop (SCORE_Kernel.cpp:6615-6747); returns 1 when

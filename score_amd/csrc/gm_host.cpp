@@ -51,6 +51,7 @@ const double kRhoWellAnnealed = 1e8;
 const double kDefaultPmax = 0.0;
 const double kDefaultSmallNumber = 1e-6;
 const double kFullRecyclingAtt = 3.0;
+const int64_t kDefaultNmax = 1000000;          /* DEFAULT_NMAX, SCORE_Defs_Physics.h:133 */
 
 bool same_bits(double a, double b) { return memcmp(&a, &b, sizeof(double)) == 0; }
 
@@ -143,6 +144,7 @@ double rh_omega_threshold(const score_gm_params &p) {
 struct score_gm_handle {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipMemPool_t pool = nullptr; /* the library's own pool on this device */
     GmDev d{};
     GmStats *hst = nullptr; /* pinned mirror of d.st (mapped, coherent), written by k_stats_out */
     GmStats *hst_dev = nullptr; /* its device address */
@@ -194,6 +196,61 @@ struct score_gm_handle {
 
 namespace {
 
+/* The library's own stream-ordered memory pool per device.  Freed blocks stay in
+ * it while any handle of the library is alive on the device, so that handles
+ * created next to running ones (batches, the request loop) reuse them without
+ * a device-wide synchronisation; when the last handle on the device is
+ * destroyed, the pool gives its unused memory back.  The device's default pool
+ * and its settings are left alone. */
+struct LibraryPool {
+    std::mutex m;
+    hipMemPool_t pool = nullptr;
+    uint32_t handles = 0;
+};
+constexpr int kMaxDevices = 64;
+LibraryPool g_pools[kMaxDevices];
+
+int pool_acquire(int device, hipMemPool_t *out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return fail(SCORE_GM_ERR_NODEVICE, "device index out of range");
+    LibraryPool &lp = g_pools[device];
+    std::lock_guard<std::mutex> lk(lp.m);
+    if (!lp.pool) {
+        hipMemPoolProps props;
+        memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = device;
+        hipMemPool_t pool = nullptr;
+        hipError_t e = hipMemPoolCreate(&pool, &props);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return hip_fail(e, "hipMemPoolCreate");
+        }
+        uint64_t keep = UINT64_MAX;
+        e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            (void)hipGetLastError();
+            return hip_fail(e, "hipMemPoolSetAttribute");
+        }
+        lp.pool = pool;
+    }
+    lp.handles++;
+    *out = lp.pool;
+    return 0;
+}
+
+/* after the handle's frees have completed (its stream synchronised) */
+void pool_release(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    LibraryPool &lp = g_pools[device];
+    std::lock_guard<std::mutex> lk(lp.m);
+    if (lp.handles == 0) return;
+    if (--lp.handles == 0 && lp.pool) (void)hipMemPoolTrimTo(lp.pool, 0);
+}
+
 /* Pinned host memory that kernels read or write directly: mapped into the
  * device's address space and coherent (not cached by the device), so that
  * what a kernel stores is in host memory when the stream has been synchronised
@@ -226,11 +283,11 @@ template <typename T>
 int dalloc(score_gm_handle *h, T **p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMallocAsync((void **)p, n * sizeof(T), h->stream);
+    hipError_t e = hipMallocFromPoolAsync((void **)p, n * sizeof(T), h->pool, h->stream);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
-        g_last_error = std::string("hipMallocAsync: ") + hipGetErrorString(e);
+        g_last_error = std::string("hipMallocFromPoolAsync: ") + hipGetErrorString(e);
         return SCORE_GM_ERR_OOM;
     }
     return 0;
@@ -256,8 +313,13 @@ void free_device(score_gm_handle *h) {
     h->hpend_dev = nullptr;
     for (hipEvent_t &ev : h->ev)
         if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
     h->stream = nullptr;
+    if (h->pool) pool_release(h->device);
+    h->pool = nullptr;
 }
 
 int validate(const score_gm_domain *dom, const score_gm_params *prm) {
@@ -290,6 +352,8 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
         return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");
     if (!(prm->maxfill > 0.0)) return fail(SCORE_GM_ERR_ARG, "maxfill must be positive");
+    /* the range SCORE accepts for NMAX (SCORE_Kernel.cpp:274) */
+    if (prm->nmax < 1 || prm->nmax > kDefaultNmax) return fail(SCORE_GM_ERR_ARG, "nmax outside [1, 1000000] (DEFAULT_NMAX)");
     uint64_t N = (uint64_t)dom->nx * dom->ny * dom->nz;
     for (uint32_t i = 0; i < dom->n_defg; i++)
         if (dom->defg_ori[i] >= dom->n_ori) return fail(SCORE_GM_ERR_ARG, "defg_ori out of range");
@@ -499,10 +563,11 @@ void update_rhomax(score_gm_handle *h) {
     h->rhomax = r;
 }
 
-/* termination test (SCORE_Kernel.cpp:6619) */
+/* termination test (SCORE_Kernel.cpp:6619); the main loop itself ends after
+ * DEFAULT_NMAX iterations (:6615) */
 bool done(const score_gm_handle *h) {
     return (h->Xcells >= (double)h->d.N) || (h->X > h->p.xmax) || ((int64_t)h->step > h->p.nmax) ||
-           (h->t > h->tprocessingend) || (h->t > h->p.tmax);
+           (h->t > h->tprocessingend) || (h->t > h->p.tmax) || ((int64_t)h->step >= kDefaultNmax);
 }
 
 uint32_t pow2_at_least(uint64_t v) {
@@ -579,11 +644,14 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
     return 0;
 }
 
-/* Double the cache once it is half full: markers in the front are resolved
- * first, then every entry is re-inserted into the larger table. */
-int grow_pcache(score_gm_handle *h) {
+int step_error(uint32_t err);
+
+/* Grow the cache (doubling) until it is at most half full with `room` more
+ * entries: markers in the front are resolved first, then every entry is
+ * re-inserted into the larger table. */
+int grow_pcache(score_gm_handle *h, uint64_t room = 0) {
     GmDev &d = h->d;
-    if ((uint64_t)h->hcount * 2 <= d.hcap) return 0;
+    if (((uint64_t)h->hcount + room) * 2 <= d.hcap) return 0;
     const GmFrontOut none{};
     HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, none, h->stream));
     HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, none, h->stream));
@@ -596,7 +664,10 @@ int grow_pcache(score_gm_handle *h) {
     std::vector<double> v2;
     for (uint32_t i = 0; i < d.hcap; i++)
         if (keys[i]) { k2.push_back(keys[i]); v2.push_back(vals[i]); }
-    const uint32_t ncap = d.hcap * 2u;
+    uint64_t ncap64 = (uint64_t)d.hcap * 2u;
+    while (((uint64_t)h->hcount + room) * 2 > ncap64) ncap64 *= 2u;
+    if (ncap64 > (1ull << 31)) return fail(SCORE_GM_ERR_OOM, "mobility-weight cache beyond 2^31 entries");
+    const uint32_t ncap = (uint32_t)ncap64;
     dfree(h, d.hkeys); dfree(h, d.hvals); dfree(h, d.pend_slot); dfree(h, d.pend_key);
     d.hkeys = nullptr; d.hvals = nullptr; d.pend_slot = nullptr; d.pend_key = nullptr;
     int rc;
@@ -612,10 +683,32 @@ int grow_pcache(score_gm_handle *h) {
     HIPCHK(hipMemcpyAsync(dk, k2.data(), sizeof(unsigned long long) * k2.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(dv, v2.data(), sizeof(double) * v2.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(gm_launch_hash_insert(d, dk, dv, (uint32_t)k2.size(), h->stream));
+    HIPCHK(gm_launch_stats_out(d, h->hst_dev, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h, dk);
     dfree(h, dv);
+    if (h->hst->err & GM_ERRF_PCACHE_FULL) return h->err = step_error(h->hst->err);
     return 0;
+}
+
+/* The window's weight lookups spilled (no room near a key's home in the cache):
+ * grow the cache for them, let the spilled cells look their pairs up again and
+ * resolve the new pairs with glibc, until none spills.  Runs after the window's
+ * own pending weights are resolved and before its deferred cells are predicted.
+ * The stats mirror is rewritten here; the caller keeps what it needs of it. */
+int resolve_spilled(score_gm_handle *h, uint32_t nspilled, uint32_t ndeferred) {
+    GmDev &d = h->d;
+    for (int round = 0; nspilled; round++) {
+        if (round == 8) return fail(SCORE_GM_ERR_STATE, "internal error: weight lookups keep spilling");
+        int rc = grow_pcache(h, nspilled);
+        if (rc) return rc;
+        HIPCHK(gm_launch_reinsert_spilled(d, h->pp, ndeferred, h->nI, h->nB, h->stream));
+        HIPCHK(gm_launch_stats_out(d, h->hst_dev, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if ((rc = resolve_pending(h, *h->hst))) return rc;
+        nspilled = h->hst->novf;
+    }
+    return grow_pcache(h);
 }
 
 /* ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934) of the list whose slot
@@ -735,7 +828,9 @@ int step_error(uint32_t err) {
         return fail(SCORE_GM_ERR_CAPACITY, "Border front list full (SCORE would write out of bounds)");
     if (err & GM_ERRF_INSIDE_FULL) return fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
     if (err & GM_ERRF_MISSED) return fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
-    return fail(SCORE_GM_ERR_CAPACITY, "mobility-weight cache full");
+    if (err & GM_ERRF_KEYS)
+        return fail(SCORE_GM_ERR_UNSUPPORTED, "more than 2^27 cells completed in one step (infection keys are 32-bit)");
+    return fail(SCORE_GM_ERR_STATE, "internal error: mobility-weight cache entry lost while growing it");
 }
 
 /* Iterations of SCORE's main loop (SCORE_Kernel.cpp:6615-6747), up to wmax of
@@ -774,9 +869,6 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_STEP_MAX - 1);
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
-    /* infection keys are p*32 + rank in 32 bits: at most 2^27 completed cells per step */
-    if (slots0 >= (1u << 27))
-        return h->err = fail(SCORE_GM_ERR_UNSUPPORTED, "front larger than 2^27 slots (infection keys are 32-bit)");
     const uint32_t W = window_length(h, wmax, sp);
     /* grids: exact for one step, with room for growth over a window (the kernels
      * take the lengths from the device and cover any grid) */
@@ -854,6 +946,13 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         h->nB = last.nB;
     }
     if ((rc = grow_pcache(h))) return h->err = rc;
+    if (st.novf) {
+        /* the window ended after the step whose lookups spilled; the deferred
+         * count and the rows stay as the window left them on the device */
+        const uint32_t ndeferred = st.ndeferred;
+        if ((rc = resolve_spilled(h, st.novf, ndeferred))) return h->err = rc;
+        if (h->hst->ndeferred != ndeferred) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: deferred cells");
+    }
     if (st.ndeferred) {
         /* cells that waited for a weight: predicted under the window's scalars at
          * the step after the window, queued before the host's master section so
@@ -1103,6 +1202,11 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return fail(SCORE_GM_ERR_HIP, "hipStreamCreate");
     }
+    if ((rc = pool_acquire(device, &h->pool))) {
+        free_device(h);
+        delete h;
+        return rc;
+    }
     if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.cstep, capTot)) ||
         (rc = dalloc(h, &d.rec, capTot)) ||
         (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
@@ -1325,29 +1429,41 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+/* ompcrit_clarify_status_infectedcells into the device buffers stage_ixyz and
+ * stage_gid (nI+nB entries each), then min(cap, count) entries to the caller */
+int status_staged(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n, uint32_t *ixyz, uint32_t *gid,
+                  uint32_t *stage_ixyz, uint32_t *stage_gid) {
+    hipStream_t s = h->stream;
+    HIPCHK(gm_launch_status(h->d, h->nI, h->nB, h->step, rxthreshold, stage_ixyz, stage_gid, s));
+    HIPCHK(gm_launch_stats_out(h->d, h->hst_dev, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const uint32_t cnt = h->hst->dcount;
+    *n = cnt;
+    const uint32_t k = std::min(cap, cnt);
+    if (k && ixyz) HIPCHK(hipMemcpyAsync(ixyz, stage_ixyz, 4ull * k, hipMemcpyDeviceToHost, s));
+    if (k && gid) HIPCHK(hipMemcpyAsync(gid, stage_gid, 4ull * k, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n, uint32_t *ixyz,
                         uint32_t *gid) {
     if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
     if (h->err) return h->err;
     HIPCHK(hipSetDevice(h->device));
-    hipStream_t s = h->stream;
     const uint32_t m = h->nI + h->nB;
     uint32_t *buf = nullptr;
     int rc = dalloc(h, &buf, 2ull * std::max<uint32_t>(m, 1u));
     if (rc) return rc;
-    auto run = [&]() -> int {
-        HIPCHK(gm_launch_status(h->d, h->nI, h->nB, h->step, rxthreshold, buf, buf + m, s));
-        HIPCHK(gm_launch_stats_out(h->d, h->hst_dev, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const uint32_t cnt = h->hst->dcount;
-        *n = cnt;
-        const uint32_t k = std::min(cap, cnt);
-        if (k && ixyz) HIPCHK(hipMemcpyAsync(ixyz, buf, 4ull * k, hipMemcpyDeviceToHost, s));
-        if (k && gid) HIPCHK(hipMemcpyAsync(gid, buf + m, 4ull * k, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        return 0;
-    };
-    rc = run();
+    rc = status_staged(h, rxthreshold, cap, n, ixyz, gid, buf, buf + m);
     dfree(h, buf);
     return rc;
 }

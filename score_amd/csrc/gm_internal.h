@@ -59,6 +59,13 @@ typedef uint16_t gm_cs_t;
 #define GM_ERRF_INSIDE_FULL 0x2u
 #define GM_ERRF_PCACHE_FULL 0x4u
 #define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
+#define GM_ERRF_KEYS 0x10u        /* more than 2^27 completed cells in one step: infection keys would not fit */
+
+/* the mobility-weight cache probes at most this many slots from a key's home;
+ * a key that finds no room within them spills: its cells hold the pending
+ * marker of slot GM_OVF_SLOT and the host re-inserts them into a larger table */
+#define GM_PROBE_MAX 1024u
+#define GM_OVF_SLOT 0xFFFFFFFFu
 
 #define GM_BLOCK 256
 /* The sweep reads a list in groups of GM_SW_TPC 64-slot tiles per wave; the
@@ -132,6 +139,7 @@ struct alignas(8) GmStats {
     uint32_t cwrap;           /* a deformed-grain count was decremented past zero this step */
     uint32_t dfirst;          /* defragmentation: first INACTIVE slot of the list */
     uint32_t dcount;          /* defragmentation: ACTIVE slots moved; status vector: entries */
+    uint32_t novf;            /* weight lookups that found the cache full (their cells wait for a larger table) */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
     /* launch windows: several steps run back to back; the host reads the rows at the end */
@@ -262,5 +270,10 @@ hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
 hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys, const double *vals, uint32_t n,
                                  hipStream_t s);
+/* after a window whose weight lookups spilled (st->novf): the DEFERRED cells
+ * holding the spill marker look their pair up again in the (grown) cache,
+ * inserting pending entries; clears st->npend and st->novf first */
+hipError_t gm_launch_reinsert_spilled(const GmDev &d, const GmPParams &pp, uint32_t ndeferred, uint32_t nI, uint32_t nB,
+                                      hipStream_t s);
 
 #endif

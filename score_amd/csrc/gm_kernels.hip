@@ -127,7 +127,8 @@ static __device__ double pair_weight(const GmDev &d, const GmPParams &pp, uint32
     }
     unsigned long long key = (unsigned long long)rxg * d.ndefg + g + 1ull;
     uint32_t h = hash_slot(key, d.hcap);
-    for (uint32_t probe = 0; probe < d.hcap; probe++) {
+    const uint32_t nprobe = d.hcap < GM_PROBE_MAX ? d.hcap : GM_PROBE_MAX;
+    for (uint32_t probe = 0; probe < nprobe; probe++) {
         unsigned long long k = __atomic_load_n(&d.hkeys[h], __ATOMIC_RELAXED);
         if (k == 0ull) {
             unsigned long long old = atomicCAS(&d.hkeys[h], 0ull, key);
@@ -146,8 +147,15 @@ static __device__ double pair_weight(const GmDev &d, const GmPParams &pp, uint32
         }
         h = (h + 1u) & (d.hcap - 1u);
     }
-    atomicOr(&d.st->err, GM_ERRF_PCACHE_FULL);
-    return pending_box(0);
+    /* no room near the key's home (every key lies within nprobe slots of it, so
+     * the key is not cached): the cell waits, marked as spilled, until the host
+     * has grown the cache and its lookup is repeated */
+    atomicAdd(&d.st->novf, 1u);
+    return pending_box(GM_OVF_SLOT);
+}
+
+static __device__ __forceinline__ bool is_spilled(double v) {
+    return (unsigned long long)__double_as_longlong(v) == (GM_PENDING_TAG | (unsigned long long)GM_OVF_SLOT);
 }
 
 /* ---------------------------------------------------------------- helpers */
@@ -219,9 +227,10 @@ static __device__ __forceinline__ void rec_set_c(GmRec *r, double P, uint32_t ce
     *reinterpret_cast<uint4 *>(&r->P) = make_uint4((uint32_t)b, (uint32_t)(b >> 32), cell, defg);
 }
 
-/* a mobility weight with its pending marker replaced by the cached value */
+/* a mobility weight with its pending marker replaced by the cached value; a
+ * spilled lookup has no cache slot yet and stays marked */
 static __device__ __forceinline__ double resolved(const GmDev &d, double P) {
-    return is_pending(P) ? d.hvals[(uint32_t)__double_as_longlong(P)] : P;
+    return is_pending(P) && !is_spilled(P) ? d.hvals[(uint32_t)__double_as_longlong(P)] : P;
 }
 
 /* ------------------------------------------------------------------- sweep */
@@ -499,6 +508,8 @@ __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
     if (threadIdx.x == 0) {
         GmStats *st = d.st;
         if (tot[BC_UN]) st->err |= GM_ERRF_MISSED;
+        /* infection keys are p*32 + rank in 32 bits, p < nfull */
+        if ((uint64_t)tot[BC_FI] + tot[BC_FB] > (1ull << 27)) st->err |= GM_ERRF_KEYS;
         st->nfullI = tot[BC_FI];
         st->totRecI = tot[BC_RI];
         st->nfullB = tot[BC_FB];
@@ -1264,14 +1275,20 @@ __global__ void k_hash_insert(GmDev d, const unsigned long long *keys, const dou
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         unsigned long long key = keys[i];
         uint32_t h = hash_slot(key, d.hcap);
-        for (uint32_t probe = 0; probe < d.hcap; probe++) {
+        const uint32_t nprobe = d.hcap < GM_PROBE_MAX ? d.hcap : GM_PROBE_MAX;
+        bool stored = false;
+        for (uint32_t probe = 0; probe < nprobe; probe++) {
             unsigned long long old = atomicCAS(&d.hkeys[h], 0ull, key);
             if (old == 0ull || old == key) {
                 d.hvals[h] = vals[i];
+                stored = true;
                 break;
             }
             h = (h + 1u) & (d.hcap - 1u);
         }
+        /* lookups stop after nprobe slots, so a key stored further away would be
+         * lost; the host grows the table and inserts again */
+        if (!stored) atomicOr(&d.st->err, GM_ERRF_PCACHE_FULL);
     }
 }
 
@@ -1476,6 +1493,7 @@ __global__ void k_window_begin(GmDev d, uint32_t k, double xcells) {
     st->pone = 0;
     st->nzeroed = 0;
     st->cwrap = 0;
+    st->novf = 0;
     st->xcells = xcells;
 }
 
@@ -1504,7 +1522,8 @@ static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmSt
     bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                 any_top || xc >= w.xstop;
     halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;
-    halt = halt || slots > d.parentCap || tiles > d.tileCap || slots >= (1ull << 27);
+    halt = halt || __hip_atomic_load(&st->novf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    halt = halt || slots > d.parentCap || tiles > d.tileCap;
     st->halt = halt ? 1u : 0u;
 }
 
@@ -1578,5 +1597,37 @@ hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, uint
     if (a.nI + a.nB == 0) return hipSuccess;
     const uint32_t n = ndeferred <= d.dlistCap ? ndeferred : a.nI + a.nB;
     k_predict_deferred<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
+    return hipGetLastError();
+}
+
+/* Cells whose weight lookup spilled (the cache had no room near the key's home):
+ * with the cache grown, each looks its (nucleus, deformed grain) pair up again
+ * from its record, which inserts a pending entry the host resolves before the
+ * cell is predicted.  The cells come from the deferred list, or from a scan of
+ * every occupied slot when the list overflowed. */
+__global__ void k_spill_begin(GmDev d) {
+    d.st->npend = 0;
+    d.st->npend0 = 0;
+    d.st->novf = 0;
+}
+
+__global__ void __launch_bounds__(GM_BLOCK) k_reinsert_spilled(GmDev d, GmPParams pp, uint32_t ndeferred, uint32_t nI,
+                                                              uint32_t nB) {
+    const bool listed = ndeferred <= d.dlistCap;
+    const uint32_t n = listed ? ndeferred : nI + nB;
+    for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
+        const uint32_t s = listed ? d.dlist[i] : (i < nI ? i : d.capI + (i - nI));
+        if (d.cstep[s] != GM_CS_DEFERRED) continue;
+        GmRec *r = d.rec + s;
+        if (!is_spilled(r->P)) continue;
+        r->P = pair_weight(d, pp, r->rxg, r->defg);
+    }
+}
+
+hipError_t gm_launch_reinsert_spilled(const GmDev &d, const GmPParams &pp, uint32_t ndeferred, uint32_t nI, uint32_t nB,
+                                      hipStream_t s) {
+    k_spill_begin<<<1, 1, 0, s>>>(d);
+    const uint32_t n = ndeferred <= d.dlistCap ? ndeferred : nI + nB;
+    if (n) k_reinsert_spilled<<<grid_for(n, GM_BLOCK), GM_BLOCK, 0, s>>>(d, pp, ndeferred, nI, nB);
     return hipGetLastError();
 }

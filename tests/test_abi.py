@@ -129,7 +129,26 @@ def test_invalid_arguments_rejected_before_device_use():
     cp, keep_p = synth.ParamsSpec().to_c()
     cp.recovery_mode = 4
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    # NMAX outside the range SCORE accepts, [1, DEFAULT_NMAX] (SCORE_Kernel.cpp:274)
+    for nmax in (0, -5, 1000001):
+        cp, keep_p = synth.ParamsSpec(nmax=nmax).to_c()
+        assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     assert not h.value
+
+
+def test_oracle_nmax_range_and_loop_bound():
+    """the oracle rejects NMAX outside [1, DEFAULT_NMAX] as SCORE's parser does;
+    NMAX = 1 runs steps 0 and 1 (the termination test is step > NMAX)"""
+    from oracle.oracle import Oracle, OracleError
+    dom = synth.make_voronoi_domain(8, 8, 8, n_grains=2, n_nuclei=2, seed=1)
+    for nmax in (0, 1000001):
+        with pytest.raises(OracleError) as e:
+            Oracle(dom, synth.ParamsSpec(nmax=nmax))
+        assert e.value.code == abi.ERR_ARG
+    o = Oracle(dom, synth.ParamsSpec(nmax=1, xmax=1.0))
+    r = o.solve_rxgrowth()
+    assert r["step"] == 2 and len(o.trajectory()) == 2
+    o.close()
 
 
 @pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a GPU")

@@ -599,3 +599,36 @@ def test_hand_built_pin_cases(require_gpu):
     prm = ParamsSpec(rh_hagb_m0=M0, rh_hagb_hact_ev=H_EV, sched_temp=(T_K, T_K), maxfill=MAXFILL, xmax=1.0)
     o, g = run_both(dom, prm, stepwise=True)
     assert_final_equal(o, g)
+
+
+def _spill_case():
+    """64^3 cells in 130,000 Voronoi grains (two cells each), 5,000 nuclei, every
+    nucleus 16 deg about [001] from identity and every deformed grain within
+    2 deg of it: each pair has 0.9 < P < 1, none short-circuits to 1.0, and the
+    first infection step meets about 69,000 new (nucleus, grain) pairs, more
+    than the 65,536-entry weight cache holds."""
+    import math
+
+    def qrot(axis, deg):
+        a = np.asarray(axis, float)
+        a = a / np.linalg.norm(a)
+        h = math.radians(deg) / 2
+        return np.array([math.cos(h), *(math.sin(h) * a)])
+
+    ng, nn = 130000, 5000
+    dom = synth.make_voronoi_domain(64, 64, 64, n_grains=ng, n_nuclei=nn, seed=5)
+    rng = np.random.default_rng(9)
+    qd = [qrot(rng.normal(size=3), rng.uniform(0, 2.0)) for _ in range(ng)]
+    dom.ori_quat = np.array(qd + [qrot([0, 0, 1], 16.0)] * nn)
+    return dom
+
+
+def test_weight_cache_spill_in_one_step(require_gpu):
+    """More new pairs in one step than the weight cache holds: the lookups that
+    find no room spill, the window ends, the cache grows and the spilled cells
+    look their pairs up again before they are predicted.  The run equals the
+    oracle's."""
+    dom = _spill_case()
+    o, g = run_both(dom, ParamsSpec())
+    assert_final_equal(o, g)
+    assert g.trajectory()["npending"].max() > 32768
