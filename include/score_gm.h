@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 8
+#define SCORE_GM_ABI_VERSION 9
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -251,6 +251,22 @@ int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t 
  * number of entries and copies min(cap, *n) of them; either array may be NULL. */
 int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n,
                         uint32_t *ixyz, uint32_t *gid);
+
+/* Largest number of occupied front slots (nextSlotNeverActive of both lists
+ * together) that score_gm_get_status_noalloc accepts. */
+#define SCORE_GM_STATUS_NOALLOC_MAX (1u << 22)
+
+/* score_gm_get_status without any device memory allocated by the call.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * it).  The entries are staged in a fixed device array of the library
+ * (SCORE_GM_STATUS_NOALLOC_MAX entries per column, 32 MiB, held on every GPU the
+ * library runs on until the process ends); a handle with more occupied slots is
+ * refused with SCORE_GM_ERR_CAPACITY.  Calls on one GPU share that array and are
+ * serialised by the library; the array keeps the last call's contents, of which
+ * only the entries a call writes are returned. */
+int score_gm_get_status_noalloc(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n,
+                                uint32_t *ixyz, uint32_t *gid);
 
 /* One domain of a batch: what ensembleHdl::SIMULATE_myCAs (SCORE_Kernel.cpp:1693-1757)
  * hands to solve_RXGROWTH for each caHdl of the rank, and what it reads back. */

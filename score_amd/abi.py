@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 OK = 0
 ERR_ARG = -1
@@ -33,6 +33,8 @@ LIST_BORDER = 1
 
 CURRENTLY_INFECTED = 4096002000
 
+STATUS_NOALLOC_MAX = 1 << 22
+
 # every symbol include/score_gm.h declares
 EXPORTED = (
     "score_gm_abi_version",
@@ -47,6 +49,7 @@ EXPORTED = (
     "score_gm_get_trajectory",
     "score_gm_get_front",
     "score_gm_get_status",
+    "score_gm_get_status_noalloc",
     "score_gm_batch_run",
     "score_gm_footprint",
     "score_gm_server_create",
@@ -193,6 +196,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), _u32p, _f64p, _f64p, _u32p, _u32p]
     lib.score_gm_get_status.restype = C.c_int
     lib.score_gm_get_status.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), _u32p, _u32p]
+    lib.score_gm_get_status_noalloc.restype = C.c_int
+    lib.score_gm_get_status_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), _u32p,
+                                                _u32p]
     lib.score_gm_batch_run.restype = C.c_int
     lib.score_gm_batch_run.argtypes = [C.POINTER(BatchItem), C.c_uint32, C.c_int32, C.c_uint32]
     lib.score_gm_footprint.restype = C.c_int

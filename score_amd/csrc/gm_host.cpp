@@ -1468,6 +1468,20 @@ int score_gm_get_status(score_gm_handle *h, double rxthreshold, uint32_t cap, ui
     return rc;
 }
 
+int score_gm_get_status_noalloc(score_gm_handle *h, double rxthreshold, uint32_t cap, uint32_t *n, uint32_t *ixyz,
+                                uint32_t *gid) {
+    if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
+    if (h->err) return h->err;
+    if ((uint64_t)h->nI + h->nB > SCORE_GM_STATUS_NOALLOC_MAX)
+        return fail(SCORE_GM_ERR_CAPACITY,
+                    "front has more occupied slots than the fixed status array holds (SCORE_GM_STATUS_NOALLOC_MAX)");
+    HIPCHK(hipSetDevice(h->device));
+    uint32_t *stage = nullptr;
+    std::unique_lock<std::mutex> lock;
+    HIPCHK(gm_scratch_acquire(h->device, &stage, lock));
+    return status_staged(h, rxthreshold, cap, n, ixyz, gid, stage, stage + SCORE_GM_STATUS_NOALLOC_MAX);
+}
+
 namespace {
 
 /* one domain of a batch, create to destroy; a domain that does not fit into

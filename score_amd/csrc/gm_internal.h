@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #define GM_INFECTED 4096002000u
 #define GM_FREE_CLAIM 0xFFFFFFFFu
 #define GM_PENDING_TAG 0x7FF8000000000000ull  /* quiet NaN; low 32 bits carry the P-cache slot */
@@ -273,6 +275,11 @@ hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys,
 /* after a window whose weight lookups spilled (st->novf): the DEFERRED cells
  * holding the spill marker look their pair up again in the (grown) cache,
  * inserting pending entries; clears st->npend and st->novf first */
+/* The fixed device array of the entry point that allocates nothing
+ * (gm_noalloc.hip): its address on `device`, which must be current, looked up
+ * on the device's first use; `lock` holds the device's array for the caller
+ * until it is released. */
+hipError_t gm_scratch_acquire(int device, uint32_t **ptr, std::unique_lock<std::mutex> &lock);
 hipError_t gm_launch_reinsert_spilled(const GmDev &d, const GmPParams &pp, uint32_t ndeferred, uint32_t nI, uint32_t nB,
                                       hipStream_t s);
 
