@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 9
+#define SCORE_GM_ABI_VERSION 10
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -65,6 +65,14 @@ typedef struct score_gm_domain {
     uint32_t n_rxg;               /* nucleus pool (myrxgpool) */
     const uint32_t *rxg_ori;      /* n_rxg orientation indices (caori) */
     const uint32_t *rxg_site;     /* n_rxg global linear sites (nucsite) */
+    /* n_rxg arrival steps, or NULL for site saturation (every nucleus at step 0).
+     * Not in SCORE's OpenMP path (SCORE_Kernel.cpp:519-522): a nucleus of step
+     * k > 0 is placed at the start of step k, where the reference marks the
+     * injection point of time-dependent nucleation (:6632), in pool order, by the
+     * same rule as the site-saturated ones (infect_JuvenileNucleation,
+     * SCORE_GrowthMachine.cpp:124-189: rejected unless its cell is still deformed,
+     * appended to the Inside or Border list, rxFrac 1.0). */
+    const uint32_t *rxg_step;
 } score_gm_domain;
 
 /* Physics and integrator settings (caHdl::init_parameter, SCORE_Kernel.cpp:5747-5880). */
@@ -123,6 +131,17 @@ typedef struct score_gm_params {
     double rec_solute_ls;         /* RecoverySoluteLsPropFactor */
     double rec_solute_conc;       /* RecoverySoluteConcentration */
     double rec_alpha3, rec_kappa2, rec_c3, rec_c5;   /* RecoveryParameterAlpha3, Kappa2, C3, C5 */
+    /* recovery_mode 4, not in SCORE: rho = rho0 * exp(-t / rec_tau) for every
+     * deformed grain, applied where SCORE applies its models; no time-step rule */
+    double rec_tau;               /* s */
+    /* grain-evolution log points (log_grainevo, SCORE_OnTheFlyCharacterization.cpp:478-530,
+     * called from log_OUTPUT_Diehl, :646-651): thresholds in cells (SCORE's X log
+     * points times the box volume, :56-57); in the master section of the first
+     * step whose Xcells reaches the next threshold, before t advances, the cell
+     * count of every grain is logged (at most one log per step) */
+    uint32_t n_grainlog;
+    uint32_t reserved4;
+    const double *grainlog_at;
 } score_gm_params;
 
 /* One row per executed growth step (the values SCORE's master section holds,
@@ -143,7 +162,20 @@ typedef struct score_gm_traj_row {
     double xcells;
     double temperature;
     double rhomax;
+    double pmax;                  /* myMobilityWeightMax after the step (log_rxfrontstats localPmax) */
+    uint32_t nplaced;             /* nuclei placed at the start of the step (arrivals after step 0) */
+    uint32_t reserved;
 } score_gm_traj_row;
+
+/* One grain-evolution log entry (loginfo_grainevo_ca, SCORE_Kernel.h:642-652). */
+typedef struct score_gm_grainlog_row {
+    double t;                     /* localtime: time at the start of the logged step */
+    double X;                     /* localX after it */
+    uint32_t logcnt;              /* locallogcnt */
+    uint32_t step;                /* localtstep */
+    uint32_t sv;                  /* localSv */
+    uint32_t ngrains;             /* nlocaldata = n_defg + n_rxg */
+} score_gm_grainlog_row;
 
 typedef struct score_gm_result {
     double t;                     /* tsimend */
@@ -233,6 +265,13 @@ int score_gm_get_counts(score_gm_handle *h, uint32_t *defg_counts, uint32_t *rxg
  * number of rows recorded. */
 int score_gm_get_trajectory(const score_gm_handle *h, score_gm_traj_row *rows,
                             size_t cap, size_t *n);
+
+/* The grain-evolution log (caHdl::mygrainevolution): copies min(cap, recorded)
+ * entries in logging order and sets *n to the number recorded.  counts, unless
+ * NULL, receives for each copied entry its n_defg deformed-grain counts, then its
+ * n_rxg nucleus counts (localdatabucket). */
+int score_gm_get_grainlog(const score_gm_handle *h, score_gm_grainlog_row *rows, uint32_t *counts, size_t cap,
+                          size_t *n);
 
 /* Slot-level copy of one front list (SCORE_GM_LIST_*), slots [0, *n) where *n
  * is nextSlotNeverActive of that list (SCORE_Kernel.h:124-137 record):

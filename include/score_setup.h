@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define SCORE_SETUP_ABI_VERSION 2
+#define SCORE_SETUP_ABI_VERSION 3
 
 #define SCORE_SETUP_OK 0
 #define SCORE_SETUP_ERR_ARG -1          /* invalid argument */

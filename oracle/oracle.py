@@ -18,7 +18,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 from score_amd import abi  # noqa: E402  (struct layouts of include/score_gm.h)
-from score_amd.growth import TRAJ_DTYPE, result_dict  # noqa: E402
+from score_amd.growth import GRAINLOG_DTYPE, TRAJ_DTYPE, result_dict  # noqa: E402
 from score_amd.synth import DomainArrays, ParamsSpec  # noqa: E402
 
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -46,6 +46,8 @@ def load() -> C.CDLL:
         lib.oracle_get_trajectory.restype = C.c_size_t
         lib.oracle_get_result.argtypes = [C.c_void_p, C.POINTER(abi.Result)]
         lib.oracle_get_result.restype = C.c_int
+        lib.oracle_get_grainlog.argtypes = [C.c_void_p, C.POINTER(abi.GrainLogRow), u32p, C.c_size_t]
+        lib.oracle_get_grainlog.restype = C.c_size_t
         lib.oracle_get_dt.argtypes = [C.c_void_p]
         lib.oracle_get_dt.restype = C.c_double
         lib.oracle_get_front.argtypes = [C.c_void_p, C.c_int, C.c_uint32, u32p, u8p, u8p, u32p, f64p, f64p, u32p,
@@ -179,6 +181,15 @@ class Oracle:
         if n:
             self._lib.oracle_get_trajectory(self._h, out.ctypes.data_as(C.POINTER(abi.TrajRow)), n)
         return out
+
+    def grainlog(self):
+        n = self._lib.oracle_get_grainlog(self._h, None, None, 0)
+        rows = np.zeros(n, GRAINLOG_DTYPE)
+        counts = np.zeros((n, self.dom.n_defg + self.dom.n_rxg), np.uint32)
+        if n:
+            self._lib.oracle_get_grainlog(self._h, rows.ctypes.data_as(C.POINTER(abi.GrainLogRow)),
+                                          counts.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+        return rows, counts
 
     def front(self, which: int) -> dict:
         n = C.c_uint32(0)

@@ -133,6 +133,16 @@ typedef struct oracle {
     score_gm_traj_row *traj;
     size_t ntraj, captraj;
     uint32_t last_infected;
+    /* nuclei arriving after step 0: pool ids ordered by arrival step, and the
+     * offset of each step's first one (arr_off[k] .. arr_off[k+1]) */
+    uint32_t *arr_ids, *arr_off;
+    uint32_t arr_last;      /* last arrival step (0: site saturation only) */
+    /* grain-evolution log (mygrainevolution) */
+    double *glog_at;
+    uint32_t glog_cnt;
+    score_gm_grainlog_row *glog;
+    uint32_t *glog_counts;  /* n_defg + n_rxg per row */
+    size_t nglog, capglog;
 } oracle;
 
 /* ---------------- orientation math ---------------- */
@@ -326,6 +336,13 @@ static double o_dt_zener(const oracle *o) {
  * r = rho^-1/2 of every grain over the step just done; solute drag (:83-86)
  * changes nothing; Michalak-Paxton (:89-125) scales rho0 by a function of t */
 static void o_update_deformedsubstructure(oracle *o) {
+    if (o->p.recovery_mode == 4) {
+        /* exponential recovery (this build's own model, not in SCORE):
+         * rho = rho0 * exp(-t / tau) for every deformed grain */
+        const double factor = exp(-o->t / o->p.rec_tau);
+        for (uint32_t dg = 0; dg < o->n_defg; dg++) o->rho[dg] = o->rho0[dg] * factor;
+        return;
+    }
     if (o->p.recovery_mode == 1) {
         double B = O_PI * o->p.rec_c3 * o->p.rec_c5 * o->p.rec_vacancy_d0 *
                    exp(-1.0 * o->p.rec_vacancy_hact / (O_KBOLTZMAN * o->T));
@@ -619,7 +636,9 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
                 if (!(p->disp_time[i] > p->disp_time[i - 1])) return SCORE_GM_ERR_ARG;
         }
     }
-    if (p->recovery_mode < 0 || p->recovery_mode > 3) return SCORE_GM_ERR_ARG;
+    if (p->recovery_mode < 0 || p->recovery_mode > 4) return SCORE_GM_ERR_ARG;
+    if (p->recovery_mode == 4 && !(p->rec_tau > 0.0)) return SCORE_GM_ERR_ARG;
+    if (p->n_grainlog && !p->grainlog_at) return SCORE_GM_ERR_ARG;
     if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
     /* the range SCORE accepts for NMAX (SCORE_Kernel.cpp:274) */
     if (p->nmax < 1 || p->nmax > O_DEFAULT_NMAX) return SCORE_GM_ERR_ARG;
@@ -641,7 +660,37 @@ void oracle_destroy(oracle *o) {
     free(o->disp_t); free(o->disp_fr);
     o_list_free(&o->in); o_list_free(&o->bd);
     free(o->traj);
+    free(o->arr_ids); free(o->arr_off);
+    free(o->glog_at); free(o->glog); free(o->glog_counts);
     free(o);
+}
+
+/* infect_JuvenileNucleation (SCORE_GrowthMachine.cpp:124-189): the nucleus takes
+ * its site unless the cell is no longer deformed; its record (rxFrac 1.0,
+ * infector 26) is appended to the Inside list for a strictly interior site,
+ * else to the Border list */
+static int o_juvenile_nucleation(oracle *o, uint32_t nuc, int *placed) {
+    *placed = 0;
+    uint32_t w = o->site[nuc];
+    uint32_t z = w / o->nxy;
+    uint32_t rem = w - (o->nxy * z);
+    uint32_t y = rem / o->nx;
+    uint32_t x = rem - (o->nx * y);
+    int inside = (x > 0 && x < o->nx - 1 && y > 0 && y < o->ny - 1 && z > 0 && z < o->nz - 1);
+    uint32_t st = o->grid[w];
+    if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) return 0;
+    o->grid[w] = O_CURRENTLY_INFECTED;
+    double PP = o_pair_weight(o, nuc, st);
+    o_list *L = inside ? &o->in : &o->bd;
+    uint32_t slot = o_next_free(L, 0);
+    if (slot == O_INVALID_ADDRESS) return SCORE_GM_ERR_CAPACITY;
+    o_cell *e = &L->c[slot];
+    e->act = 1; e->inf = 26; e->x = (int16_t)x; e->y = (int16_t)y; e->z = (int16_t)z;
+    e->frac = 1.0; e->defg = st; e->rxg = nuc; e->P = PP;
+    if (PP >= o->reg_Pmax) o->reg_Pmax = PP;
+    o->nuclei++;
+    *placed = 1;
+    return 0;
 }
 
 /* solve_RXGROWTH set-up (SCORE_Kernel.cpp:6497-6612) including site-saturated
@@ -698,6 +747,27 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o->p.disp_time = o->disp_t;
     o->p.disp_fr = o->disp_fr;
     o->fr = 0.0;
+    o->glog_at = (double *)malloc(sizeof(double) * (p->n_grainlog + 1));
+    if (!o->glog_at) { oracle_destroy(o); return SCORE_GM_ERR_OOM; }
+    if (p->n_grainlog) memcpy(o->glog_at, p->grainlog_at, sizeof(double) * p->n_grainlog);
+    o->p.grainlog_at = o->glog_at;
+    /* nuclei arriving at a later step, grouped by step in pool order */
+    o->arr_last = 0;
+    if (d->rxg_step)
+        for (uint32_t i = 0; i < o->n_rxg; i++)
+            if (d->rxg_step[i] > o->arr_last) o->arr_last = d->rxg_step[i];
+    if (o->arr_last) {
+        o->arr_off = (uint32_t *)calloc((size_t)o->arr_last + 2, sizeof(uint32_t));
+        o->arr_ids = (uint32_t *)malloc(sizeof(uint32_t) * (o->n_rxg + 1));
+        if (!o->arr_off || !o->arr_ids) { oracle_destroy(o); return SCORE_GM_ERR_OOM; }
+        for (uint32_t i = 0; i < o->n_rxg; i++) o->arr_off[d->rxg_step[i] + 1]++;
+        for (uint32_t k = 0; k <= o->arr_last; k++) o->arr_off[k + 1] += o->arr_off[k];
+        uint32_t *fillp = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)o->arr_last + 1));
+        if (!fillp) { oracle_destroy(o); return SCORE_GM_ERR_OOM; }
+        memcpy(fillp, o->arr_off, sizeof(uint32_t) * ((size_t)o->arr_last + 1));
+        for (uint32_t i = 0; i < o->n_rxg; i++) o->arr_ids[fillp[d->rxg_step[i]]++] = i;
+        free(fillp);
+    }
     /* init_processing (SCORE_Kernel.cpp:5888-5901) */
     o->tprocessingend = 0.0;
     for (uint32_t i = 0; i < p->n_sched; i++)
@@ -728,26 +798,12 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
         return SCORE_GM_ERR_OOM;
     }
 
-    /* ompshar_sim_myCA_sitesaturatedNucleation + infect_JuvenileNucleation (SCORE_GrowthMachine.cpp:50-189) */
+    /* ompshar_sim_myCA_sitesaturatedNucleation + infect_JuvenileNucleation (SCORE_GrowthMachine.cpp:50-189):
+     * the nuclei that arrive at step 0 */
     for (uint32_t nuc = 0; nuc < o->n_rxg; nuc++) {
-        uint32_t w = o->site[nuc];
-        uint32_t z = w / o->nxy;
-        uint32_t rem = w - (o->nxy * z);
-        uint32_t y = rem / o->nx;
-        uint32_t x = rem - (o->nx * y);
-        int inside = (x > 0 && x < o->nx - 1 && y > 0 && y < o->ny - 1 && z > 0 && z < o->nz - 1);
-        uint32_t st = o->grid[w];
-        if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) continue;
-        o->grid[w] = O_CURRENTLY_INFECTED;
-        double PP = o_pair_weight(o, nuc, st);
-        o_list *L = inside ? &o->in : &o->bd;
-        uint32_t slot = o_next_free(L, 0);
-        if (slot == O_INVALID_ADDRESS) { oracle_destroy(o); return SCORE_GM_ERR_CAPACITY; }
-        o_cell *e = &L->c[slot];
-        e->act = 1; e->inf = 26; e->x = (int16_t)x; e->y = (int16_t)y; e->z = (int16_t)z;
-        e->frac = 1.0; e->defg = st; e->rxg = nuc; e->P = PP;
-        if (PP >= o->reg_Pmax) o->reg_Pmax = PP;
-        o->nuclei++;
+        if (d->rxg_step && d->rxg_step[nuc] != 0) continue;
+        int placed = 0;
+        if ((rc = o_juvenile_nucleation(o, nuc, &placed))) { oracle_destroy(o); return rc; }
     }
     /* master reduction of myMobilityWeightMax (SCORE_Kernel.cpp:6598-6605) */
     if (o->reg_Pmax >= o->Pmax) o->Pmax = o->reg_Pmax;
@@ -764,6 +820,37 @@ static int o_push_traj(oracle *o, const score_gm_traj_row *r) {
         o->captraj = nc;
     }
     o->traj[o->ntraj++] = *r;
+    return 0;
+}
+
+/* log_grainevo (SCORE_OnTheFlyCharacterization.cpp:478-530): time (before t
+ * advances), X, the log counter, step and Sv, and the cell count of every grain,
+ * deformed grains first */
+static int o_log_grainevo(oracle *o, uint32_t sv) {
+    const size_t ng = (size_t)o->n_defg + o->n_rxg;
+    if (o->nglog == o->capglog) {
+        size_t nc = o->capglog ? 2 * o->capglog : 16;
+        score_gm_grainlog_row *nr = (score_gm_grainlog_row *)realloc(o->glog, nc * sizeof(*nr));
+        if (!nr) return SCORE_GM_ERR_OOM;
+        o->glog = nr;
+        uint32_t *nc2 = (uint32_t *)realloc(o->glog_counts, nc * (ng ? ng : 1) * sizeof(uint32_t));
+        if (!nc2) return SCORE_GM_ERR_OOM;
+        o->glog_counts = nc2;
+        o->capglog = nc;
+    }
+    score_gm_grainlog_row *r = &o->glog[o->nglog];
+    memset(r, 0, sizeof(*r));
+    r->t = o->t;
+    r->X = o->X;
+    r->logcnt = o->glog_cnt;
+    r->step = o->step;
+    r->sv = sv;
+    r->ngrains = (uint32_t)ng;
+    uint32_t *c = o->glog_counts + o->nglog * ng;
+    memcpy(c, o->defcnt, sizeof(uint32_t) * o->n_defg);
+    memcpy(c + o->n_defg, o->rxcnt, sizeof(uint32_t) * o->n_rxg);
+    o->nglog++;
+    o->glog_cnt++;
     return 0;
 }
 
@@ -786,6 +873,18 @@ int oracle_step(oracle *o) {
         o->defrag_cnt++;
     }
     int rc;
+    /* nuclei arriving at this step, in pool order, at the point where the
+     * reference marks the injection of time-dependent nucleation: after the
+     * defragmentation, before the growth step (SCORE_Kernel.cpp:6632; the sweep
+     * must see them, :4068) */
+    uint32_t nplaced = 0;
+    if (o->step >= 1 && o->step <= o->arr_last) {
+        for (uint32_t i = o->arr_off[o->step]; i < o->arr_off[o->step + 1]; i++) {
+            int placed = 0;
+            if ((rc = o_juvenile_nucleation(o, o->arr_ids[i], &placed))) { o->err = rc; return rc; }
+            nplaced += (uint32_t)placed;
+        }
+    }
     if ((rc = o_sweep(o, &o->in)) || (rc = o_sweep(o, &o->bd))) { o->err = rc; return rc; }
     o->last_infected = 0;
     if ((rc = o_update_fullrx(o, &o->in, 0)) || (rc = o_update_fullrx(o, &o->bd, 1))) { o->err = rc; return rc; }
@@ -813,6 +912,13 @@ int oracle_step(oracle *o) {
     r.X = o->X;
     r.xcells = o->Xcells;
     r.rhomax = o->rhomax;
+    r.pmax = o->Pmax;
+    r.nplaced = nplaced;
+    /* log_OUTPUT_Diehl (SCORE_OnTheFlyCharacterization.cpp:646-651): at most one
+     * grain-evolution log per step, once Xcells reaches the next threshold */
+    if (o->glog_cnt < o->p.n_grainlog && o->Xcells >= o->glog_at[o->glog_cnt]) {
+        if ((rc = o_log_grainevo(o, sv))) { o->err = rc; return rc; }
+    }
     o->t += o->dt;
     o_update_temperature(o);
     o_update_atomistic(o);
@@ -864,6 +970,14 @@ int oracle_get_result(const oracle *o, score_gm_result *r) {
 }
 
 double oracle_get_dt(const oracle *o) { return o->dt; }
+
+size_t oracle_get_grainlog(const oracle *o, score_gm_grainlog_row *rows, uint32_t *counts, size_t cap) {
+    const size_t n = o->nglog < cap ? o->nglog : cap;
+    const size_t ng = (size_t)o->n_defg + o->n_rxg;
+    if (rows && n) memcpy(rows, o->glog, n * sizeof(*rows));
+    if (counts && n) memcpy(counts, o->glog_counts, n * ng * sizeof(uint32_t));
+    return o->nglog;
+}
 
 int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8_t *act, uint8_t *inf, uint32_t *cell,
                      double *frac, double *P, uint32_t *defg, uint32_t *rxg) {

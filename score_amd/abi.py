@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 OK = 0
 ERR_ARG = -1
@@ -47,6 +47,7 @@ EXPORTED = (
     "score_gm_get_grainmap",
     "score_gm_get_counts",
     "score_gm_get_trajectory",
+    "score_gm_get_grainlog",
     "score_gm_get_front",
     "score_gm_get_status",
     "score_gm_get_status_noalloc",
@@ -78,6 +79,7 @@ class Domain(C.Structure):
         ("n_rxg", C.c_uint32),
         ("rxg_ori", _u32p),
         ("rxg_site", _u32p),
+        ("rxg_step", _u32p),
     ]
 
 
@@ -105,6 +107,9 @@ class Params(C.Structure):
         ("rec_solute_d0", C.c_double), ("rec_solute_hact", C.c_double),
         ("rec_solute_ls", C.c_double), ("rec_solute_conc", C.c_double),
         ("rec_alpha3", C.c_double), ("rec_kappa2", C.c_double), ("rec_c3", C.c_double), ("rec_c5", C.c_double),
+        ("rec_tau", C.c_double),
+        ("n_grainlog", C.c_uint32), ("reserved4", C.c_uint32),
+        ("grainlog_at", _f64p),
     ]
 
 
@@ -116,6 +121,14 @@ class TrajRow(C.Structure):
         ("ninfected", C.c_uint32), ("npending", C.c_uint32),
         ("t", C.c_double), ("dt", C.c_double), ("X", C.c_double),
         ("xcells", C.c_double), ("temperature", C.c_double), ("rhomax", C.c_double),
+        ("pmax", C.c_double), ("nplaced", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class GrainLogRow(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("X", C.c_double),
+        ("logcnt", C.c_uint32), ("step", C.c_uint32), ("sv", C.c_uint32), ("ngrains", C.c_uint32),
     ]
 
 
@@ -191,6 +204,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.score_gm_get_counts.argtypes = [C.c_void_p, _u32p, _u32p]
     lib.score_gm_get_trajectory.restype = C.c_int
     lib.score_gm_get_trajectory.argtypes = [C.c_void_p, C.POINTER(TrajRow), C.c_size_t, C.POINTER(C.c_size_t)]
+    lib.score_gm_get_grainlog.restype = C.c_int
+    lib.score_gm_get_grainlog.argtypes = [C.c_void_p, C.POINTER(GrainLogRow), _u32p, C.c_size_t,
+                                          C.POINTER(C.c_size_t)]
     lib.score_gm_get_front.restype = C.c_int
     lib.score_gm_get_front.argtypes = [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), _u32p, _f64p, _f64p, _u32p, _u32p]

@@ -86,6 +86,28 @@ def build_setup(out: str | None = None) -> str:
     return out
 
 
+def build_caller(out: str | None = None) -> str | None:
+    """tests/abi_caller/score_gm_caller: a C++ program compiled against the C
+    headers with g++ -std=c++11 (SCORE's language level) and linked to
+    libscore_gm.so and libscore_io.so, as SCORE's adapter would be (the ABI
+    tests run it)."""
+    out = out or os.path.join(ROOT, "tests", "abi_caller", "score_gm_caller")
+    lib_io = os.path.join(HERE, "libscore_io.so")
+    if not os.path.exists(lib_io):
+        print("score_gm_caller: libscore_io.so not built; caller not built", flush=True)
+        return None
+    cxx = os.environ.get("CXX", "g++")
+    tmp = out + ".tmp"
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    _run([cxx, "-std=c++11", "-O2", "-Wall", "-Wextra", "-I", INCLUDE,
+          os.path.join(ROOT, "tests", "abi_caller", "caller.cpp"), "-o", tmp,
+          "-L", HERE, "-lscore_gm", "-lscore_io",
+          "-Wl,-rpath,$ORIGIN/../../score_amd",
+          "-Wl,-rpath-link," + rocm_lib, "-Wl,-rpath-link," + os.path.join(HDF5_PREFIX, "lib")])
+    os.replace(tmp, out)
+    return out
+
+
 REFERENCE = "/root/reference"
 
 
@@ -105,6 +127,7 @@ def build_all(oracle: bool = True) -> None:
     build_gm()
     build_io()
     build_setup()
+    build_caller()
     if oracle:
         build_oracle()
 
@@ -112,7 +135,7 @@ def build_all(oracle: bool = True) -> None:
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true", help="also build the oracle (test infrastructure)")
-    ap.add_argument("--only", choices=["gm", "synth", "io", "setup", "oracle"], default=None)
+    ap.add_argument("--only", choices=["gm", "synth", "io", "setup", "caller", "oracle"], default=None)
     a = ap.parse_args(argv)
     if a.only == "gm":
         build_gm()
@@ -122,6 +145,8 @@ def main(argv: list[str] | None = None) -> int:
         build_io()
     elif a.only == "setup":
         build_setup()
+    elif a.only == "caller":
+        build_caller()
     elif a.only == "oracle":
         build_oracle()
     else:

@@ -192,6 +192,16 @@ struct score_gm_handle {
     std::vector<uint8_t> alive;
     size_t rho_top = 0;
     std::vector<uint32_t> zbuf;
+    /* weights of the window's new pairs, in insertion order (steps in order; a
+     * spilled step's re-inserted pairs last) */
+    std::vector<double> pend_P;
+    /* nuclei arriving after step 0: offsets per step (arr_off[k]..arr_off[k+1]) */
+    std::vector<uint32_t> arr_off;
+    /* grain-evolution log (mygrainevolution) */
+    std::vector<double> glog_at;
+    uint32_t glog_cnt = 0;
+    std::vector<score_gm_grainlog_row> glog;
+    std::vector<uint32_t> glog_counts;
 };
 
 namespace {
@@ -302,7 +312,7 @@ void free_device(score_gm_handle *h) {
     void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
-                    d.scan_bsum, d.st};
+                    d.scan_bsum, d.st, d.arr_ids, d.arr_off, d.nsite, d.nucP};
     for (void *q : ptrs) dfree(h, q);
     memset(&h->d, 0, sizeof(h->d));
     if (h->hst) (void)hipHostFree(h->hst);
@@ -347,7 +357,10 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
                     return fail(SCORE_GM_ERR_ARG, "drag table times must increase strictly");
         }
     }
-    if (prm->recovery_mode < 0 || prm->recovery_mode > 3) return fail(SCORE_GM_ERR_ARG, "unknown recovery model");
+    if (prm->recovery_mode < 0 || prm->recovery_mode > 4) return fail(SCORE_GM_ERR_ARG, "unknown recovery model");
+    if (prm->recovery_mode == 4 && !(prm->rec_tau > 0.0))
+        return fail(SCORE_GM_ERR_ARG, "exponential recovery needs a positive time constant");
+    if (prm->n_grainlog != 0 && !prm->grainlog_at) return fail(SCORE_GM_ERR_ARG, "grain-evolution log points missing");
     if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
         return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");
@@ -470,6 +483,13 @@ double dt_zener(const score_gm_handle *h) {
  * scales rho0 (nes_michalakpaxton, :89-125) */
 void update_deformedsubstructure(score_gm_handle *h) {
     const int mode = h->p.recovery_mode;
+    if (mode == 4) {
+        /* exponential recovery, not a SCORE model: rho = rho0 * exp(-t / tau) */
+        const double factor = exp(-h->t / h->p.rec_tau);
+        for (size_t g = 0; g < h->rho_host.size(); g++) h->rho_host[g] = h->rho0_host[g] * factor;
+        h->rho_changed = true;
+        return;
+    }
     if (mode == 1) {
         double B = score_orient::kPi * h->p.rec_c3 * h->p.rec_c5 * h->p.rec_vacancy_d0 *
                    exp(-1.0 * h->p.rec_vacancy_hact / (kBoltzmann * h->T));
@@ -576,10 +596,17 @@ uint32_t pow2_at_least(uint64_t v) {
     return (uint32_t)std::min<uint64_t>(c, 1ull << 31);
 }
 
-/* per-step lists, tile masks and per-parent masks sized for the current occupancy */
+/* nuclei arriving at the start of step k (after step 0) */
+uint32_t arrivals_at(const score_gm_handle *h, uint32_t k) {
+    return (k >= 1 && (size_t)k + 1 < h->arr_off.size()) ? h->arr_off[k + 1] - h->arr_off[k] : 0u;
+}
+
+/* per-step lists, tile masks and per-parent masks sized for the current
+ * occupancy and the next step's arrivals (to either list) */
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
-    const uint32_t tiles = gm_tiles_inside(h->nI) + (h->nB + 63) / 64 + GM_SW_TPC;
+    const uint32_t na = arrivals_at(h, h->step);
+    const uint32_t tiles = gm_tiles_inside(h->nI + na) + (h->nB + na + 63) / 64 + GM_SW_TPC;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
         dfree(h, d.mF); dfree(h, d.mR);
@@ -588,7 +615,7 @@ int ensure_step_capacity(score_gm_handle *h) {
         if ((rc = dalloc(h, &d.mF, nc)) || (rc = dalloc(h, &d.mR, nc))) return rc;
         d.tileCap = nc;
     }
-    const uint32_t parents = h->nI + h->nB;
+    const uint32_t parents = h->nI + h->nB + na;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
         dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par);
@@ -634,7 +661,7 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
         const uint32_t rxg = (uint32_t)(key / d.ndefg), g = (uint32_t)(key % d.ndefg);
         const double P = mobility_weight(h->p, &h->rxq[4 * (size_t)rxg], &h->defq[4 * (size_t)g]);
         h->hpend_vals[i] = P;
-        if (P >= h->regPmax) h->regPmax = P;
+        h->pend_P.push_back(P); /* folded into myMobilityWeightMax at the step that met the pair */
     }
     /* stream-ordered: the kernel reads the values from the mapped buffer and
      * stores them into the cache before the next step's kernels read it; the
@@ -784,7 +811,7 @@ void advance_scalars(score_gm_handle *h) {
 uint32_t window_length(score_gm_handle *h, uint32_t wmax, const GmSweepParams &sp) {
     if (h->prof || wmax <= 1 || h->p.mobility_model != SCORE_GM_MOB_ROLLETTHOLM) return 1;
     /* recovery that changes rho re-anchors every step */
-    if (h->p.recovery_mode == 1 || h->p.recovery_mode == 3) return 1;
+    if (h->p.recovery_mode == 1 || h->p.recovery_mode == 3 || h->p.recovery_mode == 4) return 1;
     /* a new cell starts at most at 0.0682 times the largest increment and takes
      * at least imin additions to reach 1.0; the increment is at most maxfill
      * because the time step is bounded by maxfill*dx/vmax and P <= 1 */
@@ -820,6 +847,7 @@ double window_xstop(const score_gm_handle *h) {
     while (!((c / N) > h->p.xmax) && c <= N) c += 1.0;
     double x = std::min(c, N);
     if (h->defrag_cnt < h->defrag_at.size()) x = std::min(x, h->defrag_at[h->defrag_cnt]);
+    if (h->glog_cnt < h->glog_at.size()) x = std::min(x, h->glog_at[h->glog_cnt]);
     return x;
 }
 
@@ -833,6 +861,33 @@ int step_error(uint32_t err) {
     return fail(SCORE_GM_ERR_STATE, "internal error: mobility-weight cache entry lost while growing it");
 }
 
+/* log_grainevo (SCORE_OnTheFlyCharacterization.cpp:478-530): time before t
+ * advances, X, the log counter, step, Sv and the cell count of every grain
+ * (deformed grains first), read from the device after the step */
+int log_grainevo(score_gm_handle *h, uint32_t sv) {
+    GmDev &d = h->d;
+    const size_t ng = (size_t)d.ndefg + d.nrxg;
+    const size_t off = h->glog_counts.size();
+    h->glog_counts.resize(off + ng);
+    HIPCHK(hipMemcpyAsync(h->glog_counts.data() + off, d.defcnt, sizeof(uint32_t) * d.ndefg, hipMemcpyDeviceToHost,
+                          h->stream));
+    if (d.nrxg)
+        HIPCHK(hipMemcpyAsync(h->glog_counts.data() + off + d.ndefg, d.rxcnt, sizeof(uint32_t) * d.nrxg,
+                              hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    score_gm_grainlog_row g;
+    memset(&g, 0, sizeof(g));
+    g.t = h->t;
+    g.X = h->X;
+    g.logcnt = h->glog_cnt;
+    g.step = h->step;
+    g.sv = sv;
+    g.ngrains = (uint32_t)ng;
+    h->glog.push_back(g);
+    h->glog_cnt++;
+    return 0;
+}
+
 /* Iterations of SCORE's main loop (SCORE_Kernel.cpp:6615-6747), up to wmax of
  * them launched back to back as one window.  Returns 1 when the termination
  * test fired before any step, 0 after *nsteps >= 1 steps, < 0 on error. */
@@ -843,6 +898,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     if (done(h)) { h->completed = 1; return 1; }
     GmDev &d = h->d;
     int rc;
+    h->pend_P.clear();
     /* on-the-fly defragmentation at the Xcells thresholds (SCORE_Kernel.cpp:6625-6630) */
     if (h->defrag_cnt < h->defrag_at.size() && h->Xcells >= h->defrag_at[h->defrag_cnt]) {
         if ((rc = defragment_list(h, 0u, d.capI, h->nI)) || (rc = defragment_list(h, d.capI, d.capB, h->nB)))
@@ -920,7 +976,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         float ms;
         if ((e = hipEventElapsedTime(&ms, h->ev[0], h->ev[GM_NK])) != hipSuccess)
             return h->err = hip_fail(e, "hipEventElapsedTime");
-        const uint32_t sv0 = slots0 - (r0.totRecI + r0.totRecB);
+        const uint32_t sv0 = slots0 + r0.nplaced - (r0.totRecI + r0.totRecB);
         pf.ms_step += ms;
         pf.steps++;
         pf.sum_sv += sv0;
@@ -939,7 +995,6 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     }
     /* the window's new pairs, evaluated with glibc and stored in the device cache */
     if ((rc = resolve_pending(h, st))) return h->err = rc;
-    if (st.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
     {
         const GmStepRow &last = st.rows[nexec - 1];
         h->nI = last.nI;
@@ -967,10 +1022,12 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     }
     /* master section of every step (SCORE_Kernel.cpp:6688-6742) */
     uint32_t prev_slots = slots0;
+    size_t pend_off = 0;
     for (uint32_t j = 0; j < nexec; j++) {
         const GmStepRow &row = st.rows[j];
         const uint32_t nfull = row.nfullI + row.nfullB;
-        const uint32_t sv = prev_slots - (row.totRecI + row.totRecB);
+        /* the sweep saw the slots occupied before the step and the nuclei placed at its start */
+        const uint32_t sv = prev_slots + row.nplaced - (row.totRecI + row.totRecB);
         prev_slots = row.nI + row.nB;
         score_gm_traj_row r;
         memset(&r, 0, sizeof(r));
@@ -978,6 +1035,19 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         r.t = h->t;
         r.dt = h->dt;
         r.temperature = h->T;
+        /* reg_myMobilityWeightMax of the step: the nuclei placed at its start, the
+         * new pairs its infections met (short-circuited to 1.0 or resolved with
+         * glibc; the pairs a spilled step met are resolved last) */
+        h->nuclei += row.nplaced;
+        if (row.pkey) {
+            const double Pn = gm_pkey_value(row.pkey);
+            if (Pn >= h->regPmax) h->regPmax = Pn;
+        }
+        if (row.pone && 1.0 >= h->regPmax) h->regPmax = 1.0;
+        const size_t pend_end = j + 1 == nexec ? h->pend_P.size() : std::min(h->pend_P.size(), pend_off + row.npend);
+        for (size_t i = pend_off; i < pend_end; i++)
+            if (h->pend_P[i] >= h->regPmax) h->regPmax = h->pend_P[i];
+        pend_off = pend_end;
         double mymobmax = kDefaultPmax;
         if (h->regPmax >= mymobmax) mymobmax = h->regPmax;
         h->Pmax = mymobmax;
@@ -1027,6 +1097,15 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         r.X = h->X;
         r.xcells = h->Xcells;
         r.rhomax = h->rhomax;
+        r.pmax = h->Pmax;
+        r.nplaced = row.nplaced;
+        /* log_OUTPUT_Diehl (SCORE_OnTheFlyCharacterization.cpp:646-651): the window
+         * ends at the step whose Xcells reaches the next log point, so the device's
+         * counts are this step's */
+        if (h->glog_cnt < h->glog_at.size() && h->Xcells >= h->glog_at[h->glog_cnt]) {
+            if (j + 1 != nexec) return h->err = fail(SCORE_GM_ERR_STATE, "internal error: grain log inside a window");
+            if ((rc = log_grainevo(h, sv))) return h->err = rc;
+        }
         advance_scalars(h);
         h->step++;
         h->traj.push_back(r);
@@ -1124,6 +1203,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     }
     h->p.disp_time = h->disp_t.data();
     h->p.disp_fr = h->disp_fr.data();
+    if (prm->n_grainlog) h->glog_at.assign(prm->grainlog_at, prm->grainlog_at + prm->n_grainlog);
+    h->p.grainlog_at = h->glog_at.data();
     h->cellsize = dom->cellsize;
     h->inv_dx = (1.0 / dom->cellsize);
     /* init_processing (SCORE_Kernel.cpp:5888-5901) */
@@ -1155,20 +1236,43 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     std::vector<uint32_t> defcnt(d.ndefg, 0u);
     for (uint64_t i = 0; i < d.N; i++) defcnt[dom->cellgrid[i]]++;
 
-    /* nucleus weights with glibc (calc_mobilityweight, SCORE_GrowthMachine.cpp:152) and the
-     * placed set: a site already taken by an earlier nucleus is rejected (:130-138) */
+    /* nucleus weights with glibc (calc_mobilityweight, SCORE_GrowthMachine.cpp:152); a
+     * nucleus keeps the deformed grain of its site until it arrives, since a cell
+     * changes its grain only by transforming, which rejects the nucleus.  The
+     * site-saturated set (step 0): a site already taken by an earlier nucleus is
+     * rejected (:130-138).  Later arrivals are grouped by step in pool order. */
     std::vector<double> nucP(d.nrxg + 1);
+    std::vector<uint32_t> site0(d.nrxg + 1, GM_NO_SITE);
+    std::vector<uint32_t> arr_ids;
     {
         std::vector<uint64_t> seen((d.N + 63) / 64, 0ull);
+        uint32_t last = 0;
         for (uint32_t k = 0; k < d.nrxg; k++) {
             const uint32_t w = dom->rxg_site[k];
             const uint32_t g = dom->cellgrid[w];
             nucP[k] = mobility_weight(h->p, &h->rxq[4 * (size_t)k], &h->defq[4 * (size_t)g]);
+            const uint32_t ks = dom->rxg_step ? dom->rxg_step[k] : 0u;
+            if (ks > last) last = ks;
+            if (ks != 0) continue;
+            site0[k] = w;
             if (!(seen[w >> 6] >> (w & 63) & 1ull)) {
                 seen[w >> 6] |= 1ull << (w & 63);
                 h->nuclei++;
                 if (nucP[k] >= h->regPmax) h->regPmax = nucP[k];
             }
+        }
+        if (last) {
+            if (last >= GM_STEP_MAX) {
+                delete h;
+                return fail(SCORE_GM_ERR_ARG, "nucleus arrival step out of range");
+            }
+            h->arr_off.assign((size_t)last + 2, 0u);
+            for (uint32_t k = 0; k < d.nrxg; k++) h->arr_off[dom->rxg_step[k] + 1]++;
+            for (uint32_t k = 0; k <= last; k++) h->arr_off[k + 1] += h->arr_off[k];
+            std::vector<uint32_t> fillp(h->arr_off.begin(), h->arr_off.end() - 1);
+            arr_ids.resize(d.nrxg);
+            for (uint32_t k = 0; k < d.nrxg; k++) arr_ids[fillp[dom->rxg_step[k]]++] = k;
+            d.arr_last = last;
         }
     }
 
@@ -1226,6 +1330,13 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return rc;
     }
+    if (d.arr_last &&
+        ((rc = dalloc(h, &d.arr_ids, d.nrxg)) || (rc = dalloc(h, &d.arr_off, h->arr_off.size())) ||
+         (rc = dalloc(h, &d.nsite, d.nrxg)) || (rc = dalloc(h, &d.nucP, d.nrxg)))) {
+        free_device(h);
+        delete h;
+        return rc;
+    }
     if ((rc = pinned_mapped(h, (void **)&h->hst, (void **)&h->hst_dev, sizeof(GmStats)))) {
         free_device(h);
         delete h;
@@ -1252,8 +1363,15 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         int r2;
         if ((r2 = dalloc(h, &d_site, (size_t)d.nrxg + 1)) || (r2 = dalloc(h, &d_nucP, (size_t)d.nrxg + 1))) return r2;
         if (d.nrxg) {
-            HIPCHK(hipMemcpyAsync(d_site, dom->rxg_site, sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d_site, site0.data(), sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(d_nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
+        }
+        if (d.arr_last) {
+            HIPCHK(hipMemcpyAsync(d.arr_ids, arr_ids.data(), sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d.arr_off, h->arr_off.data(), sizeof(uint32_t) * h->arr_off.size(),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d.nsite, dom->rxg_site, sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d.nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
         }
         HIPCHK(gm_launch_nucleate(d, d_site, d_nucP, d.nrxg, s));
         HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
@@ -1375,6 +1493,17 @@ int score_gm_get_counts(score_gm_handle *h, uint32_t *defg_counts, uint32_t *rxg
     if (rxg_counts && h->d.nrxg)
         HIPCHK(hipMemcpyAsync(rxg_counts, h->d.rxcnt, sizeof(uint32_t) * h->d.nrxg, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int score_gm_get_grainlog(const score_gm_handle *h, score_gm_grainlog_row *rows, uint32_t *counts, size_t cap,
+                          size_t *n) {
+    if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
+    *n = h->glog.size();
+    const size_t m = std::min(cap, h->glog.size());
+    const size_t ng = (size_t)h->d.ndefg + h->d.nrxg;
+    if (rows && m) memcpy(rows, h->glog.data(), m * sizeof(*rows));
+    if (counts && m) memcpy(counts, h->glog_counts.data(), m * ng * sizeof(uint32_t));
     return 0;
 }
 
@@ -1588,6 +1717,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     b += 4 * (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt + (double)capB * prm->maxfill * kFullRecyclingAtt);
     b += 4 * 8 * GM_MAX_BLOCKS;
     b += 52ull * dom->n_defg + 52ull * (dom->n_rxg + 1);
+    if (dom->rxg_step) b += 20ull * dom->n_rxg;   /* arrivals: ids, sites, weights */
     b += 28 * hcap;                               /* weight cache: keys, values, pending slots and keys */
     b += sizeof(GmStats) + 8 * GM_SCAN_BLOCKS;
     *bytes = b;

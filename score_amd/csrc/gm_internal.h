@@ -67,6 +67,21 @@ typedef uint16_t gm_cs_t;
  * a key that finds no room within them spills: its cells hold the pending
  * marker of slot GM_OVF_SLOT and the host re-inserts them into a larger table */
 #define GM_PROBE_MAX 1024u
+#define GM_NO_SITE 0xFFFFFFFFu    /* a nucleus left out of the site-saturated placement */
+
+/* order-preserving unsigned key of a double (larger value, larger key; never 0
+ * for a finite value), for atomicMax over weights */
+static inline __host__ __device__ unsigned long long gm_pkey(double v) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &v, sizeof(b));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+static inline __host__ __device__ double gm_pkey_value(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double v;
+    __builtin_memcpy(&v, &b, sizeof(v));
+    return v;
+}
 #define GM_OVF_SLOT 0xFFFFFFFFu
 
 #define GM_BLOCK 256
@@ -120,7 +135,9 @@ struct GmStepRow {
     uint32_t nattempt;
     uint32_t nzeroed;         /* exhausted grains so far in the window */
     uint32_t err;
-    uint32_t pad[2];
+    uint32_t nplaced;         /* nuclei placed at the start of the step */
+    uint32_t pone;            /* a new pair's weight was short-circuited to 1.0 in the step */
+    unsigned long long pkey;  /* largest weight of the nuclei placed in the step (gm_pkey order), 0: none */
 };
 
 struct alignas(8) GmStats {
@@ -142,6 +159,8 @@ struct alignas(8) GmStats {
     uint32_t dfirst;          /* defragmentation: first INACTIVE slot of the list */
     uint32_t dcount;          /* defragmentation: ACTIVE slots moved; status vector: entries */
     uint32_t novf;            /* weight lookups that found the cache full (their cells wait for a larger table) */
+    uint32_t nplaced;         /* nuclei placed at the start of the current step */
+    unsigned long long arr_pkey; /* largest weight of those nuclei (gm_pkey order) */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
     /* launch windows: several steps run back to back; the host reads the rows at the end */
@@ -198,6 +217,13 @@ struct GmDev {
     uint32_t hcap;
     uint32_t *pend_slot;
     unsigned long long *pend_key;
+    /* nuclei arriving after step 0: pool ids grouped by step (arr_off[k] ..
+     * arr_off[k+1] for step k, 1 <= k <= arr_last; arr_last 0: none), and every
+     * nucleus' site and host-computed weight */
+    uint32_t *arr_ids, *arr_off;
+    uint32_t arr_last;
+    uint32_t *nsite;
+    double *nucP;
     /* nucleation scratch */
     uint32_t *nflag, *nI_, *nB_;
     uint32_t *scan_bsum;        /* 2*GM_SCAN_BLOCKS */
@@ -242,6 +268,9 @@ hipError_t gm_launch_step_tail(const GmDev &d, const GmWindow &w, hipStream_t s)
  * step a.k under the window's scalars and predicted */
 hipError_t gm_launch_predict_deferred(const GmDev &d, const GmSweepArgs &a, uint32_t ndeferred, hipStream_t s);
 hipError_t gm_launch_nucleate(const GmDev &d, const uint32_t *site, const double *nucP, uint32_t nrxg, hipStream_t s);
+/* one block per step, between the re-anchoring and the sweep: the nuclei arriving
+ * at the step the device is at, placed in pool order and predicted under a's scalars */
+hipError_t gm_launch_arrivals(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
 hipError_t gm_launch_resolve_pending(const GmDev &d, const double *vals, uint32_t npend, hipStream_t s);
 hipError_t gm_launch_stats_out(const GmDev &d, GmStats *out, hipStream_t s);
 /* SCORE's view of slots [base, base+n) before step k, unpacked into per-field arrays */

@@ -621,6 +621,7 @@ hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, 
     A.reanchor = 0;
     if ((e = mark(ev, 0, s)) != hipSuccess) return e;
     if (a.reanchor) k_reanchor<<<grid_for(a.nI + a.nB, GM_BLOCK), GM_BLOCK, 0, s>>>(d, a);
+    if ((e = gm_launch_arrivals(d, A, s)) != hipSuccess) return e;
     const uint32_t nb = sweep_blocks(a.nI, a.nB);
     if ((e = mark(ev, 1, s)) != hipSuccess) return e;
     k_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A);
@@ -1131,7 +1132,7 @@ static __device__ __forceinline__ bool site_deformed(const GmDev &d, uint32_t w)
 __global__ void k_nuc_claim(GmDev d, const uint32_t *site, uint32_t n) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         uint32_t w = site[k];
-        if (site_deformed(d, w)) atomicMin(&d.claim[w], k);
+        if (w != GM_NO_SITE && site_deformed(d, w)) atomicMin(&d.claim[w], k);
     }
 }
 
@@ -1143,8 +1144,8 @@ static __device__ __forceinline__ bool site_interior(const GmDev &d, uint32_t w)
 __global__ void k_nuc_flag(GmDev d, const uint32_t *site, uint32_t n) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         uint32_t w = site[k];
-        bool win = site_deformed(d, w) && d.claim[w] == k;
-        bool in = site_interior(d, w);
+        bool win = w != GM_NO_SITE && site_deformed(d, w) && d.claim[w] == k;
+        bool in = win && site_interior(d, w);
         d.nI_[k] = (win && in) ? 1u : 0u;
         d.nB_[k] = (win && !in) ? 1u : 0u;
         d.nflag[k] = win ? 1u : 0u;
@@ -1477,6 +1478,106 @@ hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k
     return hipGetLastError();
 }
 
+/* ------------------------------------------------------------- arrivals
+ * Nuclei that arrive after step 0 are placed at the start of their step, before
+ * the sweep, as infect_JuvenileNucleation places the site-saturated ones
+ * (SCORE_GrowthMachine.cpp:124-189): in pool order, a nucleus whose cell is no
+ * longer deformed (taken by the front or by an earlier nucleus) is rejected;
+ * the record (rxFrac 1.0, infector 26, the host's weight) is appended to the
+ * Inside list for a strictly interior cell, else to the Border list.  The
+ * reference marks this injection point but has no model behind it
+ * (SCORE_Kernel.cpp:6632).  One block: every nucleus of the step claims its
+ * cell (the lowest pool id wins), then the winners are ranked in pool order,
+ * 1024 at a time, and each record is predicted under the step's scalars. */
+__global__ void __launch_bounds__(1024) k_arrivals(GmDev d, GmSweepArgs A0) {
+    __shared__ uint32_t ws[16][2];
+    GmStats *st = d.st;
+    if (st->halt) return;
+    const uint32_t k = st->kcur;
+    if (k == 0u || k > d.arr_last) return;
+    const GmSweepArgs A = step_args(d, A0);
+    const uint32_t lo = d.arr_off[k], hi = d.arr_off[k + 1u];
+    if (lo == hi) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    for (uint32_t i = lo + tid; i < hi; i += 1024u) {
+        const uint32_t id = d.arr_ids[i], c = d.nsite[id];
+        if (site_deformed(d, c)) atomicMin(&d.claim[c], id);
+    }
+    __syncthreads();
+    uint32_t runI = st->nI, runB = st->nB, placed = 0;
+    unsigned long long pk = 0ull;
+    for (uint32_t base = lo; base < hi; base += 1024u) {
+        const uint32_t i = base + tid;
+        uint32_t id = 0, c = 0;
+        bool win = false, in = false;
+        if (i < hi) {
+            id = d.arr_ids[i];
+            c = d.nsite[id];
+            win = site_deformed(d, c) &&
+                  __hip_atomic_load(&d.claim[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == id;
+            in = win && site_interior(d, c);
+        }
+        const uint32_t vI = (win && in) ? 1u : 0u, vB = (win && !in) ? 1u : 0u;
+        const uint32_t packed = (vI << 16) | vB;
+        const uint32_t incl = wave_incl_scan(packed);
+        if (lane == 63u) {
+            ws[w][0] = incl >> 16;
+            ws[w][1] = incl & 0xFFFFu;
+        }
+        __syncthreads();
+        uint32_t preI = 0, preB = 0, totI = 0, totB = 0;
+        for (uint32_t q = 0; q < 16u; q++) {
+            if (q < w) { preI += ws[q][0]; preB += ws[q][1]; }
+            totI += ws[q][0];
+            totB += ws[q][1];
+        }
+        if (win) {
+            const uint32_t ex = incl - packed;
+            const uint32_t rel = in ? runI + preI + (ex >> 16) : runB + preB + (ex & 0xFFFFu);
+            if (rel >= (in ? d.capI : d.capB)) {
+                atomicOr(&st->err, in ? GM_ERRF_INSIDE_FULL : GM_ERRF_BORDER_FULL);
+            } else {
+                const uint32_t gslot = in ? rel : d.capI + rel;
+                const uint32_t g = d.grid[c];
+                const double P = d.nucP[id];
+                double inc = 0.0, fin = 0.0;
+                const gm_cs_t cs = cs_code(d, predict_values(d, A, k, 1.0, P, g, 26u, inc, fin), k);
+                GmRec *r = d.rec + gslot;
+                rec_set_a(r, 1.0, k, 26u);
+                rec_set_b(r, inc, fin);
+                rec_set_c(r, P, c, g);
+                *reinterpret_cast<uint4 *>(&r->rxg) = make_uint4(id, 0u, 0u, 0u);
+                d.cstep[gslot] = cs;
+                const uint32_t z = c / d.nxy, rem = c - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+                clear_deformed(d, x, y, z);
+                const unsigned long long key = gm_pkey(P);
+                pk = key > pk ? key : pk;
+            }
+        }
+        runI += totI;
+        runB += totB;
+        placed += totI + totB;
+        __syncthreads();
+    }
+    /* the block's largest weight */
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = (unsigned long long)__shfl_down((long long)pk, o, 64);
+        pk = other > pk ? other : pk;
+    }
+    if (lane == 0u && pk) atomicMax(&st->arr_pkey, pk);
+    if (tid == 0) {
+        st->nI = runI < d.capI ? runI : d.capI;
+        st->nB = runB < d.capB ? runB : d.capB;
+        st->nplaced = placed;
+    }
+}
+
+hipError_t gm_launch_arrivals(const GmDev &d, const GmSweepArgs &a, hipStream_t s) {
+    if (d.arr_last == 0u) return hipSuccess;
+    k_arrivals<<<1, 1024, 0, s>>>(d, a);
+    return hipGetLastError();
+}
+
 /* ----------------------------------------------------------- launch windows
  * Up to GM_WMAX steps are launched back to back.  Every kernel of a step reads
  * the step index and list lengths from the stats and returns at once once a
@@ -1494,6 +1595,8 @@ __global__ void k_window_begin(GmDev d, uint32_t k, double xcells) {
     st->nzeroed = 0;
     st->cwrap = 0;
     st->novf = 0;
+    st->nplaced = 0;
+    st->arr_pkey = 0ull;
     st->xcells = xcells;
 }
 
@@ -1517,8 +1620,12 @@ static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmSt
     st->npend0 = npend;
     const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
     st->xcells = xc;
-    const uint64_t slots = (uint64_t)r.nI + r.nB;
-    const uint64_t tiles = (uint64_t)gm_tiles_inside(r.nI) + (r.nB + 63) / 64 + GM_SW_TPC;
+    /* the next step's arrivals may go to either list */
+    const uint32_t kn = st->kcur;
+    const uint64_t narr = (kn >= 1u && kn <= d.arr_last) ? (uint64_t)d.arr_off[kn + 1u] - d.arr_off[kn] : 0ull;
+    const uint64_t slots = (uint64_t)r.nI + r.nB + narr;
+    const uint64_t nIn = (uint64_t)r.nI + narr < 0xFFFFFFC0ull ? (uint64_t)r.nI + narr : 0xFFFFFFC0ull;
+    const uint64_t tiles = (uint64_t)gm_tiles_inside((uint32_t)nIn) + ((uint64_t)r.nB + narr + 63) / 64 + GM_SW_TPC;
     bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                 any_top || xc >= w.xstop;
     halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;
@@ -1557,7 +1664,12 @@ __global__ void __launch_bounds__(64) k_step_tail(GmDev d, GmWindow w) {
     r.nattempt = st->nattempt;
     r.nzeroed = z1;
     r.err = st->err;
-    r.pad[0] = r.pad[1] = 0u;
+    r.nplaced = st->nplaced;
+    r.pone = st->pone;
+    r.pkey = st->arr_pkey;
+    st->nplaced = 0;
+    st->pone = 0;
+    st->arr_pkey = 0ull;
     step_finish(d, w, r, any_top);
 }
 

@@ -303,6 +303,7 @@ struct score_setup_ensemble {
     std::vector<double> ttime, ttemp;
     /* HeuristicRXFrontListDefragmentation: X, ascending */
     std::vector<double> defrag_x;
+    std::vector<double> log_x;   /* SingleGrainVolOverTime (UserDefLogPoint_X_Output) */
     /* pools */
     std::vector<WorldOri> oris;
     std::vector<WorldDefg> defg;
@@ -657,6 +658,22 @@ int read_defrag(const score_uds *u, score_setup_ensemble *e) {
     return 0;
 }
 
+int read_logpoints(const score_uds *u, score_setup_ensemble *e) {
+    /* init_parameter_additional (SCORE_Kernel.cpp:938-957): X in column 2, sorted ascending */
+    const UdsBlock *b;
+    int rc = block_or_fail(u, "SingleGrainVolOverTime", &b);
+    if (rc) return rc;
+    for (size_t d = 0; d < b->lines.size(); d++) {
+        double X;
+        if ((rc = table_real(*b, d, 1, &X))) return rc;
+        if (X < 0.0 || X > 1.0)
+            return fail(SCORE_SETUP_ERR_RANGE, "Invalid recrystallized volume fraction X at which to log locally all grains");
+        e->log_x.push_back(X);
+    }
+    std::sort(e->log_x.begin(), e->log_x.end());
+    return 0;
+}
+
 }  // namespace
 
 /* ---------------------------------------------------------------- domain */
@@ -670,7 +687,7 @@ struct score_setup_domain {
     std::vector<uint32_t> defg_ori, rxg_ori, rxg_site;
     std::vector<double> defg_rho;
     score_gm_params prm{};
-    std::vector<double> sched_t, sched_T, defrag_at, disp_t, disp_fr;
+    std::vector<double> sched_t, sched_T, defrag_at, disp_t, disp_fr, grainlog_at;
 };
 
 extern "C" {
@@ -775,7 +792,7 @@ int score_setup_ensemble_create(const score_uds *u, score_setup_ensemble **out) 
     /* the order of SCORE's init_parameter_* calls (SCORE_Kernel.cpp:1550-1612) */
     if ((rc = read_runtime(u, e)) || (rc = read_ensemble(u, e)) || (rc = read_nucleation(u, e)) ||
         (rc = read_physics(u, e)) || (rc = read_ideals(u, e)) || (rc = read_schedule(u, e)) || (rc = read_pools(u, e)) ||
-        (rc = read_defrag(u, e))) {
+        (rc = read_defrag(u, e)) || (rc = read_logpoints(u, e))) {
         delete e;
         return rc;
     }
@@ -1004,6 +1021,12 @@ int score_setup_domain_create(const score_setup_ensemble *e, uint32_t jobid, int
     }
     p.n_defrag = (uint32_t)s->defrag_at.size();
     p.defrag_at = s->defrag_at.empty() ? nullptr : s->defrag_at.data();
+    {
+        const double boxvolume = (double)e->nvol;
+        for (double X : e->log_x) s->grainlog_at.push_back(X * boxvolume);
+    }
+    p.n_grainlog = (uint32_t)s->grainlog_at.size();
+    p.grainlog_at = s->grainlog_at.empty() ? nullptr : s->grainlog_at.data();
     /* caHdl::init_zenerdrag (SCORE_Kernel.cpp:5904-5911) */
     p.zener_mode = e->zener_mode;
     p.zener_alpha = e->zener_alpha;

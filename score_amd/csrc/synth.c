@@ -95,6 +95,51 @@ int synth_distinct_sites(uint64_t seed, uint32_t n, uint64_t N, uint32_t *sites)
     return 0;
 }
 
+/* Whether the cell at linear index w has a 26-neighbour (periodic in x and y,
+ * open in z) in another grain. */
+static int on_boundary(const uint32_t *grid, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t w) {
+    const uint64_t nxy = (uint64_t)nx * ny;
+    const uint32_t z = (uint32_t)(w / nxy), y = (uint32_t)((w % nxy) / nx), x = (uint32_t)(w % nx);
+    const uint32_t g = grid[w];
+    for (int dz = -1; dz <= 1; dz++) {
+        const int zz = (int)z + dz;
+        if (zz < 0 || zz >= (int)nz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const uint32_t yy = (uint32_t)(((int)y + dy + (int)ny) % (int)ny);
+            for (int dx = -1; dx <= 1; dx++) {
+                const uint32_t xx = (uint32_t)(((int)x + dx + (int)nx) % (int)nx);
+                if (grid[xx + (uint64_t)yy * nx + (uint64_t)zz * nxy] != g) return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+/* n distinct sites drawn one after another without replacement, a cell next to
+ * a grain boundary (a 26-neighbour in another grain) `bias` times as likely as
+ * an interior one: a uniform candidate is kept with probability 1 on a
+ * boundary, 1/bias inside (bias >= 1).  Returns 0, or -1 when n exceeds the
+ * cells or on OOM. */
+int synth_distinct_sites_gb(uint64_t seed, uint32_t n, const uint32_t *grid, uint32_t nx, uint32_t ny, uint32_t nz,
+                            double bias, uint32_t *sites) {
+    const uint64_t N = (uint64_t)nx * ny * nz;
+    if ((uint64_t)n > N || !(bias >= 1.0)) return -1;
+    uint64_t *bits = (uint64_t *)calloc((N + 63) / 64, sizeof(uint64_t));
+    if (!bits) return -1;
+    uint64_t s = seed;
+    for (uint32_t i = 0; i < n;) {
+        const uint64_t w = below(&s, N);
+        const double u = u01(&s);
+        const uint64_t m = 1ull << (w & 63);
+        if (bits[w >> 6] & m) continue;
+        if (!on_boundary(grid, nx, ny, nz, w) && u * bias >= 1.0) continue;
+        bits[w >> 6] |= m;
+        sites[i++] = (uint32_t)w;
+    }
+    free(bits);
+    return 0;
+}
+
 /*
  * Nearest-seed labelling of the nx*ny*nz voxel centres (i+0.5, j+0.5, k+0.5).
  * Metric: ((dx/sx)^2 + (dy/sy)^2 + (dz/sz)^2), so sx > 1 stretches grains along

@@ -25,7 +25,7 @@ from .synth import DomainArrays, ParamsSpec
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_setup.so")
 DEFAULT_SEED = -46356  # SCORE_SETUP_DEFAULT_SEED
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 ERR = {-1: "ERR_ARG", -2: "ERR_FILE", -3: "ERR_FORMAT", -4: "ERR_RANGE", -5: "ERR_UNSUPPORTED", -6: "ERR_OOM"}
 ERR_ARG, ERR_FILE, ERR_FORMAT, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6
@@ -262,6 +262,7 @@ class Ensemble:
         dfr = tuple(np.ctypeslib.as_array(p.defrag_at, shape=(p.n_defrag,)).tolist()) if p.n_defrag else ()
         dt_ = tuple(np.ctypeslib.as_array(p.disp_time, shape=(p.n_disp,)).tolist()) if p.n_disp else ()
         dfr_ = tuple(np.ctypeslib.as_array(p.disp_fr, shape=(p.n_disp,)).tolist()) if p.n_disp else ()
+        glg = tuple(np.ctypeslib.as_array(p.grainlog_at, shape=(p.n_grainlog,)).tolist()) if p.n_grainlog else ()
         return ParamsSpec(mobility_model=p.mobility_model,
                           rh_hagb_m0=p.rh_hagb_m0, rh_hagb_hact_ev=u.real(B, "RHModelHAGBHact"),
                           rh_cut=p.rh_cut, rh_trans=p.rh_trans, rh_exponent=p.rh_exponent,
@@ -277,7 +278,8 @@ class Ensemble:
                           rec_vacancy_hact_ev=u.real(R, "RecoveryVacancyDiffHact"),
                           rec_solute_d0=p.rec_solute_d0, rec_solute_hact_ev=u.real(R, "RecoverySoluteDiffHact"),
                           rec_solute_ls=p.rec_solute_ls, rec_solute_conc=p.rec_solute_conc,
-                          rec_alpha3=p.rec_alpha3, rec_kappa2=p.rec_kappa2, rec_c3=p.rec_c3, rec_c5=p.rec_c5)
+                          rec_alpha3=p.rec_alpha3, rec_kappa2=p.rec_kappa2, rec_c3=p.rec_c3, rec_c5=p.rec_c5,
+                          rec_tau=p.rec_tau, grainlog_at=glg)
 
     def close(self) -> None:
         if self._h:

@@ -19,9 +19,14 @@ TRAJ_DTYPE = np.dtype([
     ("step", np.uint32), ("sv", np.uint32), ("nfull_inside", np.uint32), ("nfull_border", np.uint32),
     ("nslots_inside", np.uint32), ("nslots_border", np.uint32), ("ninfected", np.uint32), ("npending", np.uint32),
     ("t", np.float64), ("dt", np.float64), ("X", np.float64), ("xcells", np.float64),
-    ("temperature", np.float64), ("rhomax", np.float64),
+    ("temperature", np.float64), ("rhomax", np.float64), ("pmax", np.float64), ("nplaced", np.uint32),
+    ("reserved", np.uint32),
 ], align=True)
 assert TRAJ_DTYPE.itemsize == C.sizeof(abi.TrajRow)
+
+GRAINLOG_DTYPE = np.dtype([("t", np.float64), ("X", np.float64), ("logcnt", np.uint32), ("step", np.uint32),
+                           ("sv", np.uint32), ("ngrains", np.uint32)], align=True)
+assert GRAINLOG_DTYPE.itemsize == C.sizeof(abi.GrainLogRow)
 
 
 def result_dict(r: abi.Result) -> dict:
@@ -85,6 +90,19 @@ class GrowthMachine:
             abi.check(self._lib, self._lib.score_gm_get_trajectory(
                 self._handle(), out.ctypes.data_as(C.POINTER(abi.TrajRow)), n.value, C.byref(n)))
         return out
+
+    def grainlog(self) -> tuple[np.ndarray, np.ndarray]:
+        """The grain-evolution log (mygrainevolution): one row per log point and
+        the cell counts at it, deformed grains first (rows x (n_defg + n_rxg))."""
+        n = C.c_size_t(0)
+        abi.check(self._lib, self._lib.score_gm_get_grainlog(self._handle(), None, None, 0, C.byref(n)))
+        rows = np.zeros(n.value, GRAINLOG_DTYPE)
+        counts = np.zeros((n.value, self.dom.n_defg + self.dom.n_rxg), np.uint32)
+        if n.value:
+            abi.check(self._lib, self._lib.score_gm_get_grainlog(
+                self._handle(), rows.ctypes.data_as(C.POINTER(abi.GrainLogRow)),
+                counts.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n)))
+        return rows, counts
 
     def front(self, which: int) -> dict:
         n = C.c_uint32(0)

@@ -11,9 +11,8 @@ the first step whose Xcells reaches each output threshold, before t advances
 trajectory the same points give the time and Xcells; the deformed grains then
 hold N - Xcells cells in all, which is all the kinetics needs, and the nuclei's
 final counts give the final grain sizes.  The macrotexture needs every grain's
-count at every log point: grain_log_by_steps records them by stepping a growth
-machine (or the oracle) one step at a time and reading its grain counts where
-SCORE's master section logs.
+count at every log point: the growth machine records them itself where SCORE's
+master section logs (grain_log_from_machine).
 """
 from __future__ import annotations
 
@@ -80,28 +79,18 @@ def grain_log_from_trajectory(traj: np.ndarray, N: int, defg_counts, rxg_counts,
     return GrainLog(N=N, ndefg=1, times=np.asarray(times), counts=counts, final_counts=final, tsimend=tend)
 
 
-def grain_log_by_steps(machine, N: int, xlog, classes=None) -> GrainLog:
-    """A domain's full log (log_grainevo): steps `machine` (a GrowthMachine or the
-    oracle) to the end of its run one step at a time; at the first step whose
-    Xcells reaches each threshold of `xlog` (in cells, ascending; one per step) the
-    cell count of every grain, deformed grains first, is recorded with the step's
-    start time, as SCORE's master section does before t advances
-    (log_OUTPUT_Diehl, SCORE_OnTheFlyCharacterization.cpp:646-651)."""
-    times, cols = [], []
-    k = 0
-    done = bool(machine.result()["completed"])
-    while not done:
-        t0 = float(machine.result()["t"])
-        done = machine.step(1)
-        if k < len(xlog) and float(machine.result()["xcells"]) >= xlog[k]:
-            dg, rg = machine.counts()
-            times.append(t0)
-            cols.append(np.concatenate([dg, rg]).astype(np.float64))
-            k += 1
+def grain_log_from_machine(machine, N: int, classes=None) -> GrainLog:
+    """A domain's full log (mygrainevolution) as the growth machine (or the
+    oracle) recorded it during its run: at the first step whose Xcells reached
+    each log point (score_gm_params.grainlog_at, one per step) the cell count of
+    every grain, deformed grains first, with the step's start time, as SCORE's
+    master section logs before t advances (log_OUTPUT_Diehl,
+    SCORE_OnTheFlyCharacterization.cpp:646-651)."""
+    rows, counts = machine.grainlog()
     dg, rg = machine.counts()
     final = np.concatenate([dg, rg]).astype(np.float64)
-    counts = np.stack(cols, axis=1) if cols else np.zeros((len(final), 1))
-    return GrainLog(N=N, ndefg=len(dg), times=np.asarray(times), counts=counts, final_counts=final,
+    cols = counts.T.astype(np.float64) if len(rows) else np.zeros((len(final), 1))
+    return GrainLog(N=N, ndefg=len(dg), times=np.asarray(rows["t"], np.float64), counts=cols, final_counts=final,
                     tsimend=float(machine.result()["t"]),
                     classes=None if classes is None else np.asarray(classes, np.int64))
 

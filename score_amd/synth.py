@@ -38,6 +38,9 @@ def _lib() -> C.CDLL:
         lib.synth_points.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, f64p]
         lib.synth_distinct_sites.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, u32p]
         lib.synth_distinct_sites.restype = C.c_int
+        lib.synth_distinct_sites_gb.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                C.c_double, u32p]
+        lib.synth_distinct_sites_gb.restype = C.c_int
         for fn in (lib.synth_voronoi, lib.synth_voronoi_bruteforce):
             fn.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, f64p, C.c_double, C.c_double,
                            C.c_double, C.c_int, u32p]
@@ -93,6 +96,39 @@ def distinct_sites(seed: int, n: int, N: int) -> np.ndarray:
     return s
 
 
+def distinct_sites_gb(seed: int, n: int, grid: np.ndarray, nx: int, ny: int, nz: int, bias: float) -> np.ndarray:
+    """n distinct sites, cells next to a grain boundary (a 26-neighbour in another
+    grain, periodic in x and y) `bias` times as likely as interior cells."""
+    s = np.empty(n, np.uint32)
+    g = np.ascontiguousarray(grid, np.uint32)
+    if n and _lib().synth_distinct_sites_gb(seed, n, _ptr(g, C.c_uint32), nx, ny, nz, float(bias),
+                                            _ptr(s, C.c_uint32)) != 0:
+        raise ValueError("cannot draw that many distinct sites")
+    return s
+
+
+def on_boundary(grid: np.ndarray, nx: int, ny: int, nz: int) -> np.ndarray:
+    """Cells with a 26-neighbour in another grain (periodic x and y, open z)."""
+    g = np.asarray(grid).reshape(nz, ny, nx)
+    out = np.zeros(g.shape, bool)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if (dx, dy, dz) == (0, 0, 0):
+                    continue
+                sh = np.roll(np.roll(g, -dy, axis=1), -dx, axis=2)
+                if dz:
+                    diff = np.zeros(g.shape, bool)
+                    if dz > 0:
+                        diff[:-1] = g[:-1] != sh[1:]
+                    else:
+                        diff[1:] = g[1:] != sh[:-1]
+                else:
+                    diff = g != sh
+                out |= diff
+    return out.ravel()
+
+
 def voronoi(nx: int, ny: int, nz: int, seeds_xyz: np.ndarray, stretch=(1.0, 1.0, 1.0), periodic_xy=True,
             brute_force=False) -> np.ndarray:
     """Nearest-seed labels of all voxel centres, x fastest (SCORE's cellgrid layout)."""
@@ -119,6 +155,7 @@ class DomainArrays:
     defg_rho: np.ndarray   # float64[n_defg]
     rxg_ori: np.ndarray    # uint32[n_rxg]
     rxg_site: np.ndarray   # uint32[n_rxg]
+    rxg_step: np.ndarray | None = None   # uint32[n_rxg] arrival step (0: site saturation); None: all 0
 
     @property
     def N(self) -> int:
@@ -137,6 +174,8 @@ class DomainArrays:
         keep = [np.ascontiguousarray(self.cellgrid, np.uint32), np.ascontiguousarray(self.ori_quat, np.float64),
                 np.ascontiguousarray(self.defg_ori, np.uint32), np.ascontiguousarray(self.defg_rho, np.float64),
                 np.ascontiguousarray(self.rxg_ori, np.uint32), np.ascontiguousarray(self.rxg_site, np.uint32)]
+        if self.rxg_step is not None:
+            keep.append(np.ascontiguousarray(self.rxg_step, np.uint32))
         d = abi.Domain()
         d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
         d.cellsize = self.cellsize
@@ -149,6 +188,7 @@ class DomainArrays:
         d.n_rxg = len(keep[4])
         d.rxg_ori = _ptr(keep[4], C.c_uint32)
         d.rxg_site = _ptr(keep[5], C.c_uint32)
+        d.rxg_step = _ptr(keep[6], C.c_uint32) if self.rxg_step is not None else C.cast(None, C.POINTER(C.c_uint32))
         return d, keep
 
 
@@ -194,12 +234,15 @@ class ParamsSpec:
     rec_kappa2: float = 0.1
     rec_c3: float = 1.0
     rec_c5: float = 0.65
+    rec_tau: float = 0.0                # s, exponential recovery (recovery_mode 4)
+    grainlog_at: tuple = ()             # grain-evolution log thresholds, in cells
 
     def to_c(self):
         keep = [np.ascontiguousarray(self.sched_time, np.float64), np.ascontiguousarray(self.sched_temp, np.float64),
                 np.ascontiguousarray(self.defrag_at if len(self.defrag_at) else [0.0], np.float64),
                 np.ascontiguousarray(self.disp_time if len(self.disp_time) else [0.0], np.float64),
-                np.ascontiguousarray(self.disp_fr if len(self.disp_fr) else [0.0], np.float64)]
+                np.ascontiguousarray(self.disp_fr if len(self.disp_fr) else [0.0], np.float64),
+                np.ascontiguousarray(self.grainlog_at if len(self.grainlog_at) else [0.0], np.float64)]
         p = abi.Params()
         p.mobility_model = self.mobility_model
         p.zener_mode = self.zener_mode
@@ -226,6 +269,9 @@ class ParamsSpec:
         p.rec_solute_d0, p.rec_solute_hact = self.rec_solute_d0, self.rec_solute_hact_ev * ECHARGE
         p.rec_solute_ls, p.rec_solute_conc = self.rec_solute_ls, self.rec_solute_conc
         p.rec_alpha3, p.rec_kappa2, p.rec_c3, p.rec_c5 = self.rec_alpha3, self.rec_kappa2, self.rec_c3, self.rec_c5
+        p.rec_tau = self.rec_tau
+        p.n_grainlog = len(self.grainlog_at)
+        p.grainlog_at = _ptr(keep[5], C.c_double)
         return p, keep
 
 
@@ -248,6 +294,33 @@ def make_voronoi_domain(nx: int, ny: int, nz: int, n_grains: int, n_nuclei: int,
         defg_rho=np.ascontiguousarray(rho, np.float64),
         rxg_ori=(n_grains + np.arange(n_nuclei)).astype(np.uint32),
         rxg_site=sites)
+
+
+def make_rolled_sheet_domain(nx: int = 1024, ny: int = 1024, nz: int = 256, n_grains: int = 200000,
+                             n_nuclei: int = 1000000, arrival_steps: int = 60, gb_bias: float = 3.0, seed: int = 1,
+                             cellsize: float = 0.5e-6) -> DomainArrays:
+    """Rolled sheet: pancake Voronoi grains (the metric stretched 4 : 1 : 0.25 along
+    RD : TD : ND), rho log-normal (median 5e14 m^-2, sigma 0.5 of ln rho), uniform
+    orientations; the nuclei at distinct sites, cells next to a grain boundary
+    `gb_bias` times as likely; the first half site-saturated (step 0), the second
+    half arriving at a constant rate over steps 1..arrival_steps, in pool order."""
+    N = nx * ny * nz
+    xyz = points(substream(seed, 0), n_grains, nx, ny, nz)
+    grid = voronoi(nx, ny, nz, xyz, stretch=(4.0, 1.0, 0.25), periodic_xy=True)
+    q_def = quaternions(substream(seed, 1), n_grains)
+    rho = lognormal(substream(seed, 2), n_grains, float(np.log(5e14)), 0.5)
+    sites = distinct_sites_gb(substream(seed, 3), n_nuclei, grid, nx, ny, nz, gb_bias)
+    q_rx = quaternions(substream(seed, 4), n_nuclei)
+    half = n_nuclei // 2
+    steps = np.zeros(n_nuclei, np.uint32)
+    if arrival_steps > 0 and n_nuclei > half:
+        j = np.arange(n_nuclei - half, dtype=np.int64)
+        steps[half:] = (1 + (j * arrival_steps) // (n_nuclei - half)).astype(np.uint32)
+    return DomainArrays(
+        nx=nx, ny=ny, nz=nz, cellsize=cellsize, cellgrid=grid,
+        ori_quat=np.concatenate([q_def, q_rx]) if n_nuclei else q_def,
+        defg_ori=np.arange(n_grains, dtype=np.uint32), defg_rho=np.ascontiguousarray(rho, np.float64),
+        rxg_ori=(n_grains + np.arange(n_nuclei)).astype(np.uint32), rxg_site=sites, rxg_step=steps)
 
 
 # Workloads (sizes and distributions of the 100^3 CPU case and the 512^3 single-GPU case)

@@ -66,7 +66,7 @@ def test_struct_layouts_match_c():
         pytest.skip("no C compiler")
     structs = {"score_gm_domain": abi.Domain, "score_gm_params": abi.Params, "score_gm_traj_row": abi.TrajRow,
                "score_gm_result": abi.Result, "score_gm_profile": abi.Profile,
-               "score_gm_batch_item": abi.BatchItem}
+               "score_gm_batch_item": abi.BatchItem, "score_gm_grainlog_row": abi.GrainLogRow}
     body = []
     for cname, py in structs.items():
         for f, _ in py._fields_:
@@ -127,7 +127,13 @@ def test_invalid_arguments_rejected_before_device_use():
     cp.zener_mode = 3
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     cp, keep_p = synth.ParamsSpec().to_c()
-    cp.recovery_mode = 4
+    cp.recovery_mode = 5
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    # exponential recovery without a positive time constant; log points announced but not given
+    cp, keep_p = synth.ParamsSpec(recovery_mode=4, rec_tau=0.0).to_c()
+    assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    cp, keep_p = synth.ParamsSpec(grainlog_at=(10.0,)).to_c()
+    cp.grainlog_at = C.cast(None, C.POINTER(C.c_double))
     assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
     # NMAX outside the range SCORE accepts, [1, DEFAULT_NMAX] (SCORE_Kernel.cpp:274)
     for nmax in (0, -5, 1000001):

@@ -17,7 +17,7 @@ def _gm(dom, prm=None):
 
 
 TRAJ_FIELDS = ("step", "sv", "nfull_inside", "nfull_border", "nslots_inside", "nslots_border", "ninfected",
-               "t", "dt", "X", "xcells", "temperature", "rhomax")
+               "t", "dt", "X", "xcells", "temperature", "rhomax", "pmax", "nplaced")
 
 
 def assert_traj_equal(a, b):
@@ -83,6 +83,8 @@ def assert_final_equal(o, g):
     assert np.array_equal(do, dg) and np.array_equal(ro_, rg_)
     for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
         assert_front_equal(o.front(lst), g.front(lst), f"final list{lst}")
+    lo, lg = o.grainlog(), g.grainlog()
+    assert lo[0].tobytes() == lg[0].tobytes() and np.array_equal(lo[1], lg[1])
 
 
 def test_tiny_stepwise(require_gpu):
@@ -500,9 +502,10 @@ def test_profile_counters(require_gpu):
 
 
 def test_per_grain_log_and_macrotexture_from_deck(require_gpu):
-    """A deck domain stepped to the end of its run, the cell count of every grain
-    read where SCORE logs its grain evolution: the same log, and so the same
-    rediscretized macrotexture, from the GPU and from the oracle."""
+    """A deck domain run to its end in launch windows, the cell count of every
+    grain logged by the growth machine where SCORE logs its grain evolution: the
+    same log, and so the same rediscretized macrotexture, from the GPU and from
+    the oracle."""
     import os
     from score_amd import domain_setup as ds
     from score_amd import postprocess as pp
@@ -511,10 +514,15 @@ def test_per_grain_log_and_macrotexture_from_deck(require_gpu):
     dom, prm, _ = e.domain(1)
     classes = e.texture_classes(dom)
     assert len(classes) == dom.n_defg + dom.n_rxg and set(classes.tolist()) <= set(range(1 + e.info.n_ideal))
-    xlog = [0.0] + [0.02 * i * dom.N for i in range(1, 51)]
+    import dataclasses
+    prm = dataclasses.replace(prm, grainlog_at=tuple([0.0] + [0.02 * i * dom.N for i in range(1, 51)]))
     o, g = Oracle(dom, prm), _gm(dom, prm)
-    lo = pp.grain_log_by_steps(o, dom.N, xlog, classes)
-    lgpu = pp.grain_log_by_steps(g, dom.N, xlog, classes)
+    o.solve_rxgrowth()
+    g.solve_rxgrowth()
+    ro, rgl = o.grainlog(), g.grainlog()
+    assert ro[0].tobytes() == rgl[0].tobytes() and np.array_equal(ro[1], rgl[1])
+    lo = pp.grain_log_from_machine(o, dom.N, classes)
+    lgpu = pp.grain_log_from_machine(g, dom.N, classes)
     assert lo.times.view(np.uint64).tolist() == lgpu.times.view(np.uint64).tolist()
     assert np.array_equal(lo.counts, lgpu.counts) and np.array_equal(lo.final_counts, lgpu.final_counts)
     assert lo.tsimend == lgpu.tsimend
@@ -558,6 +566,14 @@ def _mixed_case(seed):
     kw["rh_hagb_m0"] = float(10 ** rng.uniform(-1, 1.5))
     kw["nmax"] = 3000
     chunks = tuple(int(c) for c in rng.integers(1, 9, 4))
+    # drawn after the settings above, so that those stay as they were: exponential
+    # recovery instead of none, nuclei arriving after step 0, grain-evolution log points
+    if rec == 0 and rng.random() < 0.25:
+        kw.update(recovery_mode=4, rec_tau=float(rng.uniform(20.0, 2000.0)))
+    if rng.random() < 0.4:
+        dom.rxg_step = rng.integers(0, int(rng.integers(2, 40)), dom.n_rxg).astype(np.uint32)
+    if rng.random() < 0.5:
+        kw["grainlog_at"] = tuple(sorted(float(x) * dom.N for x in rng.uniform(0.0, 1.0, int(rng.integers(1, 6)))))
     return dom, ParamsSpec(**kw), chunks
 
 
@@ -565,7 +581,8 @@ def _mixed_case(seed):
 def test_mixed_settings_in_launch_windows(require_gpu, seed):
     dom, prm, chunks = _mixed_case(seed)
     print(f"mixed case {seed}: {dom.nx}x{dom.ny}x{dom.nz} model={prm.mobility_model} recovery={prm.recovery_mode} "
-          f"zener={prm.zener_mode} defrag={len(prm.defrag_at)} chunks={chunks}", flush=True)
+          f"zener={prm.zener_mode} defrag={len(prm.defrag_at)} arrivals={dom.rxg_step is not None} "
+          f"logs={len(prm.grainlog_at)} chunks={chunks}", flush=True)
     o, g = Oracle(dom, prm), _gm(dom, prm)
     k = 0
     while True:
@@ -632,3 +649,47 @@ def test_weight_cache_spill_in_one_step(require_gpu):
     o, g = run_both(dom, ParamsSpec())
     assert_final_equal(o, g)
     assert g.trajectory()["npending"].max() > 32768
+
+
+def _small_rolled_sheet(seed=5):
+    return synth.make_rolled_sheet_domain(64, 48, 24, n_grains=40, n_nuclei=400, arrival_steps=25, seed=seed)
+
+
+def test_arrivals_stepwise(require_gpu):
+    """Nuclei arriving after step 0 (half of them, over 25 steps, on a pancake
+    structure with sites biased to grain boundaries): every front slot equal to
+    the oracle's after every step."""
+    dom = _small_rolled_sheet()
+    o, g = run_both(dom, ParamsSpec(rh_hagb_m0=2.0), stepwise=True)
+    assert_final_equal(o, g)
+    assert int(g.trajectory()["nplaced"].sum()) > 0
+
+
+@pytest.mark.parametrize("chunks", [(8,), (3, 8, 1, 5)])
+def test_arrivals_and_grain_log_in_launch_windows(require_gpu, chunks):
+    """Arrivals placed inside launch windows (the kernels read the step's
+    arrivals on the device), grain-evolution logs ending windows at their
+    thresholds; the state equal to the oracle's at every window boundary."""
+    dom = _small_rolled_sheet(seed=6)
+    prm = ParamsSpec(rh_hagb_m0=2.0, grainlog_at=tuple(f * dom.N for f in (0.0, 0.1, 0.25, 0.5, 0.75, 0.9)))
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+    assert len(g.grainlog()[0]) == 6
+    assert g.result()["windows"] < g.result()["step"]
+
+
+def test_exponential_recovery_stepwise(require_gpu):
+    dom = synth.make_voronoi_domain(40, 36, 32, n_grains=9, n_nuclei=30, seed=8)
+    o, g = run_both(dom, ParamsSpec(recovery_mode=4, rec_tau=400.0, rh_hagb_m0=3.0), stepwise=True)
+    assert_final_equal(o, g)

@@ -185,10 +185,11 @@ def test_macrotexture_from_a_per_grain_log():
     threshold, every column summing to N; the classes' volumes then sum to N at every
     time and the deformed rows give the same kinetics as the aggregated log"""
     dom = synth.make_voronoi_domain(32, 30, 28, n_grains=6, n_nuclei=15, seed=3)
-    o = Oracle(dom, synth.ParamsSpec())
     xlog = [0.0] + [0.05 * i * dom.N for i in range(1, 21)]
+    o = Oracle(dom, synth.ParamsSpec(grainlog_at=tuple(xlog)))
+    o.solve_rxgrowth()
     classes = np.arange(dom.n_defg + dom.n_rxg) % 3
-    lg = pp.grain_log_by_steps(o, dom.N, xlog, classes)
+    lg = pp.grain_log_from_machine(o, dom.N, classes)
     tr = o.trajectory()
     agg = pp.grain_log_from_trajectory(tr, dom.N, *o.counts(), xlog)
     assert np.array_equal(lg.times, agg.times)

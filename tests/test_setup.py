@@ -230,7 +230,7 @@ def test_params_from_deck(tmp_path):
     assert lib.score_setup_domain_view(h, C.byref(d), C.byref(p)) == 0
     q, keep = prm.to_c()
     for f, _ in abi.Params._fields_:
-        if f in ("sched_time", "sched_temp", "defrag_at", "disp_time", "disp_fr"):
+        if f in ("sched_time", "sched_temp", "defrag_at", "disp_time", "disp_fr", "grainlog_at"):
             continue
         assert getattr(p, f) == getattr(q, f), f
     lib.score_setup_domain_free(h)
