@@ -53,7 +53,9 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
     kernels use what this design's sweep must touch: a 2 B completion-step code
     per occupied slot, 16 B of masks per 64-slot tile, 4 B per list entry, and on
     the steps whose scalars changed 100 B per re-anchored cell (the 64 B record
-    read, its 32 B first sector written, its 2 B code read and written)."""
+    read, its 32 B first sector written, its 2 B code read and written).  A
+    retired cell's grid entry is not written during the run: its (cell, grain)
+    pair goes to the retirement log with the step's parent data (8 B)."""
     full, att, new = pf["n_full"], pf["n_attempts"], pf["n_new"]
     scanned, tiles = pf["slots_scanned"], (pf["slots_scanned"] + 63) // 64
     table = {
@@ -62,11 +64,12 @@ def kernel_bytes(name: str, pf: dict) -> tuple[int, str]:
         "k_sweep": (2 * scanned + 16 * tiles, "2 B per occupied slot + 16 B per tile"),
         "k_compact": (16 * tiles + 4 * full, "16 B per tile + 4 B per FullRX entry"),
         "k_recycle": (4 * pf["n_recycled"], "4 B per recycled slot handed out"),
-        "k_propose": (220 * full + 4 * att, "220 B per completed cell (26 cell-grid entries 104, record 64 read, "
-                                           "parent data 48 and attempt mask 4 written) + 4 B per valid attempt"),
+        "k_propose": (228 * full + 4 * att, "228 B per completed cell (26 cell-grid entries 104, record 64 read, "
+                                           "parent data 48, attempt mask 4 and retirement-log entry 8 written) "
+                                           "+ 4 B per valid attempt"),
         "k_decide": (28 * full + 4 * att, "28 B per completed cell (position 16, attempt mask 4, winner masks 8) "
                                           "+ 4 B per valid attempt (claim)"),
-        "k_commit": (34 * new + 42 * full, "34 B per new record + 42 B per completed cell (record 30, retire 12)"),
+        "k_commit": (34 * new + 38 * full, "34 B per new record + 38 B per completed cell (record 30, counts 8)"),
     }
     return table.get(name, (0, "scan of per-block counts"))
 

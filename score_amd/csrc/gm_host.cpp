@@ -309,7 +309,7 @@ void dfree(score_gm_handle *h, void *p) {
 
 void free_device(score_gm_handle *h) {
     GmDev &d = h->d;
-    void *ptrs[] = {d.grid, d.claim, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
+    void *ptrs[] = {d.grid, d.claim, d.rlog, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, d.arr_ids, d.arr_off, d.nsite, d.nucP};
@@ -858,6 +858,7 @@ int step_error(uint32_t err) {
     if (err & GM_ERRF_MISSED) return fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
     if (err & GM_ERRF_KEYS)
         return fail(SCORE_GM_ERR_UNSUPPORTED, "more than 2^27 cells completed in one step (infection keys are 32-bit)");
+    if (err & GM_ERRF_RLOG_FULL) return fail(SCORE_GM_ERR_CAPACITY, "retirement log full (cells retired twice)");
     return fail(SCORE_GM_ERR_STATE, "internal error: mobility-weight cache entry lost while growing it");
 }
 
@@ -923,6 +924,9 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     a.k = h->step;
     a.reanchor = (h->force_reanchor || !h->have_last_sp || !same_scalars(sp, h->last_sp)) ? 1u : 0u;
     a.stepcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(h->p.nmax, 0), (int64_t)GM_STEP_MAX - 1);
+    /* recovery that changes rho re-anchors every cell at every step, so a
+     * prediction only has to tell whether the cell completes in the step itself */
+    a.next_only = (h->p.recovery_mode == 1 || h->p.recovery_mode == 3 || h->p.recovery_mode == 4) ? 1u : 0u;
     const bool prof = h->prof;
     const uint32_t slots0 = h->nI + h->nB;
     const uint32_t W = window_length(h, wmax, sp);
@@ -1081,9 +1085,12 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
             }
         }
         if (h->p.recovery_mode != 0) {
-            /* update_mydgoptimize_rho (SCORE_RecoveryModels.cpp:128-135): re-sorted by the current rho */
-            std::stable_sort(h->rho_order.begin(), h->rho_order.end(),
-                             [&](uint32_t a, uint32_t b) { return h->rho_host[a] > h->rho_host[b]; });
+            /* update_mydgoptimize_rho (SCORE_RecoveryModels.cpp:128-135): re-sorted by the
+             * current rho; a stable sort leaves an ordered list as it is, so an
+             * order that recovery kept (every rho scaled alike) is not sorted again */
+            auto desc = [&](uint32_t a, uint32_t b) { return h->rho_host[a] > h->rho_host[b]; };
+            if (!std::is_sorted(h->rho_order.begin(), h->rho_order.end(), desc))
+                std::stable_sort(h->rho_order.begin(), h->rho_order.end(), desc);
             h->rho_top = 0;
         }
         update_rhomax(h);
@@ -1311,7 +1318,11 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return rc;
     }
-    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.cstep, capTot)) ||
+    /* every cell retires once; a cell revived by a defragmentation can retire
+     * again, at most once per front slot of the Border list beyond that */
+    d.rlogCap = d.N + (uint64_t)d.capB + 4096u;
+    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.rlog, d.rlogCap)) ||
+        (rc = dalloc(h, &d.cstep, capTot)) ||
         (rc = dalloc(h, &d.rec, capTot)) ||
         (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
         (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
@@ -1709,6 +1720,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     const uint64_t front = std::max<uint64_t>(65536, N / 4);
     uint64_t b = 0;
     b += 8 * N;                                   /* grid, claim */
+    b += 8 * (N + capB + 4096);                   /* retirement log */
     b += 32 * bricks;                             /* deformed bitmap */
     b += (sizeof(gm_cs_t) + sizeof(GmRec)) * capTot; /* completion steps, records */
     b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */

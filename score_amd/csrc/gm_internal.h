@@ -2,9 +2,13 @@
  * gm_internal.h -- device layout and kernel launchers of the growth machine.
  *
  * One domain lives in HBM as:
- *   grid   uint32[N]   deformed-grain id (< ndefg) until the cell transforms, then
- *                      ndefg+rxg; a cell on the front keeps its deformed id (the
- *                      exported map marks it INFECTED from the front records)
+ *   grid   uint32[N]   deformed-grain id (< ndefg); not written during the run: only
+ *                      deformed cells' entries are ever read then, and a cell
+ *                      that transforms is appended to the retirement log
+ *   rlog   (cell, ndefg+rxg) per retired cell in retirement order, written
+ *                      coalesced with the step's parent data; the exported map
+ *                      is the grid with the log applied and the front's cells
+ *                      marked INFECTED
  *   claim  uint32[N]   per-cell infection claim key, 0xFFFFFFFF until claimed; a
  *                      claimed cell is infected in the same step and never
  *                      claimed again, so claims are never reset
@@ -55,6 +59,7 @@ typedef uint16_t gm_cs_t;
 #define GM_CS_FAR 0xFFFBu         /* active, completes at least a horizon after its prediction */
 #define GM_STEP_NONE 0xFFFFFFFFu  /* no completion step */
 #define GM_STEP_MAX 0xFFFFFFF0u   /* steps of a run stay below this */
+#define GM_STEP_NEXT 0xFFFFFFFEu  /* does not complete in the predicted step; the next step predicts it again */
 
 /* error flags raised on the device */
 #define GM_ERRF_BORDER_FULL 0x1u
@@ -62,6 +67,7 @@ typedef uint16_t gm_cs_t;
 #define GM_ERRF_PCACHE_FULL 0x4u
 #define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
 #define GM_ERRF_KEYS 0x10u        /* more than 2^27 completed cells in one step: infection keys would not fit */
+#define GM_ERRF_RLOG_FULL 0x20u   /* more retirements than the log holds (only cells retired twice can do it) */
 
 /* the mobility-weight cache probes at most this many slots from a key's home;
  * a key that finds no room within them spills: its cells hold the pending
@@ -161,6 +167,7 @@ struct alignas(8) GmStats {
     uint32_t novf;            /* weight lookups that found the cache full (their cells wait for a larger table) */
     uint32_t nplaced;         /* nuclei placed at the start of the current step */
     unsigned long long arr_pkey; /* largest weight of those nuclei (gm_pkey order) */
+    unsigned long long nret;  /* entries of the retirement log (cells retired so far) */
     uint32_t zeroed_ids[GM_ZEROED_INLINE]; /* the first of them; all are in GmDev::zlist */
     unsigned long long pend_keys[GM_PEND_INLINE]; /* the first new P-cache keys; all are in GmDev::pend_key */
     /* launch windows: several steps run back to back; the host reads the rows at the end */
@@ -191,6 +198,8 @@ struct GmDev {
     uint32_t recCapI, recCapB;
     /* state */
     uint32_t *grid, *claim;
+    uint2 *rlog;                /* retirement log, rlogCap entries */
+    uint64_t rlogCap;
     uint32_t *dmap;
     uint32_t dnbx, dnby, dnbz;  /* bricks of the deformed-cell bitmap per axis */
     gm_cs_t *cstep;
@@ -252,6 +261,9 @@ struct GmSweepArgs {
     uint32_t reanchor;          /* scalars changed since the previous step */
     uint32_t nI, nB;            /* occupied slots, for the grid sizes */
     uint32_t stepcap;           /* last step a completion can fall in */
+    uint32_t next_only;         /* every following step re-anchors: predict only whether a cell
+                                 * completes in the step it is predicted for (else FAR) */
+    uint32_t pad;
 };
 
 /* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
@@ -295,7 +307,9 @@ hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, 
  * ixyz and gid hold nI+nB entries. */
 hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k, double thr, uint32_t *ixyz,
                             uint32_t *gid, hipStream_t s);
-/* the cell grid with every cell held by an active front record marked INFECTED */
+/* the cell grid as SCORE holds it, from a copy of d.grid in `out`: every retired
+ * cell set to ndefg+rxg from the log, every cell held by an active front record
+ * marked INFECTED */
 hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);

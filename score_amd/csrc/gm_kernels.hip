@@ -263,7 +263,17 @@ static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const 
     uint32_t cs = GM_STEP_NONE;
     fin = 0.0;
     if (cell_increment(d, A.sp, P, defg, inf, inc)) {
-        if (k <= A.stepcap) {
+        if (k <= A.stepcap && A.next_only) {
+            /* the step's one addition; a cell it leaves below 1.0 is re-anchored
+             * and predicted again before the next step (GM_STEP_FAR_NEXT) */
+            const double g = f + inc;
+            if (g >= 1.0) {
+                cs = k;
+                fin = g;
+            } else {
+                cs = GM_STEP_NEXT;
+            }
+        } else if (k <= A.stepcap) {
             const uint64_t j = gm_radd_until_one(f, inc, (uint64_t)(A.stepcap - k) + 1u);
             if (j) {
                 cs = k + (uint32_t)(j - 1u);
@@ -279,6 +289,7 @@ static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const 
 /* the stored code of completion step cs predicted at step kp */
 static __device__ __forceinline__ gm_cs_t cs_code(const GmDev &d, uint32_t cs, uint32_t kp) {
     if (cs == GM_STEP_NONE) return (gm_cs_t)GM_CS_NEVER;
+    if (cs == GM_STEP_NEXT) return (gm_cs_t)GM_CS_FAR;
     if (cs - kp >= d.horizon) return (gm_cs_t)GM_CS_FAR;
     return (gm_cs_t)(cs & (GM_CS_MOD - 1u));
 }
@@ -317,7 +328,11 @@ __global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A0) 
         const RecA a = rec_a(r);
         const RecC c = rec_c(r);
         double f = a.frac0;
-        if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, r->inc, A.k - a.s0);
+        if (cs != GM_CS_UNKNOWN) {
+            const double inc = r->inc;
+            /* one step since the anchor: the one addition itself */
+            f = A.k - a.s0 == 1u ? f + inc : gm_radd_n(f, inc, A.k - a.s0);
+        }
         const double P = resolved(d, c.P);
         if (is_pending(c.P)) r->P = P;
         predict_slot(d, A, s, f, A.k, P, c.defg, a.inf);
@@ -726,6 +741,10 @@ static_assert(GM_PG == 4 || GM_PG == 8 || GM_PG == 16, "lanes per parent");
 __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
     if (d.st->halt) return;
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
+    /* the step's parents retire into the log at [nret, nret + nfull) */
+    const unsigned long long nret = d.st->nret;
+    const bool logged = nret + nfull <= d.rlogCap;
+    if (!logged && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&d.st->err, GM_ERRF_RLOG_FULL);
     const uint32_t lg = threadIdx.x & (GM_PG - 1);
     const uint32_t ngroups = gridDim.x * (GM_BLOCK / GM_PG);
     for (uint32_t p = blockIdx.x * (GM_BLOCK / GM_PG) + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
@@ -743,6 +762,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
             *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
             *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
             *reinterpret_cast<uint4 *>(&o->x) = q2;
+            if (logged) d.rlog[nret + p] = make_uint2(cell, d.ndefg + rxg);
         }
         q2.x = (uint32_t)__shfl((int)q2.x, 0, GM_PG);
         q2.y = (uint32_t)__shfl((int)q2.y, 0, GM_PG);
@@ -946,10 +966,7 @@ __global__ void __launch_bounds__(64) k_commit(GmDev d, GmPParams pp, GmSweepArg
          * reported so the host can update rhomax (update_myrhomax, SCORE_RecoveryModels.cpp:146-165).
          * Consecutive parents often share their nucleus and deformed grain (siblings),
          * so each run of equal ids among neighbouring lanes makes one atomic. */
-        if (valid) {
-            d.grid[q0.y] = d.ndefg + q0.w;
-            d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
-        }
+        if (valid) d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
         const uint32_t g = q0.z, r = q0.w;
         const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(r, 1, 64);
         const uint64_t gstart = __ballot(valid && (lane == 0 || g != gprev));
@@ -1006,6 +1023,17 @@ hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepAr
     return hipGetLastError();
 }
 
+/* exported map: retired cells from the log, in retirement order (a cell retired
+ * twice keeps the later entry, as SCORE's grid keeps its last write) */
+__global__ void k_apply_rlog(GmDev d, uint32_t *out) {
+    const unsigned long long n = d.st->nret < d.rlogCap ? d.st->nret : d.rlogCap;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint2 e = d.rlog[i];
+        out[e.x] = e.y;
+    }
+}
+
 /* exported map: cells held by active front records are INFECTED */
 __global__ void k_overlay_infected(GmDev d, uint32_t nI, uint32_t nB, uint32_t *out) {
     const uint32_t n = nI + nB;
@@ -1016,7 +1044,8 @@ __global__ void k_overlay_infected(GmDev d, uint32_t nI, uint32_t nB, uint32_t *
 }
 
 hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s) {
-    if (nI + nB == 0) return hipSuccess;
+    k_apply_rlog<<<8192, GM_BLOCK, 0, s>>>(d, out);
+    if (nI + nB == 0) return hipGetLastError();
     k_overlay_infected<<<fill_blocks(nI + nB), GM_BLOCK, 0, s>>>(d, nI, nB, out);
     return hipGetLastError();
 }
@@ -1616,6 +1645,7 @@ static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmSt
     st->rows[j] = r;
     st->nexec = j + 1;
     st->kcur = st->kcur + 1u;
+    st->nret = st->nret + (unsigned long long)(r.nfullI + r.nfullB);
     const uint32_t npend = __hip_atomic_load(&st->npend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st->npend0 = npend;
     const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
