@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SCORE_GM_ABI_VERSION 10
+#define SCORE_GM_ABI_VERSION 11
 
 /* status codes */
 #define SCORE_GM_OK                 0
@@ -80,7 +80,16 @@ typedef struct score_gm_params {
     int32_t mobility_model;       /* SCORE_GM_MOB_* */
     int32_t zener_mode;           /* dispersoid drag: 0 none, 1 constant (first f/r of the table), 2 f/r linear in time */
     int32_t recovery_mode;        /* 0 none, 1 Nes vacancy-core diffusion, 2 Nes solute drag, 3 Michalak-Paxton */
-    int32_t reserved0;
+    /* OMP_NUM_THREADS of the SCORE run to reproduce (0 or 1: one region).  SCORE
+     * splits the domain into that many memory regions (ompshar_ca2regions,
+     * SCORE_GridPartitioning.cpp:69-166): npy = smallestYFactor(n) slabs along y
+     * times n / npy along z, each with its own Inside and Border lists; an
+     * infection that crosses into another region is parked in a halo and taken up
+     * by that region after the step (SCORE_Kernel.cpp:4494-4555, :4777-4872), so
+     * the outcome depends on n.  With n > 1 each region must be more than 10 cells
+     * thick along y and z (MINIMUM_REGION_SIZE, SCORE_GridPartitioning.cpp:93);
+     * at most 1024 regions. */
+    int32_t n_regions;
     /* Rollett-Holm (MobilityModel 2) */
     double rh_hagb_m0;            /* m^4/(J s) */
     double rh_hagb_hact;          /* J (eV * 1.602176565e-19, as SCORE_Kernel.cpp:579) */

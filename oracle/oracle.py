@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of the oracle (oracle/score_oracle.c).
 
-The oracle is a serial CPU restatement of SCORE's growth machine with
-OMP_NUM_THREADS=1 semantics.  tests/, __graft_entry__.smoke() and the
+The oracle is a serial CPU restatement of SCORE's growth machine with the
+semantics of a run with OMP_NUM_THREADS = params.n_regions (one region when 0
+or 1).  tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py compare the library with it; the library never uses it.
 """
 from __future__ import annotations
@@ -56,8 +57,12 @@ def load() -> C.CDLL:
         lib.oracle_get_status.restype = C.c_uint32
         lib.oracle_defragment.argtypes = [C.c_void_p]
         lib.oracle_defragment.restype = None
-        lib.oracle_capacities.argtypes = [C.c_void_p, u32p]
+        lib.oracle_capacities.argtypes = [C.c_void_p, C.c_uint32, u32p]
         lib.oracle_capacities.restype = None
+        lib.oracle_regions.argtypes = [C.c_void_p, u32p, u32p]
+        lib.oracle_regions.restype = C.c_uint32
+        lib.oracle_smallest_y_factor.argtypes = [C.c_uint32]
+        lib.oracle_smallest_y_factor.restype = C.c_uint32
         lib.oracle_disori_angle.argtypes = [f64p, f64p]
         lib.oracle_disori_angle.restype = C.c_double
         lib.oracle_disori_omega.argtypes = [f64p, f64p]
@@ -99,6 +104,10 @@ def disori_quat(p, q) -> np.ndarray:
     out = np.zeros(4)
     load().oracle_disori_quat(pa, pb, out.ctypes.data_as(C.POINTER(C.c_double)))
     return out
+
+
+def smallest_y_factor(n: int) -> int:
+    return load().oracle_smallest_y_factor(n)
 
 
 def mobility_weight(prm: ParamsSpec, qrx, qdef) -> float:
@@ -158,10 +167,18 @@ class Oracle:
         """Both lists compacted as ompshar_defragmentation does, outside the step loop."""
         self._lib.oracle_defragment(self._h)
 
-    def capacities(self) -> np.ndarray:
+    def capacities(self, region: int = 0) -> np.ndarray:
         out = np.zeros(4, np.uint32)
-        self._lib.oracle_capacities(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        self._lib.oracle_capacities(self._h, region, out.ctypes.data_as(C.POINTER(C.c_uint32)))
         return out
+
+    def regions(self):
+        """(npy, npz) and an (n_regions, 6) array of inclusive boxes lo x,y,z, hi x,y,z."""
+        n = self._lib.oracle_regions(self._h, None, None)
+        g, boxes = np.zeros(2, np.uint32), np.zeros((n, 6), np.uint32)
+        p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+        self._lib.oracle_regions(self._h, p(g), p(boxes))
+        return (int(g[0]), int(g[1])), boxes
 
     def grainmap(self, encoding: int = abi.MAP_CELLGRID) -> np.ndarray:
         out = np.empty(self.dom.N, np.uint32)

@@ -3,9 +3,13 @@
  *
  * A serial CPU restatement of SCORE's growth machine on one domain
  * (caHdl::solve_RXGROWTH, SCORE_Kernel.cpp:6497-6792) with the semantics of a
- * run with OMP_NUM_THREADS=1: one region spanning the whole domain, an Inside
- * and a Border front list, recycle-then-append slot assignment, first-come
- * infection in list order and fixed direction order.  The GPU library is tested
+ * run with OMP_NUM_THREADS = params.n_regions: the domain split into that many
+ * memory regions, each with an Inside and a Border front list, recycle-then-append
+ * slot assignment, first-come infection in list order and fixed direction order;
+ * an infection into another region is parked in a halo and taken up by that
+ * region after the step.  The regions run in parallel in SCORE but touch only
+ * their own cells, lists and halos between barriers, so one region after another
+ * here gives the same result.  The GPU library is tested
  * against it; it is never linked into the library, and only tests/, the smoke
  * entry point and the CPU-baseline leg of bench.py load it.
  *
@@ -57,6 +61,8 @@
 #define O_FULLRXCACHING_ATT_FACTOR (1.5)
 #define O_FULLRECYCLING_ATT_FACTOR (3.0)
 #define O_DEFAULT_NMAX (1000000)   /* DEFAULT_NMAX, SCORE_Defs_Physics.h:133 */
+#define O_MINIMUM_REGION_SIZE (10) /* SCORE_Defs_Functionality.h:85 */
+#define O_NO_HALO (0xFFFFFFFFu)
 #define O_FGEOFACE (1.0000)
 #define O_FGEOEDGE (0.7957)
 #define O_FGEODIAG (0.6494)
@@ -106,6 +112,36 @@ typedef struct {
     uint32_t nupdated;
 } o_list;
 
+/* halo of a region towards one neighbouring region (ompshar_init_haloregions,
+ * SCORE_Kernel.cpp:2223-2457; ompshar_initialize_halo, :2085-2127): the layer of
+ * the neighbour's cells next to this region; an infection this region directs at
+ * such a cell is parked in the halo, first come first served, and the neighbour
+ * applies it after the step (ompshar_synchronize_haloregions_rx, :4777-4872).
+ * A halo cell's status is never reset. */
+typedef struct {
+    uint32_t ext[3], org[3];   /* extent (all 0: no halo) and origin, global coordinates */
+    uint32_t inregion;         /* the region the halo lies in */
+    uint8_t *status;           /* infection parked here at some step */
+    uint8_t *inf;
+    int16_t *xyz;
+    double *carry;
+    uint32_t *rxg;
+    uint32_t *refs;            /* theHaloRefs of this step, nextFreeHaloRefs entries */
+    uint32_t nref;
+} o_halo;
+
+/* one memory region (ompshar_ca2regions, SCORE_GridPartitioning.cpp:69-166) with
+ * its own front lists, neighbour table and halos */
+typedef struct {
+    uint32_t lo[3], hi[3];     /* inclusive box */
+    o_list in, bd;
+    uint32_t nbor[27];         /* mynborregions ids in SCORE's order */
+    uint32_t nborhalo[27];     /* mynborhalos (ompshar_link_haloregions): halo index or O_NO_HALO */
+    o_halo halo[27];
+} o_region;
+
+uint32_t oracle_smallest_y_factor(uint32_t n);
+
 typedef struct oracle {
     uint32_t nx, ny, nz, nxy;
     uint64_t N;
@@ -120,7 +156,8 @@ typedef struct oracle {
     double *disp_t, *disp_fr;  /* EvoMicrochemistry: time, f/r */
     double fr;                 /* myDragData.fr */
     double tprocessingend;
-    o_list in, bd;
+    o_region *reg;             /* the regions of OMP_NUM_THREADS = nreg threads */
+    uint32_t nreg, npy, npz;
     /* caHdl scalars */
     double t, X, Xcells, dt, T, G, b, Gbhalfsq;
     double mLAGB, mHAGB, mGS, mRHHAGB, pzmin, rhomax, Pmax, reg_Pmax;
@@ -536,9 +573,69 @@ static double o_pair_weight(oracle *o, uint32_t rxg, uint32_t defg) {
     return oracle_mobility_weight(&o->p, &o->rxq[4 * (size_t)rxg], &o->defq[4 * (size_t)defg]);
 }
 
-/* one attempt; omp_infect_OutOfRXFrontInside (SCORE_Kernel.cpp:4252-4344) when from_border == 0
- * and omp_infect_OutOfRXFrontBorder (:4347-4556) restricted to one region when from_border == 1. */
-static int o_infect(oracle *o, o_list *src, uint32_t pslot, int from_border, int intoborder, const o_dir *d,
+/* neighbour directions in the order of mynborregions (ompshar_init_neighbortopology,
+ * SCORE_Kernel.cpp:2028-2080): myself, six faces, the x-z, x-y and y-z edges, the
+ * corners; o_nb_opp[nb] is the opposite direction (the halo candidate that
+ * ompshar_link_haloregions looks up, SCORE_GridPartitioning.cpp:237-276) */
+static const int8_t o_nb_dir[27][3] = {
+    {0, 0, 0},
+    {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1},
+    {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+    {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 1, 0},
+    {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1},
+    {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1}};
+static const uint8_t o_nb_opp[27] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15,
+                                     26, 25, 24, 23, 22, 21, 20, 19};
+
+static int o_in_box(const uint32_t *lo, const uint32_t *hi, int x, int y, int z) {
+    return x >= (int)lo[0] && x <= (int)hi[0] && y >= (int)lo[1] && y <= (int)hi[1] && z >= (int)lo[2] && z <= (int)hi[2];
+}
+
+static int o_strictly_inside(const o_region *R, int x, int y, int z) {
+    return x > (int)R->lo[0] && x < (int)R->hi[0] && y > (int)R->lo[1] && y < (int)R->hi[1] && z > (int)R->lo[2] &&
+           z < (int)R->hi[2];
+}
+
+/* a new front record in list dst for the deformed cell (x,y,z) of grain st */
+static int o_claim_into(oracle *o, o_list *dst, const o_cell *par, uint8_t code, int x, int y, int z, double carry,
+                        uint32_t st, double Pn) {
+    uint64_t cxyz = (uint64_t)x + (uint64_t)y * o->nx + (uint64_t)z * o->nxy;
+    o->grid[cxyz] = O_CURRENTLY_INFECTED;
+    uint32_t slot = o_next_free(dst, 1);
+    if (slot == O_INVALID_ADDRESS) return SCORE_GM_ERR_CAPACITY;
+    if (Pn >= o->reg_Pmax) o->reg_Pmax = Pn;
+    o_write_cell(&dst->c[slot], par, code, x, y, z, carry, st, Pn);
+    o->last_infected++;
+    return 0;
+}
+
+/* an infection of region R directed at (x,y,z) in region nbortid: parked in R's
+ * halo towards it unless that halo cell was used before (SCORE_Kernel.cpp:4494-4555) */
+static void o_halo_attempt(o_region *R, uint32_t nbortid, int x, int y, int z, uint8_t code, double carry, uint32_t rxg) {
+    for (uint32_t h = 1; h < 27; h++) {
+        o_halo *H = &R->halo[h];
+        if (H->inregion != nbortid || !H->status) continue;
+        /* the reference's bounds reach one cell beyond the halo (:4506-4511) */
+        if (x < (int)H->org[0] || x > (int)(H->org[0] + H->ext[0]) || y < (int)H->org[1] ||
+            y > (int)(H->org[1] + H->ext[1]) || z < (int)H->org[2] || z > (int)(H->org[2] + H->ext[2]))
+            continue;
+        const uint32_t hx = (uint32_t)x - H->org[0], hy = (uint32_t)y - H->org[1], hz = (uint32_t)z - H->org[2];
+        const uint32_t hid = hx + hy * H->ext[0] + hz * H->ext[0] * H->ext[1];
+        if (H->status[hid]) return;
+        H->status[hid] = 1;
+        H->inf[hid] = code;
+        H->xyz[3 * (size_t)hid] = (int16_t)x; H->xyz[3 * (size_t)hid + 1] = (int16_t)y; H->xyz[3 * (size_t)hid + 2] = (int16_t)z;
+        H->carry[hid] = carry;
+        H->rxg[hid] = rxg;
+        H->refs[H->nref++] = hid;
+        return;
+    }
+}
+
+/* one attempt; omp_infect_OutOfRXFrontInside (SCORE_Kernel.cpp:4252-4344) when
+ * from_border == 0 and omp_infect_OutOfRXFrontBorder (:4347-4556) when 1, for
+ * region R */
+static int o_infect(oracle *o, o_region *R, o_list *src, uint32_t pslot, int from_border, int intoborder, const o_dir *d,
                     double carry) {
     const o_cell par = src->c[pslot];
     int x = par.x + d->dx, y = par.y + d->dy, z = par.z + d->dz;
@@ -550,29 +647,35 @@ static int o_infect(oracle *o, o_list *src, uint32_t pslot, int from_border, int
         if (y >= (int)o->ny) y -= (int)o->ny;
         if (z < 0) return 0;
         if (z >= (int)o->nz) return 0;
-        int inside = (x > 0 && x < (int)o->nx - 1 && y > 0 && y < (int)o->ny - 1 && z > 0 && z < (int)o->nz - 1);
-        dst = inside ? &o->in : &o->bd;
+        if (o_strictly_inside(R, x, y, z)) {
+            dst = &R->in;
+        } else {
+            /* the region holding the target, neighbours in mynborregions order (:4438-4447) */
+            uint32_t nbortid = O_NO_HALO;
+            for (uint32_t nb = 0; nb < 27; nb++) {
+                const o_region *Q = &o->reg[R->nbor[nb]];
+                if (o_in_box(Q->lo, Q->hi, x, y, z)) { nbortid = R->nbor[nb]; break; }
+            }
+            if (nbortid != R->nbor[0]) {
+                o_halo_attempt(R, nbortid, x, y, z, (uint8_t)d->code, carry, par.rxg);
+                return 0;
+            }
+            dst = &R->bd;
+        }
     } else {
-        dst = intoborder ? &o->bd : &o->in;
+        dst = intoborder ? &R->bd : &R->in;
     }
     uint64_t cxyz = (uint64_t)x + (uint64_t)y * o->nx + (uint64_t)z * o->nxy;
     uint32_t st = o->grid[cxyz];
     if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) return 0;
-    o->grid[cxyz] = O_CURRENTLY_INFECTED;
-    uint32_t slot = o_next_free(dst, 1);
-    if (slot == O_INVALID_ADDRESS) return SCORE_GM_ERR_CAPACITY;
     double Pn = par.P;
     if (par.defg != st) Pn = o_pair_weight(o, par.rxg, st);
-    if (Pn >= o->reg_Pmax) o->reg_Pmax = Pn;
-    o_write_cell(&dst->c[slot], &par, (uint8_t)d->code, x, y, z, carry, st, Pn);
-    o->last_infected++;
-    return 0;
+    return o_claim_into(o, dst, &par, (uint8_t)d->code, x, y, z, carry, st, Pn);
 }
 
 /* ompshar_sim_myCA_updateFullRXInside / Border (SCORE_Kernel.cpp:4559-4664, :4678-4774) */
-static int o_update_fullrx(oracle *o, o_list *L, int from_border) {
+static int o_update_fullrx(oracle *o, o_region *R, o_list *L, int from_border) {
     const double overFac1 = 0.0682, overFac2 = 0.0542, overFac3 = 0.0484;
-    uint32_t xmx = o->nx - 1, ymx = o->ny - 1, zmx = o->nz - 1;
     uint32_t nupd = 0;
     for (uint32_t k = 0; k < L->full_n; k++) {
         uint32_t c = L->full[k];
@@ -585,12 +688,12 @@ static int o_update_fullrx(oracle *o, o_list *L, int from_border) {
         int intoborder = 1;
         if (!from_border) {
             int x = L->c[c].x, y = L->c[c].y, z = L->c[c].z;
-            if (x > (int)(0 + 1) && x < (int)(xmx - 1) && y > (int)(0 + 1) && y < (int)(ymx - 1) && z > (int)(0 + 1) &&
-                z < (int)(zmx - 1))
+            if (x > (int)(R->lo[0] + 1) && x < (int)(R->hi[0] - 1) && y > (int)(R->lo[1] + 1) && y < (int)(R->hi[1] - 1) &&
+                z > (int)(R->lo[2] + 1) && z < (int)(R->hi[2] - 1))
                 intoborder = 0;
         }
         for (int dd = 0; dd < 26; dd++) {
-            int rc = o_infect(o, L, c, from_border, intoborder, &o_dirs[dd], carry[o_dirs[dd].carry]);
+            int rc = o_infect(o, R, L, c, from_border, intoborder, &o_dirs[dd], carry[o_dirs[dd].carry]);
             if (rc) return rc;
         }
         uint64_t cxyz = (uint64_t)L->c[c].x + (uint64_t)L->c[c].y * o->nx + (uint64_t)L->c[c].z * o->nxy;
@@ -602,6 +705,34 @@ static int o_update_fullrx(oracle *o, o_list *L, int from_border) {
         nupd++;
     }
     L->nupdated = nupd;
+    return 0;
+}
+
+/* ompshar_synchronize_haloregions_rx (SCORE_Kernel.cpp:4777-4872): region R applies
+ * the infections its neighbours parked in their halos towards it, neighbour by
+ * neighbour in its mynborregions order, each halo in the order of its refs, into
+ * its Border list, with the weight recomputed; a cell no longer deformed is skipped */
+static int o_sync_halos(oracle *o, uint32_t r) {
+    o_region *R = &o->reg[r];
+    for (uint32_t nb = 0; nb < 27; nb++) {
+        o_region *A = &o->reg[R->nbor[nb]];
+        const uint32_t hal = A->nborhalo[nb];
+        if (hal == O_NO_HALO) continue;
+        const o_halo *H = &A->halo[hal];
+        for (uint32_t s2 = 0; s2 < H->nref; s2++) {
+            const uint32_t hid = H->refs[s2];
+            const int x = H->xyz[3 * (size_t)hid], y = H->xyz[3 * (size_t)hid + 1], z = H->xyz[3 * (size_t)hid + 2];
+            uint64_t cxyz = (uint64_t)x + (uint64_t)y * o->nx + (uint64_t)z * o->nxy;
+            uint32_t st = o->grid[cxyz];
+            if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) continue;
+            o_cell par;
+            memset(&par, 0, sizeof(par));
+            par.rxg = H->rxg[hid];
+            const double Pn = o_pair_weight(o, H->rxg[hid], st);
+            int rc = o_claim_into(o, &R->bd, &par, H->inf[hid], x, y, z, H->carry[hid], st, Pn);
+            if (rc) return rc;
+        }
+    }
     return 0;
 }
 
@@ -637,6 +768,13 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
         }
     }
     if (p->recovery_mode < 0 || p->recovery_mode > 4) return SCORE_GM_ERR_ARG;
+    if (p->n_regions < 0 || p->n_regions > 1024) return SCORE_GM_ERR_ARG;
+    if (p->n_regions > 1) {
+        /* every region more than MINIMUM_REGION_SIZE cells thick (SCORE_GridPartitioning.cpp:87-93) */
+        const uint32_t npy = oracle_smallest_y_factor((uint32_t)p->n_regions);
+        const uint32_t npz = (uint32_t)p->n_regions / npy;
+        if (d->ny / npy <= O_MINIMUM_REGION_SIZE || d->nz / npz <= O_MINIMUM_REGION_SIZE) return SCORE_GM_ERR_ARG;
+    }
     if (p->recovery_mode == 4 && !(p->rec_tau > 0.0)) return SCORE_GM_ERR_ARG;
     if (p->n_grainlog && !p->grainlog_at) return SCORE_GM_ERR_ARG;
     if (p->n_sched < 2 || !p->sched_time || !p->sched_temp) return SCORE_GM_ERR_ARG;
@@ -653,12 +791,117 @@ static int o_validate(const score_gm_domain *d, const score_gm_params *p) {
     return 0;
 }
 
+/* mathMethods::smallestYFactor (SCORE_Math.cpp:97-115): the y factor of the
+ * region grid, the last of the listed primes that divides n before the next
+ * prime reaches the cofactor */
+uint32_t oracle_smallest_y_factor(uint32_t n) {
+    static const uint32_t primes[32] = {1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47,
+                                        53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
+    uint32_t z = n, y = 1;
+    for (uint32_t c = 0; c < 31; c++) {
+        const uint32_t q = n / primes[c];
+        if (q * primes[c] == n) { z = q; y = primes[c]; }
+        if (primes[c + 1] >= z) break;
+    }
+    return y;
+}
+
+static void o_regions_free(oracle *o) {
+    if (!o->reg) return;
+    for (uint32_t r = 0; r < o->nreg; r++) {
+        o_region *R = &o->reg[r];
+        o_list_free(&R->in);
+        o_list_free(&R->bd);
+        for (uint32_t h = 0; h < 27; h++) {
+            o_halo *H = &R->halo[h];
+            free(H->status); free(H->inf); free(H->xyz); free(H->carry); free(H->rxg); free(H->refs);
+        }
+    }
+    free(o->reg);
+    o->reg = NULL;
+}
+
+/* the region grid of ompshar_ca2regions (SCORE_GridPartitioning.cpp:69-166):
+ * npy = smallestYFactor(T) slabs along y, npz = T / npy along z, x not split;
+ * region z*npy + y spans [y*nry, y*nry + nry - 1] (the last one up to ny - 1)
+ * and likewise in z.  Then its neighbour table (ompshar_init_neighbortopology,
+ * SCORE_Kernel.cpp:2028-2080; get_nbortid, :2002-2026, periodic on the region
+ * grid), its halos (ompshar_init_haloregions, :2223-2457; determine_origins,
+ * :2134-2220) and the halo each neighbour keeps towards it
+ * (ompshar_link_haloregions, SCORE_GridPartitioning.cpp:222-276). */
+static int o_regions_init(oracle *o, uint32_t nreg) {
+    o->nreg = nreg;
+    o->npy = oracle_smallest_y_factor(nreg);
+    o->npz = nreg / o->npy;
+    o->reg = (o_region *)calloc(nreg, sizeof(o_region));
+    if (!o->reg) return SCORE_GM_ERR_OOM;
+    const uint32_t npy = o->npy, npz = o->npz;
+    const uint32_t nry = o->ny / npy, nrz = o->nz / npz;
+    for (uint32_t iz = 0; iz < npz; iz++)
+        for (uint32_t iy = 0; iy < npy; iy++) {
+            o_region *R = &o->reg[iz * npy + iy];
+            R->lo[0] = 0; R->hi[0] = o->nx - 1;
+            R->lo[1] = iy * nry; R->hi[1] = iy * nry + nry - 1;
+            R->lo[2] = iz * nrz; R->hi[2] = iz * nrz + nrz - 1;
+            if (iy == npy - 1) R->hi[1] = o->ny - 1;
+            if (iz == npz - 1) R->hi[2] = o->nz - 1;
+        }
+    for (uint32_t r = 0; r < nreg; r++) {
+        o_region *R = &o->reg[r];
+        const int iy = (int)(r % npy), iz = (int)(r / npy);
+        uint32_t ext[3];
+        for (int a = 0; a < 3; a++) ext[a] = R->hi[a] - R->lo[a] + 1;
+        for (uint32_t nb = 0; nb < 27; nb++) {
+            int jy = iy + o_nb_dir[nb][1], jz = iz + o_nb_dir[nb][2];
+            if (jy < 0) jy += (int)npy;
+            if (jy >= (int)npy) jy -= (int)npy;
+            if (jz < 0) jz += (int)npz;
+            if (jz >= (int)npz) jz -= (int)npz;
+            R->nbor[nb] = (uint32_t)jz * npy + (uint32_t)jy;
+        }
+        for (uint32_t h = 0; h < 27; h++) {
+            o_halo *H = &R->halo[h];
+            H->inregion = R->nbor[h];
+            if (h == 0 || R->nbor[h] == r) continue; /* a dummy: extent 0 */
+            const uint32_t dims[3] = {o->nx, o->ny, o->nz};
+            for (int a = 0; a < 3; a++) {
+                const int d = o_nb_dir[h][a];
+                int org;
+                if (d == -1) { org = (int)R->lo[a] - 1; if (org < 0) org += (int)dims[a]; }
+                else if (d == 0) org = (int)R->lo[a];
+                else { org = (int)R->hi[a] + 1; if (org >= (int)dims[a]) org -= (int)dims[a]; }
+                H->org[a] = (uint32_t)org;
+                H->ext[a] = d == 0 ? ext[a] : 1;
+            }
+            const size_t n = (size_t)H->ext[0] * H->ext[1] * H->ext[2];
+            H->status = (uint8_t *)calloc(n, 1);
+            H->inf = (uint8_t *)malloc(n);
+            H->xyz = (int16_t *)malloc(3 * n * sizeof(int16_t));
+            H->carry = (double *)malloc(n * sizeof(double));
+            H->rxg = (uint32_t *)malloc(n * sizeof(uint32_t));
+            H->refs = (uint32_t *)malloc(n * sizeof(uint32_t));
+            if (!H->status || !H->inf || !H->xyz || !H->carry || !H->rxg || !H->refs) return SCORE_GM_ERR_OOM;
+        }
+    }
+    /* region R + d keeps its halo opp(d) towards R whenever d moves off R on the
+     * region grid (exists_nborhalo, SCORE_GridPartitioning.cpp:187-219) */
+    for (uint32_t r = 0; r < nreg; r++) {
+        o_region *R = &o->reg[r];
+        R->nborhalo[0] = O_NO_HALO;
+        for (uint32_t nb = 1; nb < 27; nb++) {
+            const o_region *Q = &o->reg[R->nbor[nb]];
+            R->nborhalo[nb] = Q->halo[o_nb_opp[nb]].status ? o_nb_opp[nb] : O_NO_HALO;
+        }
+    }
+    return 0;
+}
+
 void oracle_destroy(oracle *o) {
     if (!o) return;
     free(o->grid); free(o->defq); free(o->rxq); free(o->rho); free(o->rho0); free(o->site);
     free(o->defcnt); free(o->rxcnt); free(o->sched_t); free(o->sched_T); free(o->defrag_at);
     free(o->disp_t); free(o->disp_fr);
-    o_list_free(&o->in); o_list_free(&o->bd);
+    o_regions_free(o);
     free(o->traj);
     free(o->arr_ids); free(o->arr_off);
     free(o->glog_at); free(o->glog); free(o->glog_counts);
@@ -676,12 +919,19 @@ static int o_juvenile_nucleation(oracle *o, uint32_t nuc, int *placed) {
     uint32_t rem = w - (o->nxy * z);
     uint32_t y = rem / o->nx;
     uint32_t x = rem - (o->nx * y);
-    int inside = (x > 0 && x < o->nx - 1 && y > 0 && y < o->ny - 1 && z > 0 && z < o->nz - 1);
+    /* the region whose box holds the site (ompshar_sim_myCA_sitesaturatedNucleation,
+     * SCORE_GrowthMachine.cpp:75-100) */
+    const uint32_t ry = o->ny / o->npy, rz = o->nz / o->npz;
+    uint32_t iy = y / ry, iz = z / rz;
+    if (iy >= o->npy) iy = o->npy - 1;
+    if (iz >= o->npz) iz = o->npz - 1;
+    o_region *R = &o->reg[iz * o->npy + iy];
+    int inside = (x > R->lo[0] && x < R->hi[0] && y > R->lo[1] && y < R->hi[1] && z > R->lo[2] && z < R->hi[2]);
     uint32_t st = o->grid[w];
     if (st >= o->n_defg || st == O_CURRENTLY_INFECTED) return 0;
     o->grid[w] = O_CURRENTLY_INFECTED;
     double PP = o_pair_weight(o, nuc, st);
-    o_list *L = inside ? &o->in : &o->bd;
+    o_list *L = inside ? &R->in : &R->bd;
     uint32_t slot = o_next_free(L, 0);
     if (slot == O_INVALID_ADDRESS) return SCORE_GM_ERR_CAPACITY;
     o_cell *e = &L->c[slot];
@@ -786,16 +1036,23 @@ int oracle_create(const score_gm_domain *d, const score_gm_params *p, oracle **o
     o_update_mobilities(o);
     o->dt = o_dt_minimum(o);
 
-    /* ompshar_init_myRXFrontInside / Border (SCORE_Kernel.cpp:3587-3710) */
-    uint32_t nin = (uint32_t)(p->rel_cell_caching * (double)o->N);
-    uint32_t nin_rec = (uint32_t)(p->rel_cell_caching * (double)o->N * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
-    uint32_t nin_full = (uint32_t)(p->rel_cell_caching * (double)o->N * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
-    uint32_t nbd = 2 * ((o->nx * o->ny) + (o->nx * o->nz) + (o->ny * o->nz));
-    uint32_t nbd_rec = (uint32_t)((double)nbd * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
-    uint32_t nbd_full = (uint32_t)((double)nbd * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
-    if (o_list_init(&o->in, nin, nin_rec, nin_full, 1) || o_list_init(&o->bd, nbd, nbd_rec, nbd_full, 0)) {
-        oracle_destroy(o);
-        return SCORE_GM_ERR_OOM;
+    if ((rc = o_regions_init(o, p->n_regions > 1 ? (uint32_t)p->n_regions : 1u))) { oracle_destroy(o); return rc; }
+    /* ompshar_init_myRXFrontInside / Border (SCORE_Kernel.cpp:3587-3710), per region
+     * from its own volume and faces */
+    for (uint32_t r = 0; r < o->nreg; r++) {
+        o_region *R = &o->reg[r];
+        const uint32_t ex = R->hi[0] - R->lo[0] + 1, ey = R->hi[1] - R->lo[1] + 1, ez = R->hi[2] - R->lo[2] + 1;
+        const double vol = (double)(ex * ey * ez);
+        uint32_t nin = (uint32_t)(p->rel_cell_caching * vol);
+        uint32_t nin_rec = (uint32_t)(p->rel_cell_caching * vol * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
+        uint32_t nin_full = (uint32_t)(p->rel_cell_caching * vol * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
+        uint32_t nbd = 2 * ((ex * ey) + (ex * ez) + (ey * ez));
+        uint32_t nbd_rec = (uint32_t)((double)nbd * p->maxfill * O_FULLRECYCLING_ATT_FACTOR);
+        uint32_t nbd_full = (uint32_t)((double)nbd * p->maxfill * O_FULLRXCACHING_ATT_FACTOR);
+        if (o_list_init(&R->in, nin, nin_rec, nin_full, 1) || o_list_init(&R->bd, nbd, nbd_rec, nbd_full, 0)) {
+            oracle_destroy(o);
+            return SCORE_GM_ERR_OOM;
+        }
     }
 
     /* ompshar_sim_myCA_sitesaturatedNucleation + infect_JuvenileNucleation (SCORE_GrowthMachine.cpp:50-189):
@@ -868,8 +1125,10 @@ int oracle_step(oracle *o) {
     if (o->completed) return 1;
     if (o_done(o)) { o->completed = 1; return 1; }
     if (o->defrag_cnt < o->p.n_defrag && o->Xcells >= o->defrag_at[o->defrag_cnt]) {
-        o_defrag_list(&o->in);
-        o_defrag_list(&o->bd);
+        for (uint32_t q = 0; q < o->nreg; q++) {
+            o_defrag_list(&o->reg[q].in);
+            o_defrag_list(&o->reg[q].bd);
+        }
         o->defrag_cnt++;
     }
     int rc;
@@ -885,9 +1144,29 @@ int oracle_step(oracle *o) {
             nplaced += (uint32_t)placed;
         }
     }
-    if ((rc = o_sweep(o, &o->in)) || (rc = o_sweep(o, &o->bd))) { o->err = rc; return rc; }
+    /* the regions run their growth step side by side and touch only their own
+     * cells, lists and halos until the barrier (SCORE_Kernel.cpp:6634-6648), so
+     * taking them one after another gives the same result */
+    for (uint32_t q = 0; q < o->nreg; q++)
+        if ((rc = o_sweep(o, &o->reg[q].in)) || (rc = o_sweep(o, &o->reg[q].bd))) { o->err = rc; return rc; }
     o->last_infected = 0;
-    if ((rc = o_update_fullrx(o, &o->in, 0)) || (rc = o_update_fullrx(o, &o->bd, 1))) { o->err = rc; return rc; }
+    for (uint32_t q = 0; q < o->nreg; q++) {
+        o_region *R = &o->reg[q];
+        if ((rc = o_update_fullrx(o, R, &R->in, 0))) { o->err = rc; return rc; }
+        for (uint32_t h = 0; h < 27; h++) R->halo[h].nref = 0; /* ompshar_clear_halo_bookkeeping (:4667-4675) */
+        if ((rc = o_update_fullrx(o, R, &R->bd, 1))) { o->err = rc; return rc; }
+    }
+    /* after the barrier every region takes up what its neighbours parked for it (:6654) */
+    if (o->nreg > 1)
+        for (uint32_t q = 0; q < o->nreg; q++)
+            if ((rc = o_sync_halos(o, q))) { o->err = rc; return rc; }
+    uint32_t sv = 0, nfi = 0, nfb = 0, nsi = 0, nsb = 0;
+    for (uint32_t q = 0; q < o->nreg; q++) {
+        const o_region *R = &o->reg[q];
+        sv += R->in.sv + R->bd.sv;
+        nfi += R->in.nupdated; nfb += R->bd.nupdated;
+        nsi += R->in.nnever; nsb += R->bd.nnever;
+    }
     score_gm_traj_row r;
     memset(&r, 0, sizeof(r));
     r.step = o->step;
@@ -895,19 +1174,19 @@ int oracle_step(oracle *o) {
     r.dt = o->dt;
     r.temperature = o->T;
     /* master section (SCORE_Kernel.cpp:6688-6742) */
-    uint32_t sv = o->in.sv + o->bd.sv;
     double mymobmax = O_DEFAULT_PMAX;
     if (o->reg_Pmax >= mymobmax) mymobmax = o->reg_Pmax;
     o->Pmax = mymobmax;
-    o->Xcells = o->Xcells + ((double)(o->in.nupdated + o->bd.nupdated));
+    for (uint32_t q = 0; q < o->nreg; q++)
+        o->Xcells = o->Xcells + ((double)(o->reg[q].in.nupdated + o->reg[q].bd.nupdated));
     o->X = (o->Xcells / ((double)o->N));
     o->sum_sv += sv;
     o_update_rhomax(o);
     r.sv = sv;
-    r.nfull_inside = o->in.nupdated;
-    r.nfull_border = o->bd.nupdated;
-    r.nslots_inside = o->in.nnever;
-    r.nslots_border = o->bd.nnever;
+    r.nfull_inside = nfi;
+    r.nfull_border = nfb;
+    r.nslots_inside = nsi;
+    r.nslots_border = nsb;
     r.ninfected = o->last_infected;
     r.X = o->X;
     r.xcells = o->Xcells;
@@ -981,7 +1260,10 @@ size_t oracle_get_grainlog(const oracle *o, score_gm_grainlog_row *rows, uint32_
 
 int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8_t *act, uint8_t *inf, uint32_t *cell,
                      double *frac, double *P, uint32_t *defg, uint32_t *rxg) {
-    const o_list *L = list == SCORE_GM_LIST_BORDER ? &o->bd : &o->in;
+    /* list 2r + SCORE_GM_LIST_* is the Inside or Border list of region r */
+    if (list < 0 || (uint32_t)list >= 2 * o->nreg) return SCORE_GM_ERR_ARG;
+    const o_region *R = &o->reg[list / 2];
+    const o_list *L = (list & 1) == SCORE_GM_LIST_BORDER ? &R->bd : &R->in;
     *n = L->nnever;
     uint32_t m = L->nnever < cap ? L->nnever : cap;
     for (uint32_t i = 0; i < m; i++) {
@@ -998,12 +1280,12 @@ int oracle_get_front(const oracle *o, int list, uint32_t cap, uint32_t *n, uint8
 }
 
 /* ompcrit_clarify_status_infectedcells (SCORE_Io_Characterization.cpp:925-997):
- * ACTIVE records of the Inside then the Border list, in slot order; returns the count */
+ * region by region, the ACTIVE records of its Inside then its Border list, in
+ * slot order; returns the count */
 uint32_t oracle_get_status(const oracle *o, double thr, uint32_t cap, uint32_t *ixyz, uint32_t *gid) {
     uint32_t n = 0;
-    const o_list *lists[2] = {&o->in, &o->bd};
-    for (int l = 0; l < 2; l++) {
-        const o_list *L = lists[l];
+    for (uint32_t l = 0; l < 2 * o->nreg; l++) {
+        const o_list *L = (l & 1) ? &o->reg[l / 2].bd : &o->reg[l / 2].in;
         for (uint32_t c = 0; c < L->nnever; c++) {
             const o_cell *e = &L->c[c];
             if (!e->act) continue;
@@ -1019,12 +1301,24 @@ uint32_t oracle_get_status(const oracle *o, double thr, uint32_t cap, uint32_t *
 
 /* ompshar_defragmentation of both lists outside the step loop (a test hook) */
 void oracle_defragment(oracle *o) {
-    o_defrag_list(&o->in);
-    o_defrag_list(&o->bd);
+    for (uint32_t q = 0; q < o->nreg; q++) {
+        o_defrag_list(&o->reg[q].in);
+        o_defrag_list(&o->reg[q].bd);
+    }
 }
 
-/* list capacities the reference derives (SCORE_Kernel.cpp:3589,3615,3631,3679,3682,3697):
- * inside, inside recycle cap, border, border recycle cap */
-void oracle_capacities(const oracle *o, uint32_t *out4) {
-    out4[0] = o->in.ntotal; out4[1] = o->in.rec_cap; out4[2] = o->bd.ntotal; out4[3] = o->bd.rec_cap;
+/* list capacities the reference derives (SCORE_Kernel.cpp:3589,3615,3631,3679,3682,3697)
+ * for region r: inside, inside recycle cap, border, border recycle cap */
+void oracle_capacities(const oracle *o, uint32_t r, uint32_t *out4) {
+    const o_region *R = &o->reg[r < o->nreg ? r : 0];
+    out4[0] = R->in.ntotal; out4[1] = R->in.rec_cap; out4[2] = R->bd.ntotal; out4[3] = R->bd.rec_cap;
+}
+
+/* the region grid: npy, npz, then per region its inclusive box lo x,y,z, hi x,y,z */
+uint32_t oracle_regions(const oracle *o, uint32_t *npy_npz, uint32_t *boxes) {
+    if (npy_npz) { npy_npz[0] = o->npy; npy_npz[1] = o->npz; }
+    if (boxes)
+        for (uint32_t r = 0; r < o->nreg; r++)
+            for (int a = 0; a < 3; a++) { boxes[6 * r + a] = o->reg[r].lo[a]; boxes[6 * r + 3 + a] = o->reg[r].hi[a]; }
+    return o->nreg;
 }

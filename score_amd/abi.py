@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscore_gm.so")
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 OK = 0
 ERR_ARG = -1
@@ -86,7 +86,7 @@ class Domain(C.Structure):
 class Params(C.Structure):
     _fields_ = [
         ("mobility_model", C.c_int32), ("zener_mode", C.c_int32),
-        ("recovery_mode", C.c_int32), ("reserved0", C.c_int32),
+        ("recovery_mode", C.c_int32), ("n_regions", C.c_int32),
         ("rh_hagb_m0", C.c_double), ("rh_hagb_hact", C.c_double),
         ("rh_cut", C.c_double), ("rh_trans", C.c_double), ("rh_exponent", C.c_double),
         ("lagb_m0", C.c_double), ("lagb_hact", C.c_double),

@@ -332,6 +332,24 @@ void free_device(score_gm_handle *h) {
     h->pool = nullptr;
 }
 
+/* mathMethods::smallestYFactor (SCORE_Math.cpp:97-115): the y factor of the region
+ * grid, the last of the listed primes that divides n before the next prime
+ * reaches the cofactor */
+uint32_t smallest_y_factor(uint32_t n) {
+    static const uint32_t primes[32] = {1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47,
+                                        53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127};
+    uint32_t z = n, y = 1;
+    for (uint32_t c = 0; c < 31; c++) {
+        const uint32_t q = n / primes[c];
+        if (q * primes[c] == n) {
+            z = q;
+            y = primes[c];
+        }
+        if (primes[c + 1] >= z) break;
+    }
+    return y;
+}
+
 int validate(const score_gm_domain *dom, const score_gm_params *prm) {
     if (!dom || !prm) return fail(SCORE_GM_ERR_ARG, "null domain or params");
     if (dom->nx < 3 || dom->ny < 3 || dom->nz < 3) return fail(SCORE_GM_ERR_ARG, "edge length below 3 cells");
@@ -361,6 +379,14 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
     if (prm->recovery_mode == 4 && !(prm->rec_tau > 0.0))
         return fail(SCORE_GM_ERR_ARG, "exponential recovery needs a positive time constant");
     if (prm->n_grainlog != 0 && !prm->grainlog_at) return fail(SCORE_GM_ERR_ARG, "grain-evolution log points missing");
+    if (prm->n_regions < 0 || prm->n_regions > 1024) return fail(SCORE_GM_ERR_ARG, "n_regions outside [0, 1024]");
+    if (prm->n_regions > 1) {
+        /* every region more than MINIMUM_REGION_SIZE cells thick (SCORE_GridPartitioning.cpp:87-93) */
+        const uint32_t npy = smallest_y_factor((uint32_t)prm->n_regions), npz = (uint32_t)prm->n_regions / npy;
+        if (dom->ny / npy <= 10 || dom->nz / npz <= 10)
+            return fail(SCORE_GM_ERR_ARG, "regions must be more than 10 cells thick along y and z");
+        return fail(SCORE_GM_ERR_UNSUPPORTED, "more than one memory region");
+    }
     if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
         return fail(SCORE_GM_ERR_ARG, "annealing schedule needs at least two points");

@@ -236,6 +236,7 @@ class ParamsSpec:
     rec_c5: float = 0.65
     rec_tau: float = 0.0                # s, exponential recovery (recovery_mode 4)
     grainlog_at: tuple = ()             # grain-evolution log thresholds, in cells
+    n_regions: int = 0                  # OMP_NUM_THREADS of the SCORE run to reproduce (0/1: one region)
 
     def to_c(self):
         keep = [np.ascontiguousarray(self.sched_time, np.float64), np.ascontiguousarray(self.sched_temp, np.float64),
@@ -247,6 +248,7 @@ class ParamsSpec:
         p.mobility_model = self.mobility_model
         p.zener_mode = self.zener_mode
         p.recovery_mode = self.recovery_mode
+        p.n_regions = self.n_regions
         p.rh_hagb_m0 = self.rh_hagb_m0
         p.rh_hagb_hact = self.rh_hagb_hact_ev * ECHARGE
         p.rh_cut, p.rh_trans, p.rh_exponent = self.rh_cut, self.rh_trans, self.rh_exponent
