@@ -27,6 +27,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import hashlib
 import json
 import os
@@ -280,6 +281,35 @@ def rolled_sheet_section(seed: int, reps: int = 1) -> dict:
     }
 
 
+def regions_section(dom, prm, T: int, reps: int = 1) -> dict:
+    """The headline workload with SCORE's OMP_NUM_THREADS = T semantics (T memory
+    regions, halo precedence): one untimed run, then `reps` timed ones; the grain
+    map must repeat.  It differs from the one-region map, as SCORE's does."""
+    from score_amd.growth import GrowthMachine
+    prmT = dataclasses.replace(prm, n_regions=T)
+    out, digest = [], None
+    for rep in range(reps + 1):
+        gm = GrowthMachine(dom, prmT)
+        t1 = time.perf_counter()
+        res = gm.solve_rxgrowth()
+        t2 = time.perf_counter()
+        h = hashlib.sha256(gm.grainmap().tobytes()).hexdigest()[:16]
+        gm.close()
+        if digest is None:
+            digest = h
+        elif digest != h:
+            raise RuntimeError(f"grain map at {T} regions differs between repetitions")
+        log(f"{T} regions {'run' if rep else 'warmup'}: {res['step']} CA steps, X={res['X']:.6f}, "
+            f"sum Sv={res['sum_sv']}, {t2 - t1:.3f}s")
+        if rep:
+            out.append((t2 - t1, res))
+    secs = sum(o[0] for o in out) / len(out)
+    res = out[0][1]
+    return {"metric": "cell-updates/s", "value": res["sum_sv"] / secs, "unit": "cell-updates/s", "n_regions": T,
+            "run_s": secs, "ca_steps": res["step"], "X": res["X"], "sum_sv": res["sum_sv"],
+            "windows": res["windows"], "grainmap_sha256_16": digest}
+
+
 def batch_bench(a) -> int:
     from score_amd.growth import device_count
     if device_count() < 1:
@@ -356,6 +386,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-rolled-sheet", action="store_true",
                     help="leave the 1024x1024x256 rolled-sheet part out of the default line")
     ap.add_argument("--rolled-sheet", action="store_true", help="only the rolled-sheet domain")
+    ap.add_argument("--regions", type=int, default=8,
+                    help="also run the workload with SCORE's OMP_NUM_THREADS semantics at this many regions (0: skip)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -547,6 +579,8 @@ def main(argv=None) -> int:
         out["batch"] = ensemble_section(a.seed, a.ensemble_domains, a.inflight)
     if a.served_requests:
         out["serve"] = served_section(a.seed, a.served_requests, a.inflight, a.rate)
+    if a.regions > 1:
+        out["regions"] = regions_section(dom, prm, a.regions)
     if not a.no_rolled_sheet:
         out["rolled_sheet"] = rolled_sheet_section(a.seed)
     print(json.dumps(out), flush=True)

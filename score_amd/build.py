@@ -37,7 +37,7 @@ def _run(cmd: list[str]) -> None:
 
 def build_gm(out: str | None = None) -> str:
     out = out or os.path.join(HERE, "libscore_gm.so")
-    srcs = [os.path.join(CSRC, "gm_kernels.hip"), os.path.join(CSRC, "gm_noalloc.hip"), os.path.join(CSRC, "gm_host.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("gm_kernels.hip", "gm_regions.hip", "gm_noalloc.hip", "gm_host.cpp")]
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", tmp])
     os.replace(tmp, out)

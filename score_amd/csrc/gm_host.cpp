@@ -202,6 +202,10 @@ struct score_gm_handle {
     uint32_t glog_cnt = 0;
     std::vector<score_gm_grainlog_row> glog;
     std::vector<uint32_t> glog_counts;
+    /* several memory regions: the region table and the lists' first slots and
+     * capacities (2T lists, region r's Inside list 2r and Border list 2r+1) */
+    std::vector<GmRegion> reg;
+    std::vector<uint32_t> lbase, lcap, lrcap, lrbase;
 };
 
 namespace {
@@ -312,7 +316,9 @@ void free_device(score_gm_handle *h) {
     void *ptrs[] = {d.grid, d.claim, d.rlog, d.dmap, d.cstep, d.rec, d.pslot, d.dlist,
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
-                    d.scan_bsum, d.st, d.arr_ids, d.arr_off, d.nsite, d.nucP};
+                    d.scan_bsum, d.st, d.arr_ids, d.arr_off, d.nsite, d.nucP,
+                    d.reg, d.claim64, d.pmH, d.L.base, d.L.cap, d.L.rcap, d.L.rbase, d.L.n, d.L.nold, d.L.rlen,
+                    d.L.win, d.L.occ, d.L.tile, d.L.par, d.L.rst, d.L.rbuf, d.L.rs, d.L.hoff};
     for (void *q : ptrs) dfree(h, q);
     memset(&h->d, 0, sizeof(h->d));
     if (h->hst) (void)hipHostFree(h->hst);
@@ -385,7 +391,11 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
         const uint32_t npy = smallest_y_factor((uint32_t)prm->n_regions), npz = (uint32_t)prm->n_regions / npy;
         if (dom->ny / npy <= 10 || dom->nz / npz <= 10)
             return fail(SCORE_GM_ERR_ARG, "regions must be more than 10 cells thick along y and z");
-        return fail(SCORE_GM_ERR_UNSUPPORTED, "more than one memory region");
+        /* SCORE's OpenMP path has no nucleation after step 0 (SCORE_Kernel.cpp:6632) */
+        if (dom->rxg_step)
+            for (uint32_t i = 0; i < dom->n_rxg; i++)
+                if (dom->rxg_step[i] != 0)
+                    return fail(SCORE_GM_ERR_UNSUPPORTED, "nuclei arriving after step 0 with more than one region");
     }
     if (prm->n_defrag != 0 && !prm->defrag_at) return fail(SCORE_GM_ERR_ARG, "defragmentation thresholds missing");
     if (prm->n_sched < 2 || !prm->sched_time || !prm->sched_temp)
@@ -402,6 +412,167 @@ int validate(const score_gm_domain *dom, const score_gm_params *prm) {
     for (uint64_t i = 0; i < N; i++)
         if (dom->cellgrid[i] >= dom->n_defg) return fail(SCORE_GM_ERR_ARG, "cellgrid holds a non-deformed id");
     return 0;
+}
+
+/* The region grid of ompshar_ca2regions (SCORE_GridPartitioning.cpp:69-166):
+ * npy = smallestYFactor(T) slabs along y, npz = T / npy along z, x not split;
+ * region iz*npy + iy spans [iy*nry, iy*nry + nry - 1] (the last slab up to
+ * ny - 1) and likewise along z.  Its neighbour table (ompshar_init_neighbortopology,
+ * SCORE_Kernel.cpp:2028-2080, periodic on the region grid) and halos
+ * (ompshar_init_haloregions, :2223-2457, origins as determine_origins,
+ * :2134-2220: the neighbour's layer next to the region, a dummy where the
+ * neighbour is the region itself).  Then the lists: region r's Inside list may
+ * hold every cell of the region; its Border list and both recycling lists have
+ * SCORE's sizes for the region (ompshar_init_myRXFrontInside/Border,
+ * SCORE_Kernel.cpp:3587-3710). */
+void build_regions(score_gm_handle *h, uint32_t T) {
+    static const int8_t dir[27][3] = {
+        {0, 0, 0},
+        {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1},
+        {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+        {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 1, 0},
+        {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1},
+        {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1}};
+    GmDev &d = h->d;
+    d.nreg = T;
+    d.nlists = 2 * T;
+    d.npy = smallest_y_factor(T);
+    d.npz = T / d.npy;
+    d.nry = d.ny / d.npy;
+    d.nrz = d.nz / d.npz;
+    h->reg.assign(T, GmRegion{});
+    for (uint32_t iz = 0; iz < d.npz; iz++)
+        for (uint32_t iy = 0; iy < d.npy; iy++) {
+            GmRegion &R = h->reg[iz * d.npy + iy];
+            R.lo[0] = 0;
+            R.hi[0] = d.nx - 1;
+            R.lo[1] = iy * d.nry;
+            R.hi[1] = iy == d.npy - 1 ? d.ny - 1 : iy * d.nry + d.nry - 1;
+            R.lo[2] = iz * d.nrz;
+            R.hi[2] = iz == d.npz - 1 ? d.nz - 1 : iz * d.nrz + d.nrz - 1;
+        }
+    const uint32_t dims[3] = {d.nx, d.ny, d.nz};
+    for (uint32_t r = 0; r < T; r++) {
+        GmRegion &R = h->reg[r];
+        const int iy = (int)(r % d.npy), iz = (int)(r / d.npy);
+        for (int nb = 0; nb < 27; nb++) {
+            int jy = iy + dir[nb][1], jz = iz + dir[nb][2];
+            if (jy < 0) jy += (int)d.npy;
+            if (jy >= (int)d.npy) jy -= (int)d.npy;
+            if (jz < 0) jz += (int)d.npz;
+            if (jz >= (int)d.npz) jz -= (int)d.npz;
+            R.nbor[nb] = (uint32_t)jz * d.npy + (uint32_t)jy;
+        }
+        for (int c = 0; c < 27; c++) {
+            GmHalo &H = R.halo[c];
+            H.inregion = R.nbor[c];
+            if (c == 0 || R.nbor[c] == r) continue;
+            for (int a = 0; a < 3; a++) {
+                int org;
+                if (dir[c][a] < 0) {
+                    org = (int)R.lo[a] - 1;
+                    if (org < 0) org += (int)dims[a];
+                } else if (dir[c][a] == 0) {
+                    org = (int)R.lo[a];
+                } else {
+                    org = (int)R.hi[a] + 1;
+                    if (org >= (int)dims[a]) org -= (int)dims[a];
+                }
+                H.org[a] = (uint16_t)org;
+                H.ext[a] = (uint16_t)(dir[c][a] == 0 ? R.hi[a] - R.lo[a] + 1 : 1);
+            }
+        }
+    }
+    const uint32_t L = 2 * T;
+    h->lcap.assign(L, 0);
+    h->lrcap.assign(L, 0);
+    h->lbase.assign(L + 1, 0);
+    h->lrbase.assign(L + 1, 0);
+    for (uint32_t r = 0; r < T; r++) {
+        const GmRegion &R = h->reg[r];
+        const uint32_t ex = R.hi[0] - R.lo[0] + 1, ey = R.hi[1] - R.lo[1] + 1, ez = R.hi[2] - R.lo[2] + 1;
+        const double vol = (double)((uint64_t)ex * ey * ez);
+        const uint32_t capB = 2 * ((ex * ey) + (ex * ez) + (ey * ez));
+        h->lcap[2 * r] = ex * ey * ez;
+        h->lcap[2 * r + 1] = capB;
+        h->lrcap[2 * r] = (uint32_t)(h->p.rel_cell_caching * vol * h->p.maxfill * kFullRecyclingAtt);
+        h->lrcap[2 * r + 1] = (uint32_t)((double)capB * h->p.maxfill * kFullRecyclingAtt);
+    }
+    for (uint32_t l = 0; l < L; l++) {
+        h->lbase[l + 1] = h->lbase[l] + (h->lcap[l] + 63) / 64 * 64;
+        h->lrbase[l + 1] = h->lrbase[l] + h->lrcap[l];
+    }
+}
+
+/* Site-saturated nucleation with several regions (ompshar_sim_myCA_sitesaturatedNucleation,
+ * SCORE_GrowthMachine.cpp:50-121; infect_JuvenileNucleation, :124-189): each
+ * region places the nuclei whose site lies in its box, in pool order, appending
+ * a record (rxFrac 1.0, infector 26) to its Inside list for a site strictly
+ * inside the box, else to its Border list.  site0[k] is nucleus k's site or
+ * GM_NO_SITE; a site met again holds a nucleus already and rejects it.  The
+ * records go to the device with the lists' tables and lengths. */
+int place_nuclei_regions(score_gm_handle *h, const score_gm_domain *dom, const std::vector<uint32_t> &site0,
+                         const std::vector<double> &nucP) {
+    GmDev &d = h->d;
+    hipStream_t s = h->stream;
+    const uint32_t L = d.nlists;
+    std::vector<uint32_t> cnt(L, 0u), slots;
+    std::vector<GmRec> recs;
+    std::vector<uint64_t> seen((d.N + 63) / 64, 0ull);
+    for (uint32_t k = 0; k < d.nrxg; k++) {
+        const uint32_t w = site0[k];
+        if (w == GM_NO_SITE || (seen[w >> 6] >> (w & 63) & 1ull)) continue;
+        seen[w >> 6] |= 1ull << (w & 63);
+        const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+        uint32_t iy = y / d.nry, iz = z / d.nrz;
+        if (iy >= d.npy) iy = d.npy - 1;
+        if (iz >= d.npz) iz = d.npz - 1;
+        const uint32_t r = iz * d.npy + iy;
+        const GmRegion &R = h->reg[r];
+        const bool inside = x > R.lo[0] && x < R.hi[0] && y > R.lo[1] && y < R.hi[1] && z > R.lo[2] && z < R.hi[2];
+        const uint32_t l = 2 * r + (inside ? 0u : 1u);
+        if (cnt[l] >= h->lcap[l]) return fail(SCORE_GM_ERR_CAPACITY, "Border front list full while placing nuclei");
+        GmRec rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.frac0 = 1.0;
+        rec.s0 = 0;
+        rec.inf = 26;
+        rec.P = nucP[k];
+        rec.cell = w;
+        rec.defg = dom->cellgrid[w];
+        rec.rxg = k;
+        slots.push_back(h->lbase[l] + cnt[l]);
+        recs.push_back(rec);
+        cnt[l]++;
+    }
+    uint32_t *d_slots = nullptr;
+    GmRec *d_recs = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &d_slots, slots.size())) || (rc = dalloc(h, &d_recs, recs.size()))) {
+        dfree(h, d_slots);
+        return rc;
+    }
+    auto up = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(d.reg, h->reg.data(), sizeof(GmRegion) * d.nreg, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.L.base, h->lbase.data(), 4ull * (L + 1), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.L.cap, h->lcap.data(), 4ull * L, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.L.rcap, h->lrcap.data(), 4ull * L, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.L.rbase, h->lrbase.data(), 4ull * (L + 1), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.L.n, cnt.data(), 4ull * L, hipMemcpyHostToDevice, s));
+        if (!slots.empty()) {
+            HIPCHK(hipMemcpyAsync(d_slots, slots.data(), 4ull * slots.size(), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(d_recs, recs.data(), sizeof(GmRec) * recs.size(), hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(gm_launch_rg_place(d, d_slots, d_recs, (uint32_t)slots.size(), s));
+        HIPCHK(gm_launch_rg_prefix(d, s));
+        /* the host vectors must outlive the asynchronous copies */
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    };
+    rc = up();
+    dfree(h, d_slots);
+    dfree(h, d_recs);
+    return rc;
 }
 
 /* ------------------------------------------------------------- physics scalars */
@@ -632,7 +803,9 @@ uint32_t arrivals_at(const score_gm_handle *h, uint32_t k) {
 int ensure_step_capacity(score_gm_handle *h) {
     GmDev &d = h->d;
     const uint32_t na = arrivals_at(h, h->step);
-    const uint32_t tiles = gm_tiles_inside(h->nI + na) + (h->nB + na + 63) / 64 + GM_SW_TPC;
+    /* with several regions each list is swept in whole 8-tile chunks */
+    const uint32_t tiles = d.nreg > 1 ? (h->nI + h->nB) / 64 + GM_SW_TPC * (d.nlists + 1)
+                                      : gm_tiles_inside(h->nI + na) + (h->nB + na + 63) / 64 + GM_SW_TPC;
     if (tiles > d.tileCap) {
         uint32_t nc = std::max<uint32_t>(tiles, d.tileCap + d.tileCap / 2);
         dfree(h, d.mF); dfree(h, d.mR);
@@ -644,13 +817,13 @@ int ensure_step_capacity(score_gm_handle *h) {
     const uint32_t parents = h->nI + h->nB + na;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
-        dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par);
-        d.pmI = d.pmB = d.pslot = nullptr;
+        dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par); dfree(h, d.pmH);
+        d.pmI = d.pmB = d.pslot = d.pmH = nullptr;
         d.par = nullptr;
         d.parentCap = 0;
         int rc;
         if ((rc = dalloc(h, &d.pmI, nc)) || (rc = dalloc(h, &d.pmB, nc)) || (rc = dalloc(h, &d.pslot, nc)) ||
-            (rc = dalloc(h, &d.par, nc)))
+            (rc = dalloc(h, &d.par, nc)) || (d.nreg > 1 && (rc = dalloc(h, &d.pmH, nc))))
             return rc;
         d.parentCap = nc;
     }
@@ -698,6 +871,7 @@ int resolve_pending(score_gm_handle *h, const GmStats &st) {
 }
 
 int step_error(uint32_t err);
+int list_lengths(score_gm_handle *h, std::vector<uint32_t> &n);
 
 /* Grow the cache (doubling) until it is at most half full with `room` more
  * entries: markers in the front are resolved first, then every entry is
@@ -706,8 +880,15 @@ int grow_pcache(score_gm_handle *h, uint64_t room = 0) {
     GmDev &d = h->d;
     if (((uint64_t)h->hcount + room) * 2 <= d.hcap) return 0;
     const GmFrontOut none{};
-    HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, none, h->stream));
-    HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, none, h->stream));
+    if (d.nreg > 1) {
+        std::vector<uint32_t> ln;
+        const int rc = list_lengths(h, ln);
+        if (rc) return rc;
+        for (uint32_t l = 0; l < d.nlists; l++) HIPCHK(gm_launch_materialize(d, h->step, h->lbase[l], ln[l], none, h->stream));
+    } else {
+        HIPCHK(gm_launch_materialize(d, h->step, 0, h->nI, none, h->stream));
+        HIPCHK(gm_launch_materialize(d, h->step, d.capI, h->nB, none, h->stream));
+    }
     std::vector<unsigned long long> keys(d.hcap);
     std::vector<double> vals(d.hcap);
     HIPCHK(hipMemcpyAsync(keys.data(), d.hkeys, sizeof(unsigned long long) * d.hcap, hipMemcpyDeviceToHost, h->stream));
@@ -762,6 +943,25 @@ int resolve_spilled(score_gm_handle *h, uint32_t nspilled, uint32_t ndeferred) {
         nspilled = h->hst->novf;
     }
     return grow_pcache(h);
+}
+
+/* with several regions: the lists' lengths from the device, and back with their
+ * prefixes and sums (d.st->nI, d.st->nB, mirrored into h->nI, h->nB) */
+int list_lengths(score_gm_handle *h, std::vector<uint32_t> &n) {
+    n.assign(h->d.nlists, 0u);
+    HIPCHK(hipMemcpyAsync(n.data(), h->d.L.n, 4ull * h->d.nlists, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int set_list_lengths(score_gm_handle *h, const std::vector<uint32_t> &n) {
+    HIPCHK(hipMemcpyAsync(h->d.L.n, n.data(), 4ull * h->d.nlists, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(gm_launch_rg_prefix(h->d, h->stream));
+    HIPCHK(gm_launch_stats_out(h->d, h->hst_dev, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->nI = h->hst->nI;
+    h->nB = h->hst->nB;
+    return 0;
 }
 
 /* ompshar_defragmentation (SCORE_Kernel.cpp:3877-3934) of the list whose slot
@@ -885,6 +1085,7 @@ int step_error(uint32_t err) {
     if (err & GM_ERRF_KEYS)
         return fail(SCORE_GM_ERR_UNSUPPORTED, "more than 2^27 cells completed in one step (infection keys are 32-bit)");
     if (err & GM_ERRF_RLOG_FULL) return fail(SCORE_GM_ERR_CAPACITY, "retirement log full (cells retired twice)");
+    if (err & GM_ERRF_HALO) return fail(SCORE_GM_ERR_STATE, "internal error: an infection into another region found no halo");
     return fail(SCORE_GM_ERR_STATE, "internal error: mobility-weight cache entry lost while growing it");
 }
 
@@ -928,13 +1129,22 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     h->pend_P.clear();
     /* on-the-fly defragmentation at the Xcells thresholds (SCORE_Kernel.cpp:6625-6630) */
     if (h->defrag_cnt < h->defrag_at.size() && h->Xcells >= h->defrag_at[h->defrag_cnt]) {
-        if ((rc = defragment_list(h, 0u, d.capI, h->nI)) || (rc = defragment_list(h, d.capI, d.capB, h->nB)))
-            return h->err = rc;
-        /* the device keeps the list lengths in its stats */
-        hipError_t e2;
-        if ((e2 = gm_launch_fill_u32(&d.st->nI, h->nI, 1, h->stream)) != hipSuccess ||
-            (e2 = gm_launch_fill_u32(&d.st->nB, h->nB, 1, h->stream)) != hipSuccess)
-            return h->err = hip_fail(e2, "defragmentation");
+        if (d.nreg > 1) {
+            /* every region defragments both its lists (the threshold test is the same for all) */
+            std::vector<uint32_t> n;
+            if ((rc = list_lengths(h, n))) return h->err = rc;
+            for (uint32_t l = 0; l < d.nlists; l++)
+                if ((rc = defragment_list(h, h->lbase[l], h->lcap[l], n[l]))) return h->err = rc;
+            if ((rc = set_list_lengths(h, n))) return h->err = rc;
+        } else {
+            if ((rc = defragment_list(h, 0u, d.capI, h->nI)) || (rc = defragment_list(h, d.capI, d.capB, h->nB)))
+                return h->err = rc;
+            /* the device keeps the list lengths in its stats */
+            hipError_t e2;
+            if ((e2 = gm_launch_fill_u32(&d.st->nI, h->nI, 1, h->stream)) != hipSuccess ||
+                (e2 = gm_launch_fill_u32(&d.st->nB, h->nB, 1, h->stream)) != hipSuccess)
+                return h->err = hip_fail(e2, "defragmentation");
+        }
         h->defrag_cnt++;
         h->force_reanchor = true;
     }
@@ -958,8 +1168,9 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     const uint32_t W = window_length(h, wmax, sp);
     /* grids: exact for one step, with room for growth over a window (the kernels
      * take the lengths from the device and cover any grid) */
-    const uint32_t nIe = W == 1 ? h->nI : (uint32_t)std::min<uint64_t>(d.capI, h->nI + h->nI / 2 + 4096u);
-    const uint32_t nBe = W == 1 ? h->nB : (uint32_t)std::min<uint64_t>(d.capB, h->nB + h->nB / 2 + 4096u);
+    const uint64_t capIe = d.nreg > 1 ? 0x7FFFFFFFull : d.capI, capBe = d.nreg > 1 ? 0x7FFFFFFFull : d.capB;
+    const uint32_t nIe = W == 1 ? h->nI : (uint32_t)std::min<uint64_t>(capIe, h->nI + h->nI / 2 + 4096u);
+    const uint32_t nBe = W == 1 ? h->nB : (uint32_t)std::min<uint64_t>(capBe, h->nB + h->nB / 2 + 4096u);
     a.nI = nIe;
     a.nB = nBe;
     GmWindow w;
@@ -976,6 +1187,13 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     for (uint32_t j = 0; j < W; j++) {
         GmSweepArgs aj = a;
         if (j) aj.reanchor = 0;
+        if (d.nreg > 1) {
+            if ((e = gm_launch_rg_sweep(d, aj, h->stream, ev)) != hipSuccess ||
+                (e = gm_launch_rg_infect(d, h->pp, aj, std::max(slots0, nIe + nBe), h->stream, ev)) != hipSuccess ||
+                (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
+                return h->err = hip_fail(e, "growth step");
+            continue;
+        }
         if ((e = gm_launch_sweep(d, aj, h->stream, ev)) != hipSuccess ||
             (e = gm_launch_infect(d, h->pp, aj, std::max(slots0, nIe + nBe), h->stream, ev)) != hipSuccess ||
             (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
@@ -1253,10 +1471,18 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
      * the Inside list may grow up to N here (rounded up to whole 64-slot tiles, so
      * that the Border slots start on a tile); the Border list and both recycling
      * lists are fixed */
-    d.capI = (uint32_t)((d.N + 63) / 64 * 64);
-    d.capB = 2 * ((d.nx * d.ny) + (d.nx * d.nz) + (d.ny * d.nz));
-    d.recCapI = (uint32_t)(prm->rel_cell_caching * (double)d.N * prm->maxfill * kFullRecyclingAtt);
-    d.recCapB = (uint32_t)((double)d.capB * prm->maxfill * kFullRecyclingAtt);
+    d.nreg = 1;
+    d.nlists = 2;
+    if (prm->n_regions > 1) {
+        /* several regions: per-region lists (build_regions); the one-region sizes stay 0 */
+        build_regions(h, (uint32_t)prm->n_regions);
+    } else {
+        d.capI = (uint32_t)((d.N + 63) / 64 * 64);
+        d.capB = 2 * ((d.nx * d.ny) + (d.nx * d.nz) + (d.ny * d.nz));
+        d.recCapI = (uint32_t)(prm->rel_cell_caching * (double)d.N * prm->maxfill * kFullRecyclingAtt);
+        d.recCapB = (uint32_t)((double)d.capB * prm->maxfill * kFullRecyclingAtt);
+    }
+    const bool multi = d.nreg > 1;
 
     h->defq.resize(4 * (size_t)d.ndefg);
     h->rxq.resize(4 * (size_t)d.nrxg + 4);
@@ -1314,7 +1540,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     h->pp.fac = 1.0 / sqrt(2.0);
 
     /* slots plus a margin of one sweep chunk (512 slots) */
-    const size_t capTot = (size_t)d.capI + (d.capB + 63) / 64 * 64 + 512;
+    const size_t capTot = multi ? (size_t)h->lbase[d.nlists] + 512 : (size_t)d.capI + (d.capB + 63) / 64 * 64 + 512;
     d.hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * d.nrxg));
     d.dlistCap = (uint32_t)std::min<uint64_t>(d.N, 1u << 20);
     /* completion steps further ahead than the horizon are re-predicted at its
@@ -1346,15 +1572,19 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     }
     /* every cell retires once; a cell revived by a defragmentation can retire
      * again, at most once per front slot of the Border list beyond that */
-    d.rlogCap = d.N + (uint64_t)d.capB + 4096u;
-    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) || (rc = dalloc(h, &d.rlog, d.rlogCap)) ||
+    uint64_t capBsum = d.capB;
+    if (multi)
+        for (uint32_t r = 0; r < d.nreg; r++) capBsum += h->lcap[2 * r + 1];
+    d.rlogCap = d.N + capBsum + 4096u;
+    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, multi ? 1 : d.N)) ||
+        (rc = dalloc(h, &d.rlog, d.rlogCap)) ||
         (rc = dalloc(h, &d.cstep, capTot)) ||
         (rc = dalloc(h, &d.rec, capTot)) ||
         (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
         (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
         (rc = dalloc(h, &d.par, d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||
         (rc = dalloc(h, &d.mF, d.tileCap)) || (rc = dalloc(h, &d.mR, d.tileCap)) || (rc = dalloc(h, &d.pmI, d.parentCap)) ||
-        (rc = dalloc(h, &d.pmB, d.parentCap)) || (rc = dalloc(h, &d.bcnt, 8 * GM_MAX_BLOCKS)) ||
+        (rc = dalloc(h, &d.pmB, d.parentCap)) || (rc = dalloc(h, &d.bcnt, GM_BCNT_ROWS * GM_MAX_BLOCKS)) ||
         (rc = dalloc(h, &d.rho, d.ndefg)) || (rc = dalloc(h, &d.defq, 4 * (size_t)d.ndefg)) ||
         (rc = dalloc(h, &d.rxq, 4 * (size_t)d.nrxg + 4)) || (rc = dalloc(h, &d.defcnt, d.ndefg)) ||
         (rc = dalloc(h, &d.rxcnt, (size_t)d.nrxg + 1)) || (rc = dalloc(h, &d.zlist, d.ndefg)) ||
@@ -1366,6 +1596,20 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         free_device(h);
         delete h;
         return rc;
+    }
+    if (multi) {
+        const uint32_t L = d.nlists;
+        if ((rc = dalloc(h, &d.reg, d.nreg)) || (rc = dalloc(h, &d.claim64, d.N)) || (rc = dalloc(h, &d.pmH, d.parentCap)) ||
+            (rc = dalloc(h, &d.L.base, L + 1)) || (rc = dalloc(h, &d.L.cap, L)) || (rc = dalloc(h, &d.L.rcap, L)) ||
+            (rc = dalloc(h, &d.L.rbase, L + 1)) || (rc = dalloc(h, &d.L.n, L)) || (rc = dalloc(h, &d.L.nold, L)) ||
+            (rc = dalloc(h, &d.L.rlen, L)) || (rc = dalloc(h, &d.L.win, L)) || (rc = dalloc(h, &d.L.occ, L + 1)) ||
+            (rc = dalloc(h, &d.L.tile, L + 1)) || (rc = dalloc(h, &d.L.par, L + 1)) || (rc = dalloc(h, &d.L.rst, L + 1)) ||
+            (rc = dalloc(h, &d.L.rbuf, h->lrbase[L])) || (rc = dalloc(h, &d.L.rs, (size_t)(d.nreg + 1) * GM_RROWS)) ||
+            (rc = dalloc(h, &d.L.hoff, (size_t)d.nreg * GM_NHALO))) {
+            free_device(h);
+            delete h;
+            return rc;
+        }
     }
     if (d.arr_last &&
         ((rc = dalloc(h, &d.arr_ids, d.nrxg)) || (rc = dalloc(h, &d.arr_off, h->arr_off.size())) ||
@@ -1384,7 +1628,8 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     auto setup = [&]() -> int {
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
-        HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
+        if (multi) HIPCHK(gm_launch_fill_u64(d.claim64, ~0ull, d.N, s));
+        else HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
         HIPCHK(gm_launch_dmap_init(d, s));
         /* two INACTIVE codes per word (capTot is a multiple of 64) */
         HIPCHK(gm_launch_fill_u32((uint32_t *)d.cstep, 0xFFFFFFFFu, capTot / 2, s));
@@ -1410,7 +1655,11 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
             HIPCHK(hipMemcpyAsync(d.nsite, dom->rxg_site, sizeof(uint32_t) * d.nrxg, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(d.nucP, nucP.data(), sizeof(double) * d.nrxg, hipMemcpyHostToDevice, s));
         }
-        HIPCHK(gm_launch_nucleate(d, d_site, d_nucP, d.nrxg, s));
+        if (multi) {
+            if ((r2 = place_nuclei_regions(h, dom, site0, nucP))) return r2;
+        } else {
+            HIPCHK(gm_launch_nucleate(d, d_site, d_nucP, d.nrxg, s));
+        }
         HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
         HIPCHK(hipStreamSynchronize(s));
         return 0;
@@ -1555,10 +1804,17 @@ int score_gm_get_trajectory(const score_gm_handle *h, score_gm_traj_row *rows, s
 int score_gm_get_front(score_gm_handle *h, int32_t list, uint32_t cap, uint32_t *n, uint8_t *activity,
                        uint8_t *infector, uint32_t *cell, double *rxfrac, double *P, uint32_t *defg, uint32_t *rxg) {
     if (!h || !n) return fail(SCORE_GM_ERR_ARG, "null argument");
-    if (list != SCORE_GM_LIST_INSIDE && list != SCORE_GM_LIST_BORDER) return fail(SCORE_GM_ERR_ARG, "unknown list");
+    if (list < 0 || (uint32_t)list >= h->d.nlists) return fail(SCORE_GM_ERR_ARG, "unknown list");
     HIPCHK(hipSetDevice(h->device));
-    const uint32_t cnt = list == SCORE_GM_LIST_INSIDE ? h->nI : h->nB;
-    const uint32_t base = list == SCORE_GM_LIST_INSIDE ? 0u : h->d.capI;
+    uint32_t cnt = list == SCORE_GM_LIST_INSIDE ? h->nI : h->nB;
+    uint32_t base = list == SCORE_GM_LIST_INSIDE ? 0u : h->d.capI;
+    if (h->d.nreg > 1) {
+        std::vector<uint32_t> ln;
+        int rc = list_lengths(h, ln);
+        if (rc) return rc;
+        cnt = ln[list];
+        base = h->lbase[list];
+    }
     *n = cnt;
     const uint32_t m = std::min(cap, cnt);
     hipStream_t s = h->stream;
@@ -1739,12 +1995,32 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     int rc = validate(dom, prm);
     if (rc) return rc;
     const uint64_t N = (uint64_t)dom->nx * dom->ny * dom->nz;
-    const uint64_t capB = 2ull * ((uint64_t)dom->nx * dom->ny + (uint64_t)dom->nx * dom->nz + (uint64_t)dom->ny * dom->nz);
-    const uint64_t capTot = (N + 63) / 64 * 64 + (capB + 63) / 64 * 64 + 512;
+    uint64_t capB = 2ull * ((uint64_t)dom->nx * dom->ny + (uint64_t)dom->nx * dom->nz + (uint64_t)dom->ny * dom->nz);
+    uint64_t capTot = (N + 63) / 64 * 64 + (capB + 63) / 64 * 64 + 512;
+    uint64_t rbuf = (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt) +
+                    (uint64_t)((double)capB * prm->maxfill * kFullRecyclingAtt);
     const uint64_t bricks = (uint64_t)((dom->nx + 15) / 16) * ((dom->ny + 7) / 8) * ((dom->nz + 7) / 8);
     const uint64_t hcap = pow2_at_least(std::max<uint64_t>(65536, 4ull * dom->n_rxg));
     const uint64_t front = std::max<uint64_t>(65536, N / 4);
     uint64_t b = 0;
+    if (prm->n_regions > 1) {
+        /* the lists of every region (build_regions), 64-bit claims, halo masks and tables */
+        const uint32_t T = (uint32_t)prm->n_regions, npy = smallest_y_factor(T), npz = T / npy;
+        const uint64_t nry = dom->ny / npy, nrz = dom->nz / npz, ex = dom->nx;
+        capB = 0;
+        capTot = 512;
+        rbuf = 0;
+        for (uint32_t iz = 0; iz < npz; iz++)
+            for (uint32_t iy = 0; iy < npy; iy++) {
+                const uint64_t ey = iy == npy - 1 ? dom->ny - iy * nry : nry, ez = iz == npz - 1 ? dom->nz - iz * nrz : nrz;
+                const uint64_t vol = ex * ey * ez, cb = 2 * (ex * ey + ex * ez + ey * ez);
+                capB += cb;
+                capTot += (vol + 63) / 64 * 64 + (cb + 63) / 64 * 64;
+                rbuf += (uint64_t)(prm->rel_cell_caching * (double)vol * prm->maxfill * kFullRecyclingAtt) +
+                        (uint64_t)((double)cb * prm->maxfill * kFullRecyclingAtt);
+            }
+        b += 4 * N + 4 * front + (sizeof(GmRegion) + 4ull * (GM_RROWS + GM_NHALO + 26)) * (T + 1) + 16ull * GM_SW_TPC * 2 * T;
+    }
     b += 8 * N;                                   /* grid, claim */
     b += 8 * (N + capB + 4096);                   /* retirement log */
     b += 32 * bricks;                             /* deformed bitmap */
@@ -1752,8 +2028,8 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */
     b += 4 * std::min<uint64_t>(N, 1u << 20);     /* deferred-cell list */
     b += 16 * (capTot / 64 + GM_SW_TPC);          /* tile masks */
-    b += 4 * (uint64_t)((double)N * prm->rel_cell_caching * prm->maxfill * kFullRecyclingAtt + (double)capB * prm->maxfill * kFullRecyclingAtt);
-    b += 4 * 8 * GM_MAX_BLOCKS;
+    b += 4 * rbuf;                                /* recycled slots */
+    b += 4 * GM_BCNT_ROWS * GM_MAX_BLOCKS;
     b += 52ull * dom->n_defg + 52ull * (dom->n_rxg + 1);
     if (dom->rxg_step) b += 20ull * dom->n_rxg;   /* arrivals: ids, sites, weights */
     b += 28 * hcap;                               /* weight cache: keys, values, pending slots and keys */

@@ -68,6 +68,7 @@ typedef uint16_t gm_cs_t;
 #define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
 #define GM_ERRF_KEYS 0x10u        /* more than 2^27 completed cells in one step: infection keys would not fit */
 #define GM_ERRF_RLOG_FULL 0x20u   /* more retirements than the log holds (only cells retired twice can do it) */
+#define GM_ERRF_HALO 0x40u        /* an infection into another region found no halo to park in: internal error */
 
 /* the mobility-weight cache probes at most this many slots from a key's home;
  * a key that finds no room within them spills: its cells hold the pending
@@ -189,6 +190,54 @@ struct GmWindow {
     uint32_t pad;
 };
 
+/* Several memory regions (SCORE with OMP_NUM_THREADS = T > 1, ompshar_ca2regions,
+ * SCORE_GridPartitioning.cpp:69-166).  Region r = iz*npy + iy owns front lists
+ * 2r (Inside) and 2r+1 (Border), laid out one after another in the slot arrays.
+ * One halo of a region (ompshar_initialize_halo, SCORE_Kernel.cpp:2085-2127): the
+ * region it lies in, its origin and extent in cells (extent 0: a dummy). */
+struct GmHalo {
+    uint32_t inregion;
+    uint16_t org[3], ext[3];
+};
+static_assert(sizeof(GmHalo) == 16, "GmHalo is 16 bytes");
+
+struct GmRegion {
+    uint32_t lo[3], hi[3];      /* inclusive box */
+    uint32_t nbor[27];          /* mynborregions ids: myself, faces, edges, corners */
+    GmHalo halo[27];            /* halo towards each neighbour; [0] unused */
+};
+
+/* halos an infection can be parked in (the other 18 are dummies or always
+ * shadowed by one of these, which come first and span the whole x range) */
+#define GM_NHALO 8
+/* rows of the per-block counts of the infection pass with several regions:
+ * winners into the own Inside / Border list, attempts, halo winners by halo */
+#define GM_RW_I 0
+#define GM_RW_B 1
+#define GM_RW_A 2
+#define GM_RW_H0 3
+#define GM_RROWS (GM_RW_H0 + GM_NHALO)
+#define GM_BCNT_ROWS 16
+
+/* per-list tables with several regions: nlists entries, the prefix arrays nlists+1 */
+struct GmLists {
+    uint32_t *base;             /* first global slot of each list (multiples of 64) */
+    uint32_t *cap;              /* slots of each list */
+    uint32_t *rcap;             /* recycling-list capacity of each list (fixed, as SCORE's) */
+    uint32_t *rbase;            /* start of each list's recycled slots in rbuf */
+    uint32_t *n;                /* nextSlotNeverActive */
+    uint32_t *nold;             /* the same before the current step's appends */
+    uint32_t *rlen;             /* recycled slots that may be handed out this step */
+    uint32_t *win;              /* this step's new records into each list */
+    uint32_t *occ;              /* exclusive prefix of n: occupied slot i lies in list l iff occ[l] <= i < occ[l+1] */
+    uint32_t *tile;             /* exclusive prefix of the sweep's tiles per list (whole 8-tile chunks) */
+    uint32_t *par;              /* completed cells before each list's first, in FullRX order */
+    uint32_t *rst;              /* INACTIVE slots before each list's first */
+    uint32_t *rbuf;             /* recycled slots (list-relative), ascending per list */
+    uint32_t *rs;               /* per region r (and r = T: totals), GM_RROWS prefix counts at its first parent */
+    uint32_t *hoff;             /* per region, GM_NHALO offsets of the halo winners in its Border list */
+};
+
 struct GmDev {
     /* geometry */
     uint32_t nx, ny, nz, nxy;
@@ -237,6 +286,13 @@ struct GmDev {
     uint32_t *nflag, *nI_, *nB_;
     uint32_t *scan_bsum;        /* 2*GM_SCAN_BLOCKS */
     GmStats *st;
+    /* several memory regions (nreg > 1); with one region none of this is used */
+    uint32_t nreg, nlists;      /* T and 2T */
+    uint32_t npy, npz, nry, nrz;/* region grid and slab thickness along y and z */
+    GmRegion *reg;              /* nreg entries */
+    GmLists L;
+    unsigned long long *claim64;/* per-cell claim key with several regions (2^64-1 unclaimed) */
+    uint32_t *pmH;              /* per parent: winning ranks into another region's halo */
 };
 
 /* mobility-weight evaluation parameters the kernels need */
@@ -269,6 +325,16 @@ struct GmSweepArgs {
 /* launchers (gm_kernels.hip); all asynchronous on `s`.  With `ev` non-null,
  * ev[i] is recorded before kernel i of the step and ev[GM_NK] after the last. */
 hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
+/* every active cell advanced to step a.k with its old increment and predicted again */
+hipError_t gm_launch_reanchor(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
+/* the same two halves of a step with several regions (gm_regions.hip) */
+hipError_t gm_launch_rg_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
+hipError_t gm_launch_rg_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
+                               hipStream_t s, hipEvent_t *ev);
+/* with several regions: the lists' prefixes and summed lengths from d.L.n */
+hipError_t gm_launch_rg_prefix(const GmDev &d, hipStream_t s);
+/* with several regions: nuclei the host placed, records scattered to their slots */
+hipError_t gm_launch_rg_place(const GmDev &d, const uint32_t *slots, const GmRec *recs, uint32_t n, hipStream_t s);
 hipError_t gm_launch_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                             hipStream_t s, hipEvent_t *ev);
 /* start of a window: step index and Xcells; clears the window's accumulators */

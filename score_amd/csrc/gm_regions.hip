@@ -1,0 +1,739 @@
+/*
+ * gm_regions.hip -- the growth step with several memory regions.
+ *
+ * SCORE run with OMP_NUM_THREADS = T > 1 splits the domain into T regions
+ * (ompshar_ca2regions, SCORE_GridPartitioning.cpp:69-166), each with its own
+ * Inside and Border lists, and runs their growth steps side by side
+ * (SCORE_Kernel.cpp:6634-6654): each region sweeps its lists, lets its
+ * completed Inside cells infect, then its completed Border cells.  An infection
+ * aimed at a cell of another region is parked in a halo of the infecting
+ * region, first come first served (omp_infect_OutOfRXFrontBorder, :4494-4555);
+ * after the barrier each region takes up what its neighbours parked for it,
+ * neighbour by neighbour in its mynborregions order and each halo in parking
+ * order, into its Border list, with the weight looked up again
+ * (ompshar_synchronize_haloregions_rx, :4777-4872).
+ *
+ * That outcome is one claim per target cell with a 64-bit key:
+ *   attempt of the cell's own region    p*32 + r
+ *   attempt parked in a halo            2^40 + nb*2^35 + p*32 + r
+ * p is the parent's position in the step's FullRX order, which runs region by
+ * region, each Inside then Border, so that p orders one region's parents as its
+ * serial loop does; r is the attempt's rank and nb the receiving region's
+ * neighbour index of the halo.  Only a cell's own region attempts it directly,
+ * so the smallest key is the serial winner: the own region's first attempt,
+ * else the first parked one in the receiving region's order.  A halo cell,
+ * once parked, is never parked again (its status is never reset, :4527-4531);
+ * that can only matter for a cell that is no longer deformed, which no attempt
+ * here reaches (it left the deformed bitmap).
+ *
+ * The 2T lists keep SCORE's slot numbering: a new record takes its list's k-th
+ * recycled slot if there is one, else appends, k being its rank among the
+ * list's new records in serial order (for a Border list: the region's own
+ * infections, then the halo winners by neighbour).  Ranks come from prefix
+ * counts over the parents, less the count at the region's first parent.
+ *
+ * Per step, in the profiling slots of the one-region kernels:
+ *   k_reanchor (shared)   k_rg_sweep   k_rg_sweep_scan (+ per-list starts)
+ *   k_rg_compact (+ recycled slots)   k_rg_propose   k_rg_decide
+ *   k_rg_decide_scan (+ per-region starts, list lengths)   -   k_rg_commit
+ */
+#include <algorithm>
+
+#include "gm_device.h"
+
+/* opposite neighbour direction (mynborregions order, ompshar_init_neighbortopology,
+ * SCORE_Kernel.cpp:2028-2080) */
+static __constant__ uint8_t c_opp[27] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15,
+                                         26, 25, 24, 23, 22, 21, 20, 19};
+/* the halos an infection can be parked in, and each direction's index among them */
+static __constant__ uint8_t c_hdir[GM_NHALO] = {3, 4, 5, 6, 15, 16, 17, 18};
+static __constant__ uint8_t c_hidx[27] = {0xFF, 0xFF, 0xFF, 0, 1, 2, 3, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF,
+                                          0xFF, 4, 5, 6, 7, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF};
+
+/* rows of the sweep's per-block counts */
+#define RS_F 0
+#define RS_R 1
+#define RS_UN 2
+
+/* the part_of block size for a grid of nb blocks, from any thread */
+static __device__ __forceinline__ uint32_t part_per(uint32_t n, uint32_t nb, uint32_t q) {
+    uint32_t per = (n + nb - 1u) / nb;
+    return (per + q - 1u) / q * q;
+}
+
+/* sweep tiles of a list of n occupied slots: whole 8-tile chunks */
+static __device__ __forceinline__ uint32_t list_tiles(uint32_t n) {
+    return (n + 64u * GM_SW_TPC - 1u) / (64u * GM_SW_TPC) * GM_SW_TPC;
+}
+
+static __device__ __forceinline__ uint32_t list_of_slot(const GmDev &d, uint32_t s) {
+    return upper_index(d.L.base, d.nlists, s);
+}
+
+static __device__ __forceinline__ uint32_t region_of(const GmDev &d, uint32_t y, uint32_t z) {
+    uint32_t iy = y / d.nry, iz = z / d.nrz;
+    if (iy >= d.npy) iy = d.npy - 1u;
+    if (iz >= d.npz) iz = d.npz - 1u;
+    return iz * d.npy + iy;
+}
+
+struct RgTarget {
+    uint32_t x, y, z;
+    uint32_t dl;                /* list the new record goes to */
+    uint32_t hidx;              /* halo index among the GM_NHALO, for a parked attempt */
+    unsigned long long khi;     /* upper part of the claim key: 0 for the own region */
+};
+
+/* Target of attempt rank rk of a parent at (x,y,z) in list l, routed as the
+ * reference does.  Inside parents (omp_infect_OutOfRXFrontInside,
+ * SCORE_Kernel.cpp:4252-4344) stay in their region and send every target to its
+ * Border list unless they lie more than one cell inside each face of the
+ * region (:4606-4609).  Border parents (:4347-4556) wrap x and y, drop targets
+ * outside z, send targets strictly inside the region to its Inside list and the
+ * region's other targets to its Border list; a target in another region goes to
+ * the first halo towards that region whose box, taken one cell beyond its
+ * extent as the reference's test does (:4500-4512), holds it.  False: no
+ * target. */
+static __device__ __forceinline__ bool rg_target(const GmDev &d, uint32_t l, uint32_t x, uint32_t y, uint32_t z,
+                                                  int rk, RgTarget &o) {
+    const uint32_t r = l >> 1;
+    const GmRegion &R = d.reg[r];
+    int tx = (int)x + c_dx[rk], ty = (int)y + c_dy[rk], tz = (int)z + c_dz[rk];
+    const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
+    const int lo0 = (int)R.lo[0], lo1 = (int)R.lo[1], lo2 = (int)R.lo[2];
+    const int hi0 = (int)R.hi[0], hi1 = (int)R.hi[1], hi2 = (int)R.hi[2];
+    o.khi = 0ull;
+    o.hidx = GM_NHALO;
+    if (!(l & 1u)) {
+        /* an Inside cell is strictly inside its region: this never fires, and
+         * keeps a corrupted record from addressing outside the grid */
+        if (tx < 0 || tx >= nx || ty < 0 || ty >= ny || tz < 0 || tz >= nz) return false;
+        const bool deep = (int)x > lo0 + 1 && (int)x < hi0 - 1 && (int)y > lo1 + 1 && (int)y < hi1 - 1 &&
+                          (int)z > lo2 + 1 && (int)z < hi2 - 1;
+        o.dl = deep ? 2u * r : 2u * r + 1u;
+    } else {
+        if (tx < 0) tx += nx;
+        if (tx >= nx) tx -= nx;
+        if (ty < 0) ty += ny;
+        if (ty >= ny) ty -= ny;
+        if (tz < 0 || tz >= nz) return false;
+        if (tx > lo0 && tx < hi0 && ty > lo1 && ty < hi1 && tz > lo2 && tz < hi2) {
+            o.dl = 2u * r;
+        } else {
+            const uint32_t q = region_of(d, (uint32_t)ty, (uint32_t)tz);
+            if (q == r) {
+                o.dl = 2u * r + 1u;
+            } else {
+                uint32_t h = 0;
+                for (uint32_t c = 1; c < 27; c++) {
+                    const GmHalo &H = R.halo[c];
+                    if (H.inregion != q || H.ext[0] == 0) continue;
+                    if (tx >= (int)H.org[0] && tx <= (int)H.org[0] + (int)H.ext[0] && ty >= (int)H.org[1] &&
+                        ty <= (int)H.org[1] + (int)H.ext[1] && tz >= (int)H.org[2] && tz <= (int)H.org[2] + (int)H.ext[2]) {
+                        h = c;
+                        break;
+                    }
+                }
+                const uint32_t hx = c_hidx[h];
+                if (hx >= GM_NHALO) {
+                    atomicOr(&d.st->err, GM_ERRF_HALO);
+                    return false;
+                }
+                o.hidx = hx;
+                o.dl = 2u * q + 1u;
+                o.khi = (1ull << 40) | ((unsigned long long)c_opp[h] << 35);
+            }
+        }
+    }
+    o.x = (uint32_t)tx;
+    o.y = (uint32_t)ty;
+    o.z = (uint32_t)tz;
+    return true;
+}
+
+/* ------------------------------------------------------------- list tables */
+
+/* One block of 1024 threads: the lists' occupied and tile prefixes from their
+ * lengths, and the lengths summed over Inside and Border lists; thread t takes
+ * region t's two lists (at most 1024 regions). */
+static __device__ void rg_prefix_block(const GmDev &d) {
+    __shared__ uint32_t ws[16][4];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const bool v = t < d.nreg;
+    const uint32_t nI = v ? d.L.n[2u * t] : 0u, nB = v ? d.L.n[2u * t + 1u] : 0u;
+    const uint32_t tI = list_tiles(nI), tB = list_tiles(nB);
+    const uint32_t val[4] = {nI + nB, tI + tB, nI, nB};
+    uint32_t incl[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        incl[c] = wave_incl_scan(val[c]);
+        if (lane == 63u) ws[w][c] = incl[c];
+    }
+    __syncthreads();
+    uint32_t pre[4] = {0, 0, 0, 0}, tot[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < 16u; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (i < w) pre[c] += ws[i][c];
+            tot[c] += ws[i][c];
+        }
+    if (v) {
+        const uint32_t o = pre[0] + incl[0] - val[0], ti = pre[1] + incl[1] - val[1];
+        d.L.occ[2u * t] = o;
+        d.L.occ[2u * t + 1u] = o + nI;
+        d.L.tile[2u * t] = ti;
+        d.L.tile[2u * t + 1u] = ti + tI;
+    }
+    if (t == 0) {
+        d.L.occ[d.nlists] = tot[0];
+        d.L.tile[d.nlists] = tot[1];
+        d.st->nI = tot[2];
+        d.st->nB = tot[3];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_rg_prefix(GmDev d) { rg_prefix_block(d); }
+
+hipError_t gm_launch_rg_prefix(const GmDev &d, hipStream_t s) {
+    k_rg_prefix<<<1, 1024, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+/* nuclei placed by the host: records and codes scattered to their slots, and
+ * their cells taken out of the deformed bitmap */
+__global__ void k_rg_place(GmDev d, const uint32_t *slots, const GmRec *recs, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = slots[i];
+        const GmRec r = recs[i];
+        d.rec[s] = r;
+        d.cstep[s] = (gm_cs_t)GM_CS_UNKNOWN;
+        const uint32_t w = r.cell;
+        const uint32_t z = w / d.nxy, rem = w - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+        clear_deformed(d, x, y, z);
+    }
+}
+
+hipError_t gm_launch_rg_place(const GmDev &d, const uint32_t *slots, const GmRec *recs, uint32_t n, hipStream_t s) {
+    if (n) k_rg_place<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, slots, recs, n);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------- sweep */
+
+/* Pass over the occupied slots of every list, as the one-region sweep does:
+ * the lists' tiles follow one another, each list in whole 8-tile chunks, so a
+ * chunk belongs to one list.  Completed and INACTIVE slots go into per-tile
+ * masks, counted per block. */
+__global__ void __launch_bounds__(GM_BLOCK) k_rg_sweep(GmDev d, GmSweepArgs A0) {
+    __shared__ uint32_t red[GM_BLOCK / 64][3];
+    if (d.st->halt) return;
+    const GmSweepArgs A = step_args(d, A0);
+    const uint32_t T = d.L.tile[d.nlists];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t lo, hi;
+    part_of(T, GM_SW_TPC, lo, hi);
+    const uint32_t c_lo = lo / GM_SW_TPC, c_hi = hi / GM_SW_TPC; /* T is a whole number of chunks */
+    const uint32_t kc = A.k & (GM_CS_MOD - 1u), horizon = d.horizon;
+    uint32_t cF = 0, cR = 0, unknown = 0;
+    for (uint32_t c = c_lo + w; c < c_hi; c += GM_BLOCK / 64) {
+        const uint32_t tile0 = c * GM_SW_TPC;
+        const uint32_t l = upper_index(d.L.tile, d.nlists, tile0);
+        const uint32_t s0 = d.L.base[l] + (tile0 - d.L.tile[l]) * 64u;
+        const uint32_t base = s0 + 8u * lane;
+        const uint32_t lim = d.L.base[l] + d.L.n[l];
+        const uint32_t nocc = lim > base ? (lim - base < 8u ? lim - base : 8u) : 0u;
+        const uint4 cs = nocc ? *reinterpret_cast<const uint4 *>(d.cstep + base)
+                              : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        const uint32_t wd[4] = {cs.x, cs.y, cs.z, cs.w};
+        uint32_t pack = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+            const uint32_t in = (uint32_t)q < nocc ? 1u : 0u;
+            pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
+            unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
+        }
+        uint64_t myF = 0, myR = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < GM_SW_TPC; t++) {
+            const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(8u * t + (lane >> 3)), 64);
+            const uint64_t mf = __ballot((pk >> (lane & 7u)) & 1u);
+            const uint64_t mr = __ballot((pk >> (8u + (lane & 7u))) & 1u);
+            myF = lane == t ? mf : myF;
+            myR = lane == t ? mr : myR;
+            cF += (uint32_t)__popcll(mf);
+            cR += (uint32_t)__popcll(mr);
+        }
+        if (lane < GM_SW_TPC) {
+            d.mF[tile0 + lane] = myF;
+            d.mR[tile0 + lane] = myR;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) unknown += __shfl_down(unknown, o, 64);
+    if (lane == 0) {
+        red[w][RS_F] = cF;
+        red[w][RS_R] = cR;
+        red[w][RS_UN] = unknown;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t t = 0;
+        for (int i = 0; i < GM_BLOCK / 64; i++) t += red[i][threadIdx.x];
+        d.bcnt[threadIdx.x * GM_MAX_BLOCKS + blockIdx.x] = t;
+    }
+}
+
+/* One block: per-block offsets; then each list's first completed and first
+ * INACTIVE position (the block offset of the tile where the list starts plus
+ * the block's tiles before it), the per-list counts, and their sums over the
+ * Inside and the Border lists. */
+__global__ void __launch_bounds__(1024) k_rg_sweep_scan(GmDev d, uint32_t nb) {
+    __shared__ uint32_t acc[6];
+    GmStats *st = d.st;
+    if (st->halt) return;
+    uint32_t tot[3];
+    scan_block_counts<3>(d.bcnt, nb, tot);
+    if (threadIdx.x < 6) acc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        if (tot[RS_UN]) st->err |= GM_ERRF_MISSED;
+        if ((uint64_t)tot[RS_F] > (1ull << 27)) st->err |= GM_ERRF_KEYS;
+    }
+    __syncthreads();
+    const uint32_t T = d.L.tile[d.nlists];
+    const uint32_t per = T ? part_per(T, nb, GM_SW_TPC) : 1u;
+    for (uint32_t l = threadIdx.x; l <= d.nlists; l += 1024) {
+        const uint32_t t0 = d.L.tile[l];
+        uint32_t ef = tot[RS_F], er = tot[RS_R];
+        if (t0 < T) {
+            const uint32_t b = t0 / per;
+            ef = d.bcnt[RS_F * GM_MAX_BLOCKS + b];
+            er = d.bcnt[RS_R * GM_MAX_BLOCKS + b];
+            for (uint32_t t = b * per; t < t0; t++) {
+                ef += (uint32_t)__popcll(d.mF[t]);
+                er += (uint32_t)__popcll(d.mR[t]);
+            }
+        }
+        d.L.par[l] = ef;
+        d.L.rst[l] = er;
+    }
+    __syncthreads();
+    uint32_t a[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t l = threadIdx.x; l < d.nlists; l += 1024) {
+        const uint32_t nf = d.L.par[l + 1] - d.L.par[l], nr = d.L.rst[l + 1] - d.L.rst[l];
+        const uint32_t rl = nr < d.L.rcap[l] ? nr : d.L.rcap[l];
+        d.L.rlen[l] = rl;
+        const uint32_t k = (l & 1u) * 3u;
+        a[k] += nf;
+        a[k + 1] += nr;
+        a[k + 2] += rl;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+        if (a[c]) atomicAdd(&acc[c], a[c]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->nfullI = acc[0];
+        st->totRecI = acc[1];
+        st->recLenI = acc[2];
+        st->nfullB = acc[3];
+        st->totRecB = acc[4];
+        st->recLenB = acc[5];
+    }
+}
+
+/* One wave per sweep block, one lane per tile: the completed slots in FullRX
+ * order (append_to_fullrx_*_list, SCORE_Kernel.cpp:3994-4042, :4133-4180), the
+ * lists one after another, and each list's INACTIVE slots in ascending order up
+ * to its recycling capacity (append_to_recycling_*_list, :4037-4041, :4175-4179). */
+__global__ void __launch_bounds__(64) k_rg_compact(GmDev d) {
+    if (d.st->halt) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t T = d.L.tile[d.nlists];
+    uint32_t lo, hi;
+    part_of(T, GM_SW_TPC, lo, hi);
+    uint32_t runF = d.bcnt[RS_F * GM_MAX_BLOCKS + blockIdx.x], runR = d.bcnt[RS_R * GM_MAX_BLOCKS + blockIdx.x];
+    for (uint32_t batch = lo; batch < hi; batch += 64) {
+        const uint32_t t = batch + lane;
+        const bool valid = t < hi;
+        const uint64_t bf = valid ? d.mF[t] : 0ull, br = valid ? d.mR[t] : 0ull;
+        const uint32_t cf = (uint32_t)__popcll(bf), cr = (uint32_t)__popcll(br);
+        const uint32_t incf = wave_incl_scan(cf), incr = wave_incl_scan(cr);
+        uint32_t of = runF + incf - cf, orr = runR + incr - cr;
+        runF += __shfl(incf, 63, 64);
+        runR += __shfl(incr, 63, 64);
+        if (bf | br) {
+            const uint32_t l = upper_index(d.L.tile, d.nlists, t);
+            const uint32_t rel0 = (t - d.L.tile[l]) * 64u;
+            const uint32_t s0 = d.L.base[l] + rel0;
+            for (uint64_t m = bf; m; m &= m - 1ull) d.pslot[of++] = s0 + (uint32_t)__ffsll((long long)m) - 1u;
+            uint32_t k = orr - d.L.rst[l];
+            const uint32_t cap = d.L.rcap[l];
+            uint32_t *rb = d.L.rbuf + d.L.rbase[l];
+            for (uint64_t m = br; m && k < cap; m &= m - 1ull) rb[k++] = rel0 + (uint32_t)__ffsll((long long)m) - 1u;
+        }
+    }
+}
+
+hipError_t gm_launch_rg_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev) {
+    hipError_t e;
+    GmSweepArgs A = a;
+    A.reanchor = 0;
+    if ((e = mark(ev, 0, s)) != hipSuccess) return e;
+    if (a.reanchor && (e = gm_launch_reanchor(d, a, s)) != hipSuccess) return e;
+    const uint64_t tiles = (uint64_t)(a.nI + a.nB) / 64u + (uint64_t)GM_SW_TPC * (d.nlists + 1u);
+    const uint32_t nb = grid_for((uint32_t)std::min<uint64_t>(tiles, 0xFFFFFFFFull), GM_SW_TPC * (GM_BLOCK / 64));
+    if ((e = mark(ev, 1, s)) != hipSuccess) return e;
+    k_rg_sweep<<<nb, GM_BLOCK, 0, s>>>(d, A);
+    if ((e = mark(ev, 2, s)) != hipSuccess) return e;
+    k_rg_sweep_scan<<<1, 1024, 0, s>>>(d, nb);
+    if ((e = mark(ev, 3, s)) != hipSuccess) return e;
+    k_rg_compact<<<nb, 64, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+/* --------------------------------------------------------------- infection */
+
+/* GM_PG lanes per parent, as the one-region propose pass: the parent's data
+ * into the dense array (with its list in place of the Border flag), its
+ * retirement into the log, and every attempt on a deformed cell claims it with
+ * its 64-bit key. */
+__global__ void __launch_bounds__(GM_BLOCK) k_rg_propose(GmDev d) {
+    if (d.st->halt) return;
+    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
+    const unsigned long long nret = d.st->nret;
+    const bool logged = nret + nfull <= d.rlogCap;
+    if (!logged && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&d.st->err, GM_ERRF_RLOG_FULL);
+    const uint32_t lg = threadIdx.x & (GM_PG - 1);
+    const uint32_t ngroups = gridDim.x * (GM_BLOCK / GM_PG);
+    for (uint32_t p = blockIdx.x * (GM_BLOCK / GM_PG) + threadIdx.x / GM_PG; p < nfull; p += ngroups) {
+        uint4 q2 = make_uint4(0u, 0u, 0u, 0u);
+        if (lg == 0) {
+            const uint32_t s = d.pslot[p];
+            const GmRec *r = d.rec + s;
+            const RecC c = rec_c(r);
+            const double fin = r->fin;
+            const uint32_t rxg = r->rxg;
+            const uint32_t cell = c.cell;
+            const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
+            GmPar *o = d.par + p;
+            q2 = make_uint4(x, y, z, list_of_slot(d, s));
+            *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
+            *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
+            *reinterpret_cast<uint4 *>(&o->x) = q2;
+            if (logged) d.rlog[nret + p] = make_uint2(cell, d.ndefg + rxg);
+        }
+        q2.x = (uint32_t)__shfl((int)q2.x, 0, GM_PG);
+        q2.y = (uint32_t)__shfl((int)q2.y, 0, GM_PG);
+        q2.z = (uint32_t)__shfl((int)q2.z, 0, GM_PG);
+        q2.w = (uint32_t)__shfl((int)q2.w, 0, GM_PG);
+        const uint32_t gsh = (threadIdx.x & 63) & ~(GM_PG - 1);
+        uint32_t am = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < GM_PG_ITERS; i++) {
+            const uint32_t rk = lg + GM_PG * i;
+            bool att = false;
+            if (rk < 26) {
+                RgTarget o;
+                if (rg_target(d, q2.w, q2.x, q2.y, q2.z, (int)rk, o) && is_deformed(d, o.x, o.y, o.z)) {
+                    att = true;
+                    atomicMin(&d.claim64[cell_of(d, o.x, o.y, o.z)], o.khi | (unsigned long long)(p * 32u + rk));
+                }
+            }
+            am |= (uint32_t)((__ballot(att) >> gsh) & ((1ull << GM_PG) - 1u)) << (GM_PG * i);
+        }
+        if (lg == 0) d.pmI[p] = am;
+    }
+}
+
+/* One lane per parent: the attempts that hold their target's smallest key, as
+ * masks by destination (own Inside, own Border, a halo), counted per block by
+ * row (GM_RW_*); the partition into blocks is the commit pass's. */
+__global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
+    if (d.st->halt) return;
+    const uint32_t nfull = d.st->nfullI + d.st->nfullB;
+    uint32_t lo, hi;
+    part_of(nfull, GM_PCHUNK, lo, hi);
+    uint32_t cnt[GM_RROWS];
+#pragma unroll
+    for (int c = 0; c < GM_RROWS; c++) cnt[c] = 0;
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += 64) {
+        const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+        const uint32_t am = d.pmI[p];
+        uint32_t mI = 0, mB = 0, mH = 0;
+        for (uint32_t m = am; m; m &= m - 1u) {
+            const uint32_t rk = (uint32_t)__ffs(m) - 1u;
+            RgTarget o;
+            rg_target(d, q2.w, q2.x, q2.y, q2.z, (int)rk, o);
+            if (d.claim64[cell_of(d, o.x, o.y, o.z)] != (o.khi | (unsigned long long)(p * 32u + rk))) continue;
+            if (o.khi) {
+                mH |= 1u << rk;
+#pragma unroll
+                for (uint32_t j = 0; j < GM_NHALO; j++) cnt[GM_RW_H0 + j] += o.hidx == j ? 1u : 0u;
+            } else if (o.dl & 1u) {
+                mB |= 1u << rk;
+            } else {
+                mI |= 1u << rk;
+            }
+        }
+        d.pmI[p] = mI;
+        d.pmB[p] = mB;
+        d.pmH[p] = mH;
+        cnt[GM_RW_I] += (uint32_t)__popc(mI);
+        cnt[GM_RW_B] += (uint32_t)__popc(mB);
+        cnt[GM_RW_A] += (uint32_t)__popc(am);
+    }
+#pragma unroll
+    for (int c = 0; c < GM_RROWS; c++) {
+        uint32_t v = cnt[c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (threadIdx.x == 0) d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x] = v;
+    }
+}
+
+/* the halo index of every parked winner of parent p (whose data is q2), as
+ * GM_NHALO masks */
+static __device__ __forceinline__ void halo_masks(const GmDev &d, const uint4 &q2, uint32_t mH, uint32_t *mh) {
+#pragma unroll
+    for (uint32_t j = 0; j < GM_NHALO; j++) mh[j] = 0;
+    for (uint32_t m = mH; m; m &= m - 1u) {
+        const uint32_t rk = (uint32_t)__ffs(m) - 1u;
+        RgTarget o;
+        rg_target(d, q2.w, q2.x, q2.y, q2.z, (int)rk, o);
+#pragma unroll
+        for (uint32_t j = 0; j < GM_NHALO; j++) mh[j] |= o.hidx == j ? 1u << rk : 0u;
+    }
+}
+
+/* One block: per-block offsets of every row; then, per region, the counts of
+ * each row before its first parent (block offset plus the block's parents
+ * before it); per region, where each neighbour's halo winners start in its
+ * Border list (after its own winners, in its neighbour order: the infecting
+ * region S = nbor[nb] parked them in its halo opp(nb)); the new records per
+ * list, the lists' new lengths (recycled slots first, then appended) and their
+ * prefixes for the next step. */
+__global__ void __launch_bounds__(1024) k_rg_decide_scan(GmDev d, uint32_t nb) {
+    __shared__ uint32_t acc[2];
+    GmStats *st = d.st;
+    if (st->halt) return;
+    uint32_t tot[GM_RROWS];
+    scan_block_counts<GM_RROWS>(d.bcnt, nb, tot);
+    const uint32_t nfull = st->nfullI + st->nfullB;
+    const uint32_t per = nfull ? part_per(nfull, nb, GM_PCHUNK) : 1u;
+    if (threadIdx.x < 2) acc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) st->nattempt = tot[GM_RW_A];
+    for (uint32_t r = threadIdx.x; r <= d.nreg; r += 1024) {
+        const uint32_t p0 = d.L.par[2u * r];
+        uint32_t e[GM_RROWS];
+#pragma unroll
+        for (int c = 0; c < GM_RROWS; c++) e[c] = tot[c];
+        if (p0 < nfull) {
+            const uint32_t b = p0 / per;
+#pragma unroll
+            for (int c = 0; c < GM_RROWS; c++) e[c] = d.bcnt[c * GM_MAX_BLOCKS + b];
+            for (uint32_t p = b * per; p < p0; p++) {
+                e[GM_RW_I] += (uint32_t)__popc(d.pmI[p]);
+                e[GM_RW_B] += (uint32_t)__popc(d.pmB[p]);
+                const uint32_t mH = d.pmH[p];
+                if (mH) {
+                    uint32_t mh[GM_NHALO];
+                    halo_masks(d, *reinterpret_cast<const uint4 *>(&d.par[p].x), mH, mh);
+#pragma unroll
+                    for (uint32_t j = 0; j < GM_NHALO; j++) e[GM_RW_H0 + j] += (uint32_t)__popc(mh[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < GM_RROWS; c++) d.L.rs[r * GM_RROWS + c] = e[c];
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < d.nreg; q += 1024) {
+        const uint32_t *a = d.L.rs + q * GM_RROWS, *b = a + GM_RROWS;
+        uint32_t run = b[GM_RW_B] - a[GM_RW_B];
+        for (uint32_t j = 0; j < GM_NHALO; j++) {
+            const uint32_t nbj = c_hdir[j], S = d.reg[q].nbor[nbj];
+            d.L.hoff[q * GM_NHALO + j] = run;
+            if (S != q) {
+                const uint32_t hs = GM_RW_H0 + c_hidx[c_opp[nbj]];
+                run += d.L.rs[(S + 1u) * GM_RROWS + hs] - d.L.rs[S * GM_RROWS + hs];
+            }
+        }
+        d.L.win[2u * q] = b[GM_RW_I] - a[GM_RW_I];
+        d.L.win[2u * q + 1u] = run;
+    }
+    __syncthreads();
+    uint32_t wI = 0, wB = 0;
+    for (uint32_t l = threadIdx.x; l < d.nlists; l += 1024) {
+        const uint32_t n = d.L.n[l], w = d.L.win[l], rl = d.L.rlen[l];
+        d.L.nold[l] = n;
+        d.L.n[l] = w > rl ? n + (w - rl) : n;
+        if (l & 1u) wB += w;
+        else wI += w;
+    }
+    if (wI) atomicAdd(&acc[0], wI);
+    if (wB) atomicAdd(&acc[1], wB);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->winI = acc[0];
+        st->winB = acc[1];
+    }
+    rg_prefix_block(d);
+}
+
+/* New records and retiring parents as the one-region commit pass writes them;
+ * a winner's slot is its rank in its list: own Inside and own Border winners
+ * by the parents' prefix counts less the count at the region's first parent,
+ * halo winners after the receiving region's own, by neighbour, then by the
+ * infecting region's prefix count of that halo.  A halo winner's weight is
+ * looked up again whatever its deformed grain (SCORE_Kernel.cpp:4853). */
+__global__ void __launch_bounds__(64) k_rg_commit(GmDev d, GmPParams pp, GmSweepArgs A) {
+    __shared__ uint32_t sI[64], sB[64], oI[64], oB[64];
+    __shared__ uint32_t sH[GM_NHALO][64], oH[GM_NHALO][64];
+    __shared__ uint4 sq0[64], sq2[64];
+    __shared__ double sfin[64], sP[64];
+    __shared__ uint16_t wl[26 * 64];
+    GmStats *st = d.st;
+    if (st->halt) return;
+    const uint32_t k = st->kcur;
+    const uint32_t nfull = st->nfullI + st->nfullB;
+    const uint32_t lane = threadIdx.x;
+    uint32_t lo, hi;
+    part_of(nfull, GM_PCHUNK, lo, hi);
+    uint32_t run[GM_RROWS];
+#pragma unroll
+    for (int c = 0; c < GM_RROWS; c++) run[c] = d.bcnt[c * GM_MAX_BLOCKS + blockIdx.x];
+    for (uint32_t batch = lo; batch < hi; batch += 64) {
+        const uint32_t pt = batch + lane;
+        const bool valid = pt < hi;
+        const uint32_t mI = valid ? d.pmI[pt] : 0u, mB = valid ? d.pmB[pt] : 0u, mH = valid ? d.pmH[pt] : 0u;
+        uint4 q0 = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu), q2 = make_uint4(0u, 0u, 0u, 0u);
+        double2 fp = make_double2(0.0, 0.0);
+        if (valid) {
+            const GmPar *pr = d.par + pt;
+            q0 = *reinterpret_cast<const uint4 *>(&pr->slot);
+            fp = *reinterpret_cast<const double2 *>(&pr->fin);
+            q2 = *reinterpret_cast<const uint4 *>(&pr->x);
+        }
+        uint32_t mh[GM_NHALO];
+        halo_masks(d, q2, mH, mh);
+        /* exclusive prefixes within the batch, two 16-bit counts per scan
+         * (a batch holds at most 64*26 winners of a row) */
+        uint32_t cnt[GM_RROWS], ex[GM_RROWS], bt[GM_RROWS];
+        cnt[GM_RW_I] = (uint32_t)__popc(mI);
+        cnt[GM_RW_B] = (uint32_t)__popc(mB);
+        cnt[GM_RW_A] = 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < GM_NHALO; j++) cnt[GM_RW_H0 + j] = (uint32_t)__popc(mh[j]);
+#pragma unroll
+        for (int c = 0; c < GM_RROWS; c += 2) {
+            const uint32_t hi16 = c + 1 < GM_RROWS ? cnt[c + 1] : 0u;
+            const uint32_t packed = (cnt[c] << 16) | hi16;
+            const uint32_t incl = wave_incl_scan(packed);
+            const uint32_t e = incl - packed, t = __shfl(incl, 63, 64);
+            ex[c] = e >> 16;
+            bt[c] = t >> 16;
+            if (c + 1 < GM_RROWS) {
+                ex[c + 1] = e & 0xFFFFu;
+                bt[c + 1] = t & 0xFFFFu;
+            }
+        }
+        oI[lane] = run[GM_RW_I] + ex[GM_RW_I];
+        oB[lane] = run[GM_RW_B] + ex[GM_RW_B];
+#pragma unroll
+        for (uint32_t j = 0; j < GM_NHALO; j++) {
+            oH[j][lane] = run[GM_RW_H0 + j] + ex[GM_RW_H0 + j];
+            sH[j][lane] = mh[j];
+        }
+        sI[lane] = mI;
+        sB[lane] = mB;
+        sq0[lane] = q0;
+        sq2[lane] = q2;
+        sfin[lane] = fp.x;
+        sP[lane] = fp.y;
+        /* this parent's winners at their place in the flat list, by rank */
+        const uint32_t mall = mI | mB | mH, call = (uint32_t)__popc(mall);
+        const uint32_t inc = wave_incl_scan(call);
+        const uint32_t nw = __shfl(inc, 63, 64);
+        uint32_t pos = inc - call;
+        for (uint32_t m = mall; m; m &= m - 1u) wl[pos++] = (uint16_t)(lane | ((uint32_t)__ffs(m) - 1u) << 8);
+        __syncthreads();
+        for (uint32_t w = lane; w < nw; w += 64) {
+            const uint32_t e = wl[w], j = e & 0xFFu, r = e >> 8, bit = 1u << r;
+            const uint4 pq0 = sq0[j], pq2 = sq2[j];
+            RgTarget o;
+            rg_target(d, pq2.w, pq2.x, pq2.y, pq2.z, (int)r, o);
+            const uint32_t t = cell_of(d, o.x, o.y, o.z);
+            const uint32_t g = d.grid[t];
+            const uint32_t *rsp = d.L.rs + (pq2.w >> 1) * GM_RROWS; /* the parent's region at its first parent */
+            uint32_t rank;
+            if (o.khi) {
+                const uint32_t q = o.dl >> 1, nbi = c_hidx[(uint32_t)(o.khi >> 35) & 31u];
+                rank = d.L.hoff[q * GM_NHALO + nbi] + (oH[o.hidx][j] - rsp[GM_RW_H0 + o.hidx]) +
+                       (uint32_t)__popc(sH[o.hidx][j] & (bit - 1u));
+            } else if (o.dl & 1u) {
+                rank = oB[j] - rsp[GM_RW_B] + (uint32_t)__popc(sB[j] & (bit - 1u));
+            } else {
+                rank = oI[j] - rsp[GM_RW_I] + (uint32_t)__popc(sI[j] & (bit - 1u));
+            }
+            const uint32_t l = o.dl, rl = d.L.rlen[l];
+            const uint32_t rel = rank < rl ? d.L.rbuf[d.L.rbase[l] + rank] : d.L.nold[l] + (rank - rl);
+            if (rel >= d.L.cap[l]) {
+                atomicOr(&st->err, (l & 1u) ? GM_ERRF_BORDER_FULL : GM_ERRF_INSIDE_FULL);
+                continue;
+            }
+            infect_one(d, pp, A, k, sfin[j], sP[j], pq0, r, t, g, o.x, o.y, o.z, d.L.base[l] + rel, o.khi != 0ull);
+        }
+        /* retire the batch's parents (updateFullRX*, SCORE_Kernel.cpp:4640-4653),
+         * one atomic per run of equal grain ids among neighbouring lanes */
+        if (valid) d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
+        const uint32_t g = q0.z, rx = q0.w;
+        const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(rx, 1, 64);
+        const uint64_t gstart = __ballot(valid && (lane == 0 || g != gprev));
+        const uint64_t rstart = __ballot(valid && (lane == 0 || rx != rprev));
+        const uint64_t vmask = __ballot(valid);
+        if (valid) {
+            const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+            if ((gstart >> lane) & 1ull) {
+                const uint64_t nxt = (gstart | ~vmask) & above;
+                const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+                const uint32_t n = end - lane;
+                const uint32_t old = atomicSub(&d.defcnt[g], n);
+                if (old == n) {
+                    const uint32_t i = atomicAdd(&st->nzeroed, 1u);
+                    if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
+                    if (i < d.ndefg) d.zlist[i] = g;
+                } else if (old < n) {
+                    st->cwrap = 1u;
+                }
+            }
+            if ((rstart >> lane) & 1ull) {
+                const uint64_t nxt = (rstart | ~vmask) & above;
+                const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+                atomicAdd(&d.rxcnt[rx], end - lane);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < GM_RROWS; c++) run[c] += bt[c];
+        __syncthreads();
+    }
+}
+
+hipError_t gm_launch_rg_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
+                               hipStream_t s, hipEvent_t *ev) {
+    hipError_t e;
+    if ((e = mark(ev, 4, s)) != hipSuccess) return e;
+    k_rg_propose<<<grid_for(parents_upper, GM_BLOCK / GM_PG), GM_BLOCK, 0, s>>>(d);
+    const uint32_t nb = grid_for(parents_upper, GM_PCHUNK);
+    if ((e = mark(ev, 5, s)) != hipSuccess) return e;
+    k_rg_decide<<<nb, 64, 0, s>>>(d);
+    if ((e = mark(ev, 6, s)) != hipSuccess) return e;
+    k_rg_decide_scan<<<1, 1024, 0, s>>>(d, nb);
+    if ((e = mark(ev, 7, s)) != hipSuccess) return e;
+    /* no separate recycling pass: the compaction wrote the recycled slots */
+    if ((e = mark(ev, 8, s)) != hipSuccess) return e;
+    GmSweepArgs A = a;
+    A.reanchor = 0;
+    k_rg_commit<<<nb, 64, 0, s>>>(d, pp, A);
+    if ((e = mark(ev, 9, s)) != hipSuccess) return e;
+    return hipGetLastError();
+}

@@ -7,6 +7,7 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from score_amd import abi, synth
@@ -139,6 +140,16 @@ def test_invalid_arguments_rejected_before_device_use():
     for nmax in (0, -5, 1000001):
         cp, keep_p = synth.ParamsSpec(nmax=nmax).to_c()
         assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    # region counts outside [0, 1024], or regions 10 cells thick or less (8^3 at T = 2)
+    for T in (-1, 1025, 2):
+        cp, keep_p = synth.ParamsSpec(n_regions=T).to_c()
+        assert lib.score_gm_create(C.byref(cd), C.byref(cp), 0, C.byref(h)) == abi.ERR_ARG
+    # nuclei arriving after step 0 have no counterpart in SCORE's region path
+    dom2 = synth.make_voronoi_domain(12, 12, 24, n_grains=2, n_nuclei=3, seed=1)
+    dom2.rxg_step = np.array([0, 2, 0], np.uint32)
+    cd2, keep_d2 = dom2.to_c()
+    cp, keep_p = synth.ParamsSpec(n_regions=2).to_c()
+    assert lib.score_gm_create(C.byref(cd2), C.byref(cp), 0, C.byref(h)) == abi.ERR_UNSUPPORTED
     assert not h.value
 
 
