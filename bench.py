@@ -373,6 +373,8 @@ def main(argv=None) -> int:
     ap.add_argument("--profile-run-only", action="store_true",
                     help="only the profiled run (one step per launch window, events between kernels); for rocprofv3")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--profile-regions", type=int, default=0,
+                    help="with --profile-run-only: profile the run with this many memory regions")
     ap.add_argument("--batch", type=int, default=0,
                     help="ensemble mode: this many independent domains, domains/hour batched vs one at a time")
     ap.add_argument("--inflight", type=int, default=8, help="domains in flight in ensemble and served modes")
@@ -449,6 +451,8 @@ def main(argv=None) -> int:
         return t2 - t1, res, pf, h
 
     if a.profile_run_only:
+        if a.profile_regions > 1:
+            prm = dataclasses.replace(prm, n_regions=a.profile_regions)
         psecs, pres, pf, h = one_run(True)
         log(f"profiled run: {psecs:.3f}s, {pf['steps']} steps, grain map {h}; per-kernel ms "
             f"{json.dumps({k: round(v, 3) for k, v in pf['ms_kernel'].items()})}")

@@ -415,6 +415,119 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
 }
 
 
+/* Retires a batch of parents (updateFullRX*, SCORE_Kernel.cpp:4640-4653): each
+ * lane's parent, q0 = (slot, cell, defg, rxg), becomes INACTIVE and moves a cell
+ * from its deformed grain's count to its nucleus'.  A deformed grain losing its
+ * last cell is reported so the host can update rhomax (update_myrhomax,
+ * SCORE_RecoveryModels.cpp:146-165).  Consecutive parents often share their
+ * nucleus and deformed grain (siblings), so each run of equal ids among
+ * neighbouring lanes makes one atomic.  Every lane of the wave takes part. */
+static __device__ __forceinline__ void retire_batch(const GmDev &d, const uint4 &q0, bool valid) {
+    GmStats *st = d.st;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (valid) d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
+    const uint32_t g = q0.z, r = q0.w;
+    const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(r, 1, 64);
+    const uint64_t gstart = __ballot(valid && (lane == 0 || g != gprev));
+    const uint64_t rstart = __ballot(valid && (lane == 0 || r != rprev));
+    const uint64_t vmask = __ballot(valid);
+    if (!valid) return;
+    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1)); /* lanes after this one */
+    if ((gstart >> lane) & 1ull) {
+        /* run length: up to the next run start or the last valid lane */
+        const uint64_t nxt = (gstart | ~vmask) & above;
+        const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+        const uint32_t n = end - lane;
+        const uint32_t old = atomicSub(&d.defcnt[g], n);
+        if (old == n) {
+            const uint32_t i = atomicAdd(&st->nzeroed, 1u);
+            if (i < GM_ZEROED_INLINE) st->zeroed_ids[i] = g;
+            if (i < d.ndefg) d.zlist[i] = g;
+        } else if (old < n) {
+            /* only a cell held by two records retires twice; the count wraps
+             * as SCORE's uint32 does and the host re-reads them all */
+            st->cwrap = 1u;
+        }
+    }
+    if ((rstart >> lane) & 1ull) {
+        const uint64_t nxt = (rstart | ~vmask) & above;
+        const uint32_t end = nxt ? (uint32_t)__ffsll((long long)nxt) - 1u : 64u;
+        atomicAdd(&d.rxcnt[r], end - lane);
+    }
+}
+
+/* The row of the step just done, and whether the next one may run without the
+ * host: not after an error, a count that wrapped, the exhaustion of the grain
+ * that sets rhomax (any_top), Xcells reaching a threshold the host acts on, a
+ * weight the window cannot wait for, or lists outgrowing their allocations.
+ * One thread. */
+static __device__ void step_finish(const GmDev &d, const GmWindow &w, const GmStepRow &r, bool any_top) {
+    GmStats *st = d.st;
+    const uint32_t j = st->nexec;
+    st->rows[j] = r;
+    st->nexec = j + 1;
+    st->kcur = st->kcur + 1u;
+    st->nret = st->nret + (unsigned long long)(r.nfullI + r.nfullB);
+    const uint32_t npend = __hip_atomic_load(&st->npend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->npend0 = npend;
+    const double xc = st->xcells + (double)(r.nfullI + r.nfullB);
+    st->xcells = xc;
+    /* the next step's arrivals may go to either list */
+    const uint32_t kn = st->kcur;
+    const uint64_t narr = (kn >= 1u && kn <= d.arr_last) ? (uint64_t)d.arr_off[kn + 1u] - d.arr_off[kn] : 0ull;
+    const uint64_t slots = (uint64_t)r.nI + r.nB + narr;
+    const uint64_t nIn = (uint64_t)r.nI + narr < 0xFFFFFFC0ull ? (uint64_t)r.nI + narr : 0xFFFFFFC0ull;
+    /* with several regions each list is swept in whole 8-tile chunks */
+    const uint64_t tiles = d.nreg > 1u ? slots / 64u + (uint64_t)GM_SW_TPC * (d.nlists + 1u)
+                                       : (uint64_t)gm_tiles_inside((uint32_t)nIn) + ((uint64_t)r.nB + narr + 63) / 64 + GM_SW_TPC;
+    bool halt = r.err != 0u || __hip_atomic_load(&st->cwrap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                any_top || xc >= w.xstop;
+    halt = halt || (npend != 0u && !w.defer) || npend > w.pend_limit;
+    halt = halt || __hip_atomic_load(&st->novf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    halt = halt || slots > d.parentCap || tiles > d.tileCap;
+    st->halt = halt ? 1u : 0u;
+}
+
+/* whether one of the grains exhausted in this step (zlist[z0, z1)) sets rhomax;
+ * every lane of the calling wave takes part */
+static __device__ __forceinline__ bool top_exhausted(const GmDev &d, const GmWindow &w, uint32_t z0, uint32_t z1) {
+    bool top = false;
+    for (uint32_t i = z0 + (threadIdx.x & 63u); i < z1; i += 64) top |= d.zlist[i] == w.top_grain;
+    return __ballot(top) != 0ull;
+}
+
+/* The end of a step, run by one wave after the commit pass: the step's row and
+ * whether the next step may run. */
+static __device__ void step_tail(const GmDev &d, const GmWindow &w) {
+    GmStats *st = d.st;
+    const uint32_t j = st->nexec;
+    const uint32_t z0 = j ? st->rows[j - 1].nzeroed : 0u, z1 = st->nzeroed < d.ndefg ? st->nzeroed : d.ndefg;
+    const bool any_top = top_exhausted(d, w, z0, z1);
+    if (threadIdx.x != 0) return;
+    GmStepRow r;
+    r.nI = st->nI;
+    r.nB = st->nB;
+    r.nfullI = st->nfullI;
+    r.nfullB = st->nfullB;
+    r.totRecI = st->totRecI;
+    r.totRecB = st->totRecB;
+    r.recLenI = st->recLenI;
+    r.recLenB = st->recLenB;
+    r.winI = st->winI;
+    r.winB = st->winB;
+    r.npend = st->npend - st->npend0;
+    r.nattempt = st->nattempt;
+    r.nzeroed = z1;
+    r.err = st->err;
+    r.nplaced = st->nplaced;
+    r.pone = st->pone;
+    r.pkey = st->arr_pkey;
+    st->nplaced = 0;
+    st->pone = 0;
+    st->arr_pkey = 0ull;
+    step_finish(d, w, r, any_top);
+}
+
 /* the l with a[l] <= v < a[l+1] among a[0..n], a nondecreasing and
  * a[0] <= v < a[n]: the last of equal entries, so an empty list is skipped */
 static __device__ __forceinline__ uint32_t upper_index(const uint32_t *a, uint32_t n, uint32_t v) {

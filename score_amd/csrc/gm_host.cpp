@@ -317,8 +317,9 @@ void free_device(score_gm_handle *h) {
                     d.par, d.recI, d.recB, d.mF, d.mR, d.pmI, d.pmB, d.bcnt, d.rho, d.defq,
                     d.rxq, d.defcnt, d.rxcnt, d.zlist, d.hkeys, d.hvals, d.pend_slot, d.pend_key, d.nflag, d.nI_, d.nB_,
                     d.scan_bsum, d.st, d.arr_ids, d.arr_off, d.nsite, d.nucP,
-                    d.reg, d.claim64, d.pmH, d.L.base, d.L.cap, d.L.rcap, d.L.rbase, d.L.n, d.L.nold, d.L.rlen,
-                    d.L.win, d.L.occ, d.L.tile, d.L.par, d.L.rst, d.L.rbuf, d.L.rs, d.L.hoff};
+                    d.reg, d.pmH, d.L.base, d.L.cap, d.L.rcap, d.L.rbase, d.L.n, d.L.nold, d.L.rlen,
+                    d.L.win, d.L.occ, d.L.tile, d.L.tile_s, d.L.par, d.L.rst, d.L.rbuf, d.L.rs, d.L.bstart, d.L.breg,
+                    d.L.hoff, d.plist};
     for (void *q : ptrs) dfree(h, q);
     memset(&h->d, 0, sizeof(h->d));
     if (h->hst) (void)hipHostFree(h->hst);
@@ -817,13 +818,13 @@ int ensure_step_capacity(score_gm_handle *h) {
     const uint32_t parents = h->nI + h->nB + na;
     if (parents > d.parentCap) {
         uint32_t nc = std::max<uint32_t>(parents, d.parentCap + d.parentCap / 2);
-        dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par); dfree(h, d.pmH);
-        d.pmI = d.pmB = d.pslot = d.pmH = nullptr;
+        dfree(h, d.pmI); dfree(h, d.pmB); dfree(h, d.pslot); dfree(h, d.par); dfree(h, d.pmH); dfree(h, d.plist);
+        d.pmI = d.pmB = d.pslot = d.pmH = d.plist = nullptr;
         d.par = nullptr;
         d.parentCap = 0;
         int rc;
         if ((rc = dalloc(h, &d.pmI, nc)) || (rc = dalloc(h, &d.pmB, nc)) || (rc = dalloc(h, &d.pslot, nc)) ||
-            (rc = dalloc(h, &d.par, nc)) || (d.nreg > 1 && (rc = dalloc(h, &d.pmH, nc))))
+            (rc = dalloc(h, &d.par, nc)) || (d.nreg > 1 && ((rc = dalloc(h, &d.pmH, nc)) || (rc = dalloc(h, &d.plist, nc)))))
             return rc;
         d.parentCap = nc;
     }
@@ -1083,7 +1084,8 @@ int step_error(uint32_t err) {
     if (err & GM_ERRF_INSIDE_FULL) return fail(SCORE_GM_ERR_CAPACITY, "Inside front list full");
     if (err & GM_ERRF_MISSED) return fail(SCORE_GM_ERR_STATE, "internal error: missed completion");
     if (err & GM_ERRF_KEYS)
-        return fail(SCORE_GM_ERR_UNSUPPORTED, "more than 2^27 cells completed in one step (infection keys are 32-bit)");
+        return fail(SCORE_GM_ERR_UNSUPPORTED,
+                    "more cells completed in one step than 32-bit infection keys hold (2^27; 2^23 with several regions)");
     if (err & GM_ERRF_RLOG_FULL) return fail(SCORE_GM_ERR_CAPACITY, "retirement log full (cells retired twice)");
     if (err & GM_ERRF_HALO) return fail(SCORE_GM_ERR_STATE, "internal error: an infection into another region found no halo");
     return fail(SCORE_GM_ERR_STATE, "internal error: mobility-weight cache entry lost while growing it");
@@ -1188,7 +1190,7 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
         GmSweepArgs aj = a;
         if (j) aj.reanchor = 0;
         if (d.nreg > 1) {
-            if ((e = gm_launch_rg_sweep(d, aj, h->stream, ev)) != hipSuccess ||
+            if ((e = gm_launch_rg_sweep(d, aj, std::max(slots0, nIe + nBe), h->stream, ev)) != hipSuccess ||
                 (e = gm_launch_rg_infect(d, h->pp, aj, std::max(slots0, nIe + nBe), h->stream, ev)) != hipSuccess ||
                 (e = gm_launch_step_tail(d, w, h->stream)) != hipSuccess)
                 return h->err = hip_fail(e, "growth step");
@@ -1576,7 +1578,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     if (multi)
         for (uint32_t r = 0; r < d.nreg; r++) capBsum += h->lcap[2 * r + 1];
     d.rlogCap = d.N + capBsum + 4096u;
-    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, multi ? 1 : d.N)) ||
+    if ((rc = dalloc(h, &d.grid, d.N)) || (rc = dalloc(h, &d.claim, d.N)) ||
         (rc = dalloc(h, &d.rlog, d.rlogCap)) ||
         (rc = dalloc(h, &d.cstep, capTot)) ||
         (rc = dalloc(h, &d.rec, capTot)) ||
@@ -1599,11 +1601,14 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     }
     if (multi) {
         const uint32_t L = d.nlists;
-        if ((rc = dalloc(h, &d.reg, d.nreg)) || (rc = dalloc(h, &d.claim64, d.N)) || (rc = dalloc(h, &d.pmH, d.parentCap)) ||
+        if ((rc = dalloc(h, &d.reg, d.nreg)) || (rc = dalloc(h, &d.pmH, d.parentCap)) ||
+            (rc = dalloc(h, &d.plist, d.parentCap)) || (rc = dalloc(h, &d.L.bstart, d.nreg + 1)) ||
+            (rc = dalloc(h, &d.L.breg, GM_MAX_BLOCKS)) ||
             (rc = dalloc(h, &d.L.base, L + 1)) || (rc = dalloc(h, &d.L.cap, L)) || (rc = dalloc(h, &d.L.rcap, L)) ||
             (rc = dalloc(h, &d.L.rbase, L + 1)) || (rc = dalloc(h, &d.L.n, L)) || (rc = dalloc(h, &d.L.nold, L)) ||
             (rc = dalloc(h, &d.L.rlen, L)) || (rc = dalloc(h, &d.L.win, L)) || (rc = dalloc(h, &d.L.occ, L + 1)) ||
-            (rc = dalloc(h, &d.L.tile, L + 1)) || (rc = dalloc(h, &d.L.par, L + 1)) || (rc = dalloc(h, &d.L.rst, L + 1)) ||
+            (rc = dalloc(h, &d.L.tile, L + 1)) || (rc = dalloc(h, &d.L.tile_s, L + 1)) ||
+            (rc = dalloc(h, &d.L.par, L + 1)) || (rc = dalloc(h, &d.L.rst, L + 1)) ||
             (rc = dalloc(h, &d.L.rbuf, h->lrbase[L])) || (rc = dalloc(h, &d.L.rs, (size_t)(d.nreg + 1) * GM_RROWS)) ||
             (rc = dalloc(h, &d.L.hoff, (size_t)d.nreg * GM_NHALO))) {
             free_device(h);
@@ -1628,8 +1633,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     auto setup = [&]() -> int {
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(d.grid, dom->cellgrid, sizeof(uint32_t) * d.N, hipMemcpyHostToDevice, s));
-        if (multi) HIPCHK(gm_launch_fill_u64(d.claim64, ~0ull, d.N, s));
-        else HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
+        HIPCHK(gm_launch_fill_u32(d.claim, GM_FREE_CLAIM, d.N, s));
         HIPCHK(gm_launch_dmap_init(d, s));
         /* two INACTIVE codes per word (capTot is a multiple of 64) */
         HIPCHK(gm_launch_fill_u32((uint32_t *)d.cstep, 0xFFFFFFFFu, capTot / 2, s));
@@ -2004,7 +2008,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     const uint64_t front = std::max<uint64_t>(65536, N / 4);
     uint64_t b = 0;
     if (prm->n_regions > 1) {
-        /* the lists of every region (build_regions), 64-bit claims, halo masks and tables */
+        /* the lists of every region (build_regions), halo masks and tables */
         const uint32_t T = (uint32_t)prm->n_regions, npy = smallest_y_factor(T), npz = T / npy;
         const uint64_t nry = dom->ny / npy, nrz = dom->nz / npz, ex = dom->nx;
         capB = 0;
@@ -2019,7 +2023,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
                 rbuf += (uint64_t)(prm->rel_cell_caching * (double)vol * prm->maxfill * kFullRecyclingAtt) +
                         (uint64_t)((double)cb * prm->maxfill * kFullRecyclingAtt);
             }
-        b += 4 * N + 4 * front + (sizeof(GmRegion) + 4ull * (GM_RROWS + GM_NHALO + 26)) * (T + 1) + 16ull * GM_SW_TPC * 2 * T;
+        b += 4 * front + (sizeof(GmRegion) + 4ull * (GM_RROWS + GM_NHALO + 26)) * (T + 1) + 16ull * GM_SW_TPC * 2 * T;
     }
     b += 8 * N;                                   /* grid, claim */
     b += 8 * (N + capB + 4096);                   /* retirement log */

@@ -66,7 +66,7 @@ typedef uint16_t gm_cs_t;
 #define GM_ERRF_INSIDE_FULL 0x2u
 #define GM_ERRF_PCACHE_FULL 0x4u
 #define GM_ERRF_MISSED 0x8u       /* a prediction lies in the past: internal error */
-#define GM_ERRF_KEYS 0x10u        /* more than 2^27 completed cells in one step: infection keys would not fit */
+#define GM_ERRF_KEYS 0x10u        /* more completed cells in one step than infection keys hold (2^27; 2^23 with regions) */
 #define GM_ERRF_RLOG_FULL 0x20u   /* more retirements than the log holds (only cells retired twice can do it) */
 #define GM_ERRF_HALO 0x40u        /* an infection into another region found no halo to park in: internal error */
 
@@ -166,6 +166,7 @@ struct alignas(8) GmStats {
     uint32_t dfirst;          /* defragmentation: first INACTIVE slot of the list */
     uint32_t dcount;          /* defragmentation: ACTIVE slots moved; status vector: entries */
     uint32_t novf;            /* weight lookups that found the cache full (their cells wait for a larger table) */
+    uint32_t rg_per;          /* with several regions: parents per block of the infection passes */
     uint32_t nplaced;         /* nuclei placed at the start of the current step */
     unsigned long long arr_pkey; /* largest weight of those nuclei (gm_pkey order) */
     unsigned long long nret;  /* entries of the retirement log (cells retired so far) */
@@ -231,10 +232,13 @@ struct GmLists {
     uint32_t *win;              /* this step's new records into each list */
     uint32_t *occ;              /* exclusive prefix of n: occupied slot i lies in list l iff occ[l] <= i < occ[l+1] */
     uint32_t *tile;             /* exclusive prefix of the sweep's tiles per list (whole 8-tile chunks) */
+    uint32_t *tile_s;           /* the same as the current step's sweep saw it (tile moves on with the lengths) */
     uint32_t *par;              /* completed cells before each list's first, in FullRX order */
     uint32_t *rst;              /* INACTIVE slots before each list's first */
     uint32_t *rbuf;             /* recycled slots (list-relative), ascending per list */
     uint32_t *rs;               /* per region r (and r = T: totals), GM_RROWS prefix counts at its first parent */
+    uint32_t *bstart;           /* first block of each region in the infection passes (nreg+1: blocks in use) */
+    uint32_t *breg;             /* region of each block of the infection passes (GM_MAX_BLOCKS; ~0u past those in use) */
     uint32_t *hoff;             /* per region, GM_NHALO offsets of the halo winners in its Border list */
 };
 
@@ -291,8 +295,8 @@ struct GmDev {
     uint32_t npy, npz, nry, nrz;/* region grid and slab thickness along y and z */
     GmRegion *reg;              /* nreg entries */
     GmLists L;
-    unsigned long long *claim64;/* per-cell claim key with several regions (2^64-1 unclaimed) */
     uint32_t *pmH;              /* per parent: winning ranks into another region's halo */
+    uint32_t *plist;            /* per parent: its list */
 };
 
 /* mobility-weight evaluation parameters the kernels need */
@@ -328,7 +332,8 @@ hipError_t gm_launch_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, 
 /* every active cell advanced to step a.k with its old increment and predicted again */
 hipError_t gm_launch_reanchor(const GmDev &d, const GmSweepArgs &a, hipStream_t s);
 /* the same two halves of a step with several regions (gm_regions.hip) */
-hipError_t gm_launch_rg_sweep(const GmDev &d, const GmSweepArgs &a, hipStream_t s, hipEvent_t *ev);
+hipError_t gm_launch_rg_sweep(const GmDev &d, const GmSweepArgs &a, uint32_t parents_upper, hipStream_t s,
+                              hipEvent_t *ev);
 hipError_t gm_launch_rg_infect(const GmDev &d, const GmPParams &pp, const GmSweepArgs &a, uint32_t parents_upper,
                                hipStream_t s, hipEvent_t *ev);
 /* with several regions: the lists' prefixes and summed lengths from d.L.n */
