@@ -528,6 +528,50 @@ static __device__ void step_tail(const GmDev &d, const GmWindow &w) {
     step_finish(d, w, r, any_top);
 }
 
+/* orders a wave's LDS writes before its later LDS reads (the lanes of a wave
+ * run in lockstep; this keeps the compiler from reordering across it) */
+static __device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Classifies a lane's eight slots of a sweep chunk (2-byte codes in cs, the
+ * first nocc of them occupied): bit q of the low byte is set for a slot that
+ * completes at step code kc, bit 8+q for an INACTIVE slot.  unknown counts
+ * slots without a valid prediction: a cell is predicted before the sweep (when
+ * it is infected, or by k_reanchor) or is DEFERRED, NEVER or FAR, so a code a
+ * horizon or more ahead (a step in the past) or UNKNOWN is an internal error. */
+static __device__ __forceinline__ uint32_t sweep_classify(const uint4 &cs, uint32_t nocc, uint32_t kc,
+                                                          uint32_t horizon, uint32_t &unknown) {
+    const uint32_t wd[4] = {cs.x, cs.y, cs.z, cs.w};
+    uint32_t pack = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+        const uint32_t in = (uint32_t)q < nocc ? 1u : 0u;
+        pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
+        unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
+    }
+    return pack;
+}
+
+/* The chunk's per-tile masks from every lane's pack: tile t holds slots
+ * 64t .. 64t+63, the eight slots of each of lanes 8t .. 8t+7, so its completed
+ * mask is those lanes' low bytes in lane order and its INACTIVE mask their high
+ * bytes.  The bytes go through the wave's 128 bytes of LDS (sF, sR: eight
+ * 64-bit words each); lane t < 8 returns tile t's masks. */
+static __device__ __forceinline__ void sweep_masks(uint32_t pack, uint64_t *sF, uint64_t *sR, uint64_t &mF,
+                                                   uint64_t &mR) {
+    const uint32_t lane = threadIdx.x & 63u;
+    reinterpret_cast<uint8_t *>(sF)[lane] = (uint8_t)pack;
+    reinterpret_cast<uint8_t *>(sR)[lane] = (uint8_t)(pack >> 8);
+    wave_sync();
+    mF = lane < 8u ? sF[lane] : 0ull;
+    mR = lane < 8u ? sR[lane] : 0ull;
+    wave_sync();
+}
+
 /* the l with a[l] <= v < a[l+1] among a[0..n], a nondecreasing and
  * a[0] <= v < a[n]: the last of equal entries, so an empty list is skipped */
 static __device__ __forceinline__ uint32_t upper_index(const uint32_t *a, uint32_t n, uint32_t v) {

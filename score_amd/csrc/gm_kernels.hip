@@ -121,6 +121,7 @@ static __device__ __forceinline__ uint32_t tile_slot0(const GmDev &d, uint32_t t
 static_assert(GM_SW_TPC == 8, "a 16-byte load of 2-byte codes covers 8 slots per lane");
 __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
     __shared__ uint32_t red[GM_BLOCK / 64][5];
+    __shared__ uint64_t sF[GM_BLOCK / 64][8], sR[GM_BLOCK / 64][8];
     if (d.st->halt) return;
     const GmSweepArgs A = step_args(d, A0);
     const uint32_t nI = d.st->nI, nB = d.st->nB;
@@ -153,43 +154,32 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
         for (int u = 0; u < GM_SW_UNROLL; u++) {
             const uint32_t c = c0 + u * (GM_BLOCK / 64);
             if (c >= c_hi) break;
-            const uint32_t wd[4] = {cs[u].x, cs[u].y, cs[u].z, cs[u].w};
-            uint32_t pack = 0;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
-                const uint32_t in = (uint32_t)q < nocc[u] ? 1u : 0u;
-                pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
-                /* every cell is predicted before the sweep (when it is infected,
-                 * or by k_reanchor) or DEFERRED, NEVER or FAR: a step before k (a
-                 * code a horizon or more ahead) or a missing prediction is an
-                 * internal error */
-                unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
-            }
+            const uint32_t pack = sweep_classify(cs[u], nocc[u], kc, horizon, unknown);
             const uint32_t tile0 = c * GM_SW_TPC;
-            const bool isB = tile0 >= tI;
-            uint64_t myF = 0, myR = 0;
-#pragma unroll
-            for (uint32_t t = 0; t < GM_SW_TPC; t++) {
-                /* slot 64t + lane of the chunk is bit lane%8 of lane 8t + lane/8 */
-                const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(8u * t + (lane >> 3)), 64);
-                const uint64_t mf = __ballot((pk >> (lane & 7u)) & 1u);
-                const uint64_t mr = __ballot((pk >> (8u + (lane & 7u))) & 1u);
-                myF = lane == t ? mf : myF;
-                myR = lane == t ? mr : myR;
-                const uint32_t nf = (uint32_t)__popcll(mf), nr = (uint32_t)__popcll(mr);
-                cFB += isB ? nf : 0u;
-                cRB += isB ? nr : 0u;
-                cFI += isB ? 0u : nf;
-                cRI += isB ? 0u : nr;
+            /* per-lane counts, summed over the wave once at the end */
+            const uint32_t nf = (uint32_t)__popc(pack & 0xFFu), nr = (uint32_t)__popc(pack >> 8);
+            if (tile0 >= tI) {
+                cFB += nf;
+                cRB += nr;
+            } else {
+                cFI += nf;
+                cRI += nr;
             }
+            uint64_t myF, myR;
+            sweep_masks(pack, sF[w], sR[w], myF, myR);
             if (lane < GM_SW_TPC && tile0 + lane < hi) {
                 d.mF[tile0 + lane] = myF;
                 d.mR[tile0 + lane] = myR;
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) unknown += __shfl_down(unknown, o, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        unknown += __shfl_down(unknown, o, 64);
+        cFI += __shfl_down(cFI, o, 64);
+        cRI += __shfl_down(cRI, o, 64);
+        cFB += __shfl_down(cFB, o, 64);
+        cRB += __shfl_down(cRB, o, 64);
+    }
     if (lane == 0) {
         red[w][0] = cFI; red[w][1] = cRI; red[w][2] = cFB; red[w][3] = cRB; red[w][4] = unknown;
     }

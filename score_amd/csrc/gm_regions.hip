@@ -315,6 +315,7 @@ hipError_t gm_launch_rg_place(const GmDev &d, const uint32_t *slots, const GmRec
  * masks, counted per block. */
 __global__ void __launch_bounds__(GM_BLOCK) k_rg_sweep(GmDev d, GmSweepArgs A0) {
     __shared__ uint32_t red[GM_BLOCK / 64][3];
+    __shared__ uint64_t sF[GM_BLOCK / 64][8], sR[GM_BLOCK / 64][8];
     if (d.st->halt) return;
     const GmSweepArgs A = step_args(d, A0);
     const uint32_t T = d.L.tile[d.nlists];
@@ -333,32 +334,21 @@ __global__ void __launch_bounds__(GM_BLOCK) k_rg_sweep(GmDev d, GmSweepArgs A0) 
         const uint32_t nocc = lim > base ? (lim - base < 8u ? lim - base : 8u) : 0u;
         const uint4 cs = nocc ? *reinterpret_cast<const uint4 *>(d.cstep + base)
                               : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-        const uint32_t wd[4] = {cs.x, cs.y, cs.z, cs.w};
-        uint32_t pack = 0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
-            const uint32_t in = (uint32_t)q < nocc ? 1u : 0u;
-            pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
-            unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
-        }
-        uint64_t myF = 0, myR = 0;
-#pragma unroll
-        for (uint32_t t = 0; t < GM_SW_TPC; t++) {
-            const uint32_t pk = (uint32_t)__shfl((int)pack, (int)(8u * t + (lane >> 3)), 64);
-            const uint64_t mf = __ballot((pk >> (lane & 7u)) & 1u);
-            const uint64_t mr = __ballot((pk >> (8u + (lane & 7u))) & 1u);
-            myF = lane == t ? mf : myF;
-            myR = lane == t ? mr : myR;
-            cF += (uint32_t)__popcll(mf);
-            cR += (uint32_t)__popcll(mr);
-        }
+        const uint32_t pack = sweep_classify(cs, nocc, kc, horizon, unknown);
+        cF += (uint32_t)__popc(pack & 0xFFu);
+        cR += (uint32_t)__popc(pack >> 8);
+        uint64_t myF, myR;
+        sweep_masks(pack, sF[w], sR[w], myF, myR);
         if (lane < GM_SW_TPC) {
             d.mF[tile0 + lane] = myF;
             d.mR[tile0 + lane] = myR;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) unknown += __shfl_down(unknown, o, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        unknown += __shfl_down(unknown, o, 64);
+        cF += __shfl_down(cF, o, 64);
+        cR += __shfl_down(cR, o, 64);
+    }
     if (lane == 0) {
         red[w][RS_F] = cF;
         red[w][RS_R] = cR;
