@@ -108,6 +108,17 @@ def build_caller(out: str | None = None) -> str | None:
     return out
 
 
+def build_radd_device(out: str | None = None) -> str:
+    """tests/radd_device/radd_device: the device build of gm_radd.h against
+    plain device additions (a GPU test runs it)."""
+    out = out or os.path.join(ROOT, "tests", "radd_device", "radd_device")
+    flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
+    tmp = out + ".tmp"
+    _run([HIPCC, *flags, "-I", CSRC, os.path.join(ROOT, "tests", "radd_device", "radd_device.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 REFERENCE = "/root/reference"
 
 
@@ -128,6 +139,7 @@ def build_all(oracle: bool = True) -> None:
     build_io()
     build_setup()
     build_caller()
+    build_radd_device()
     if oracle:
         build_oracle()
 
@@ -135,7 +147,7 @@ def build_all(oracle: bool = True) -> None:
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true", help="also build the oracle (test infrastructure)")
-    ap.add_argument("--only", choices=["gm", "synth", "io", "setup", "caller", "oracle"], default=None)
+    ap.add_argument("--only", choices=["gm", "synth", "io", "setup", "caller", "radd", "oracle"], default=None)
     a = ap.parse_args(argv)
     if a.only == "gm":
         build_gm()
@@ -147,6 +159,8 @@ def main(argv: list[str] | None = None) -> int:
         build_setup()
     elif a.only == "caller":
         build_caller()
+    elif a.only == "radd":
+        build_radd_device()
     elif a.only == "oracle":
         build_oracle()
     else:

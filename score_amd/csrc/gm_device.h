@@ -363,6 +363,11 @@ static __device__ __forceinline__ void clear_deformed(const GmDev &d, uint32_t x
 }
 
 
+/* plane GM_PAR_* of the parent array */
+static __device__ __forceinline__ uint4 *par_plane(const GmDev &d, uint32_t plane) {
+    return d.par + (size_t)plane * d.parentCap;
+}
+
 static __device__ __forceinline__ uint32_t cell_of(const GmDev &d, uint32_t x, uint32_t y, uint32_t z) {
     return x + y * d.nx + z * d.nxy;
 }

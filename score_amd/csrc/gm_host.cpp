@@ -824,7 +824,7 @@ int ensure_step_capacity(score_gm_handle *h) {
         d.parentCap = 0;
         int rc;
         if ((rc = dalloc(h, &d.pmI, nc)) || (rc = dalloc(h, &d.pmB, nc)) || (rc = dalloc(h, &d.pslot, nc)) ||
-            (rc = dalloc(h, &d.par, nc)) || (d.nreg > 1 && ((rc = dalloc(h, &d.pmH, nc)) || (rc = dalloc(h, &d.plist, nc)))))
+            (rc = dalloc(h, &d.par, (size_t)GM_PAR_PLANES * nc)) || (d.nreg > 1 && ((rc = dalloc(h, &d.pmH, nc)) || (rc = dalloc(h, &d.plist, nc)))))
             return rc;
         d.parentCap = nc;
     }
@@ -1584,7 +1584,7 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         (rc = dalloc(h, &d.rec, capTot)) ||
         (rc = dalloc(h, &d.dlist, d.dlistCap)) ||
         (rc = dalloc(h, &d.dmap, (size_t)d.dnbx * d.dnby * d.dnbz * 32)) || (rc = dalloc(h, &d.pslot, d.parentCap)) ||
-        (rc = dalloc(h, &d.par, d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||
+        (rc = dalloc(h, &d.par, (size_t)GM_PAR_PLANES * d.parentCap)) || (rc = dalloc(h, &d.recI, d.recCapI)) || (rc = dalloc(h, &d.recB, d.recCapB)) ||
         (rc = dalloc(h, &d.mF, d.tileCap)) || (rc = dalloc(h, &d.mR, d.tileCap)) || (rc = dalloc(h, &d.pmI, d.parentCap)) ||
         (rc = dalloc(h, &d.pmB, d.parentCap)) || (rc = dalloc(h, &d.bcnt, GM_BCNT_ROWS * GM_MAX_BLOCKS)) ||
         (rc = dalloc(h, &d.rho, d.ndefg)) || (rc = dalloc(h, &d.defq, 4 * (size_t)d.ndefg)) ||
@@ -2029,7 +2029,7 @@ int score_gm_footprint(const score_gm_domain *dom, const score_gm_params *prm, u
     b += 8 * (N + capB + 4096);                   /* retirement log */
     b += 32 * bricks;                             /* deformed bitmap */
     b += (sizeof(gm_cs_t) + sizeof(GmRec)) * capTot; /* completion steps, records */
-    b += (4 + sizeof(GmPar) + 8) * front;         /* parents: slots, data, winner masks */
+    b += (4 + 16 * GM_PAR_PLANES + 8) * front;    /* parents: slots, data, winner masks */
     b += 4 * std::min<uint64_t>(N, 1u << 20);     /* deferred-cell list */
     b += 16 * (capTot / 64 + GM_SW_TPC);          /* tile masks */
     b += 4 * rbuf;                                /* recycled slots */

@@ -121,15 +121,22 @@ struct __attribute__((aligned(64))) GmRec {
     uint32_t pad[3];
 };
 static_assert(sizeof(GmRec) == 64, "GmRec is one 64-byte record");
+/* the kernels move records as four 16-byte quarters */
+static_assert(__builtin_offsetof(GmRec, inc) == 16 && __builtin_offsetof(GmRec, fin) == 24 &&
+                  __builtin_offsetof(GmRec, P) == 32 && __builtin_offsetof(GmRec, cell) == 40 &&
+                  __builtin_offsetof(GmRec, defg) == 44 && __builtin_offsetof(GmRec, rxg) == 48,
+              "GmRec quarters: (frac0, s0, inf) | (inc, fin) | (P, cell, defg) | (rxg, pad)");
 
-/* One completed cell of the current step, in FullRX list order (Inside list
- * first): what the infection kernels need of the parent, read coalesced. */
-struct __attribute__((aligned(16))) GmPar {
-    uint32_t slot, cell, defg, rxg;
-    double fin, P;
-    uint32_t x, y, z, fromB;
-};
-static_assert(sizeof(GmPar) == 48, "GmPar is three 16-byte quarters");
+/* The completed cells of the current step, in FullRX list order (Inside list
+ * first): what the infection kernels need of each parent, as three planes of
+ * parentCap 16-byte entries, so that a wave's loads of one plane are contiguous.
+ *   GM_PAR_IDS  (slot, cell, defg, rxg)
+ *   GM_PAR_FP   (fin, P) as two doubles
+ *   GM_PAR_XYZ  (x, y, z, Border flag or list) */
+#define GM_PAR_IDS 0u
+#define GM_PAR_FP 1u
+#define GM_PAR_XYZ 2u
+#define GM_PAR_PLANES 3u
 
 /* what one step of a window leaves for the host's master section */
 struct GmStepRow {
@@ -260,7 +267,7 @@ struct GmDev {
     GmRec *rec;
     /* per-step lists */
     uint32_t *pslot;            /* completed cells (global slots), Inside then Border, ascending each */
-    GmPar *par;                 /* the same cells' parent data */
+    uint4 *par;                 /* the same cells' parent data: GM_PAR_PLANES planes */
     uint32_t *recI, *recB;      /* recycled slots (list-relative), ascending, capped */
     uint64_t *mF, *mR;          /* per 64-slot tile: completed / INACTIVE masks */
     uint32_t *pmI, *pmB;        /* per parent: winning ranks into Inside / Border */

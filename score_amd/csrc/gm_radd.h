@@ -45,6 +45,38 @@ GM_HD double gm_dfrom(uint64_t b) {
 /* parity of an integer-valued double below 2^53 */
 GM_HD bool gm_odd(double v) { return v - 2.0 * __builtin_floor(v * 0.5) != 0.0; }
 
+/* floor(num / D) for integer-valued doubles 0 <= num < 2^53 and 1 <= D < 2^53.
+ * On the device the division sequence is replaced by the hardware reciprocal
+ * refined by two Newton steps; the estimate is then within one of the quotient,
+ * and the remainder num - m*D, an integer of magnitude below 2D and so exact in
+ * one fma, moves it to the floor.  The host divides. */
+#if defined(__HIP_DEVICE_COMPILE__)
+GM_HD double gm_floor_div(double num, double D) {
+    const double r0 = __builtin_amdgcn_rcp(D);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-D, r0, 1.0), r0);
+    const double r2 = __builtin_fma(r1, __builtin_fma(-D, r1, 1.0), r1);
+    double m = __builtin_floor(num * r2);
+    double r = __builtin_fma(-m, D, num);
+    while (r < 0.0) {
+        m -= 1.0;
+        r += D;
+    }
+    while (r >= D) {
+        m += 1.0;
+        r -= D;
+    }
+    return m;
+}
+#else
+GM_HD double gm_floor_div(double num, double D) {
+    /* the rounded quotient lies below 2^53, so its floor is the true floor or
+     * one more */
+    double m = __builtin_floor(num / D);
+    if (m * D > num) m -= 1.0;
+    return m;
+}
+#endif
+
 /* One round of at most `budget` additions: the run that provably stays inside
  * the binade of f, as one exact multiply-add, then one hardware addition (which
  * leaves the binade, or makes f/u even in a tie).  Counts are integer-valued
@@ -88,12 +120,9 @@ GM_HD double gm_radd_round(double &f, double inc, double budget) {
     if (D == 0.0) return 0.0; /* the increment is absorbed by rounding */
     /* addition i of the run keeps its exact sum below 2^(e+1) when
      * F + (i-1)*D + Q + 1 <= 2^53, so the run takes floor(num/D) + 1 additions
-     * with num = 2^53 - 1 - F - Q >= 0 (exact).  The rounded quotient lies below
-     * 2^53, so its floor is the true floor or one more. */
+     * with num = 2^53 - 1 - F - Q >= 0 (exact). */
     const double num = (9007199254740991.0 - F) - qd;
-    double m = __builtin_floor(num / D);
-    if (m * D > num) m -= 1.0;
-    m += 1.0;
+    const double m = gm_floor_div(num, D) + 1.0;
     const double step = D * u;
     if (m >= budget) {
         f = f + budget * step;

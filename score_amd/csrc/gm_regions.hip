@@ -558,11 +558,10 @@ __global__ void __launch_bounds__(GM_BLOCK) k_rg_propose(GmDev d) {
             const uint32_t rxg = r->rxg;
             const uint32_t cell = c.cell;
             const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
-            GmPar *o = d.par + p;
             q2 = make_uint4(x, y, z, d.plist[p]);
-            *reinterpret_cast<uint4 *>(&o->slot) = make_uint4(s, cell, c.defg, rxg);
-            *reinterpret_cast<double2 *>(&o->fin) = make_double2(fin, c.P);
-            *reinterpret_cast<uint4 *>(&o->x) = q2;
+            par_plane(d, GM_PAR_IDS)[p] = make_uint4(s, cell, c.defg, rxg);
+            reinterpret_cast<double2 *>(par_plane(d, GM_PAR_FP))[p] = make_double2(fin, c.P);
+            par_plane(d, GM_PAR_XYZ)[p] = q2;
             if (logged) d.rlog[nret + p] = make_uint2(cell, d.ndefg + rxg);
         }
         q2.x = (uint32_t)__shfl((int)q2.x, 0, GM_PG);
@@ -602,7 +601,7 @@ __global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
 #pragma unroll
     for (int c = 0; c < GM_RROWS; c++) cnt[c] = 0;
     for (uint32_t p = lo + threadIdx.x; p < hi; p += 64) {
-        const uint4 q2 = *reinterpret_cast<const uint4 *>(&d.par[p].x);
+        const uint4 q2 = par_plane(d, GM_PAR_XYZ)[p];
         const uint32_t am = d.pmI[p];
         uint32_t mI = 0, mB = 0, mH = 0;
         for (uint32_t m = am; m; m &= m - 1u) {
@@ -757,10 +756,9 @@ __global__ void __launch_bounds__(64) k_rg_commit(GmDev d, GmPParams pp, GmSweep
         uint4 q0 = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu), q2 = make_uint4(0u, 0u, 0u, 0u);
         double2 fp = make_double2(0.0, 0.0);
         if (valid) {
-            const GmPar *pr = d.par + pt;
-            q0 = *reinterpret_cast<const uint4 *>(&pr->slot);
-            fp = *reinterpret_cast<const double2 *>(&pr->fin);
-            q2 = *reinterpret_cast<const uint4 *>(&pr->x);
+            q0 = par_plane(d, GM_PAR_IDS)[pt];
+            fp = reinterpret_cast<const double2 *>(par_plane(d, GM_PAR_FP))[pt];
+            q2 = par_plane(d, GM_PAR_XYZ)[pt];
         }
         uint32_t mh[GM_NHALO];
         halo_masks(d, box, q2, mH, mh);
