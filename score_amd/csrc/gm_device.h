@@ -247,6 +247,18 @@ static __device__ __forceinline__ gm_cs_t cs_code(const GmDev &d, uint32_t cs, u
     return (gm_cs_t)(cs & (GM_CS_MOD - 1u));
 }
 
+/* Whether code cs marks an INACTIVE slot at step code kc (a step mod GM_CS_MOD):
+ * the INACTIVE sentinel, or the code of a step before kc.  A cell that completes
+ * keeps the code of its completing step, and from the next step on that code
+ * lies in the past, so a retiring cell's slot is not written.  A live cell's
+ * code names a step in [kc, kc + horizon).  A past code is rewritten to
+ * INACTIVE by the next re-anchoring, which comes at the latest at the next
+ * multiple of the horizon, before the code could wrap into that range
+ * (horizon <= GM_CS_MOD / 2). */
+static __device__ __forceinline__ bool cs_inactive(uint32_t cs, uint32_t kc, uint32_t horizon) {
+    return cs == GM_CS_INACTIVE || (cs < GM_CS_MOD && ((cs - kc) & (GM_CS_MOD - 1u)) >= horizon);
+}
+
 /* Predicts slot s from its value f before step A.k (its new anchor): rewrites
  * the record's first sector (anchor, increment, final value) and stores the
  * completion step's code. */
@@ -421,8 +433,9 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
 
 
 /* Retires a batch of parents (updateFullRX*, SCORE_Kernel.cpp:4640-4653): each
- * lane's parent, q0 = (slot, cell, defg, rxg), becomes INACTIVE and moves a cell
- * from its deformed grain's count to its nucleus'.  A deformed grain losing its
+ * lane's parent, q0 = (slot, cell, defg, rxg), moves a cell from its deformed
+ * grain's count to its nucleus' (its slot reads as INACTIVE from the next step
+ * on by its completion code, cs_inactive).  A deformed grain losing its
  * last cell is reported so the host can update rhomax (update_myrhomax,
  * SCORE_RecoveryModels.cpp:146-165).  Consecutive parents often share their
  * nucleus and deformed grain (siblings), so each run of equal ids among
@@ -430,7 +443,6 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
 static __device__ __forceinline__ void retire_batch(const GmDev &d, const uint4 &q0, bool valid) {
     GmStats *st = d.st;
     const uint32_t lane = threadIdx.x & 63u;
-    if (valid) d.cstep[q0.x] = (gm_cs_t)GM_CS_INACTIVE;
     const uint32_t g = q0.z, r = q0.w;
     const uint32_t gprev = __shfl_up(g, 1, 64), rprev = __shfl_up(r, 1, 64);
     const uint64_t gstart = __ballot(valid && (lane == 0 || g != gprev));
@@ -543,10 +555,10 @@ static __device__ __forceinline__ void wave_sync() {
 
 /* Classifies a lane's eight slots of a sweep chunk (2-byte codes in cs, the
  * first nocc of them occupied): bit q of the low byte is set for a slot that
- * completes at step code kc, bit 8+q for an INACTIVE slot.  unknown counts
- * slots without a valid prediction: a cell is predicted before the sweep (when
- * it is infected, or by k_reanchor) or is DEFERRED, NEVER or FAR, so a code a
- * horizon or more ahead (a step in the past) or UNKNOWN is an internal error. */
+ * completes at step code kc, bit 8+q for an INACTIVE slot (cs_inactive: the
+ * sentinel or a past step's code).  unknown counts slots without a valid
+ * prediction: a cell is predicted before the sweep (when it is infected, or by
+ * k_reanchor) or is DEFERRED, NEVER or FAR, so UNKNOWN is an internal error. */
 static __device__ __forceinline__ uint32_t sweep_classify(const uint4 &cs, uint32_t nocc, uint32_t kc,
                                                           uint32_t horizon, uint32_t &unknown) {
     const uint32_t wd[4] = {cs.x, cs.y, cs.z, cs.w};
@@ -555,8 +567,8 @@ static __device__ __forceinline__ uint32_t sweep_classify(const uint4 &cs, uint3
     for (int q = 0; q < 8; q++) {
         const uint32_t x = (wd[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
         const uint32_t in = (uint32_t)q < nocc ? 1u : 0u;
-        pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (x == GM_CS_INACTIVE ? 1u : 0u)) << (8 + q));
-        unknown += in & ((x == GM_CS_UNKNOWN || (x < GM_CS_MOD && ((x - kc) & (GM_CS_MOD - 1u)) >= horizon)) ? 1u : 0u);
+        pack |= ((in & (x == kc ? 1u : 0u)) << q) | ((in & (cs_inactive(x, kc, horizon) ? 1u : 0u)) << (8 + q));
+        unknown += in & (x == GM_CS_UNKNOWN ? 1u : 0u);
     }
     return pack;
 }

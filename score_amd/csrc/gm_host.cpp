@@ -972,12 +972,12 @@ int defragment_list(score_gm_handle *h, uint32_t base, uint32_t cap, uint32_t &n
     hipStream_t s = h->stream;
     /* the first INACTIVE slot; the reference's search runs on past nnever while
      * it meets ACTIVE records left behind by an earlier defragmentation */
-    HIPCHK(gm_launch_defrag_first_inactive(d, base, 0, nnever, s));
+    HIPCHK(gm_launch_defrag_first_inactive(d, base, 0, nnever, h->step, s));
     HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
     HIPCHK(hipStreamSynchronize(s));
     uint32_t first = h->hst->dfirst;
     if (first >= nnever && nnever < cap) {
-        HIPCHK(gm_launch_defrag_first_inactive(d, base, nnever, cap, s));
+        HIPCHK(gm_launch_defrag_first_inactive(d, base, nnever, cap, h->step, s));
         HIPCHK(gm_launch_stats_out(d, h->hst_dev, s));
         HIPCHK(hipStreamSynchronize(s));
         first = h->hst->dfirst;
@@ -1762,7 +1762,7 @@ int score_gm_get_grainmap(score_gm_handle *h, uint32_t *out, int32_t encoding) {
     if (rc) return rc;
     auto copy = [&]() -> int {
         HIPCHK(hipMemcpyAsync(tmp, h->d.grid, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(gm_launch_overlay_infected(h->d, h->nI, h->nB, tmp, h->stream));
+        HIPCHK(gm_launch_overlay_infected(h->d, h->nI, h->nB, h->step, tmp, h->stream));
         HIPCHK(hipMemcpyAsync(out, tmp, sizeof(uint32_t) * h->d.N, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;

@@ -47,11 +47,16 @@
 /* Completion steps are stored in 16 bits.  A step cs is kept as cs % GM_CS_MOD
  * when it lies less than the horizon ahead of the step that predicted it, else
  * the cell is FAR; every cell is predicted again at each multiple of the
- * horizon.  So at step k every stored step lies in [k, k + horizon), and
- * cs % GM_CS_MOD names it.  Codes at or above GM_CS_MOD are sentinels. */
+ * horizon.  So at step k every live cell's stored step lies in [k, k + horizon),
+ * and cs % GM_CS_MOD names it.  A cell that completed keeps its code, which from
+ * the next step on names a past step and reads as INACTIVE until a
+ * re-anchoring (at the latest at the next multiple of the horizon) or a
+ * defragmentation writes the INACTIVE sentinel.  Codes at or above GM_CS_MOD
+ * are sentinels. */
 typedef uint16_t gm_cs_t;
 #define GM_CS_MOD 0x8000u
 #define GM_CS_HORIZON_MAX 0x4000u  /* horizon in steps, at most GM_CS_MOD / 2 */
+static_assert(GM_CS_HORIZON_MAX <= GM_CS_MOD / 2, "a past code must not wrap into the horizon before it is cleared");
 #define GM_CS_INACTIVE 0xFFFFu
 #define GM_CS_NEVER 0xFFFEu       /* active, does not reach rxFrac >= 1 within the run */
 #define GM_CS_UNKNOWN 0xFFFDu     /* placed by nucleation, increment not yet known (the first step re-anchors) */
@@ -377,7 +382,8 @@ hipError_t gm_launch_dmap_init(const GmDev &d, hipStream_t s);
  * move compacts the ACTIVE slots of (first, nnever) to [first, first+count) through
  * the scratch records trec/tcs (nnever-first-1 entries), leaves count in st->dcount
  * and freezes the ACTIVE records left behind at their value before step k. */
-hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s);
+hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, uint32_t k,
+                                           hipStream_t s);
 hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, uint32_t nnever, uint32_t k,
                                  GmRec *trec, gm_cs_t *tcs, hipStream_t s);
 /* ompcrit_clarify_status_infectedcells: (cell, grain shown) of every ACTIVE record,
@@ -388,7 +394,8 @@ hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k
 /* the cell grid as SCORE holds it, from a copy of d.grid in `out`: every retired
  * cell set to ndefg+rxg from the log, every cell held by an active front record
  * marked INFECTED */
-hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s);
+hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k, uint32_t *out,
+                                      hipStream_t s);
 hipError_t gm_launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
 hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys, const double *vals, uint32_t n,

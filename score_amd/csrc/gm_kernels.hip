@@ -62,15 +62,21 @@ hipError_t gm_launch_fill_u64(unsigned long long *p, unsigned long long v, size_
 
 /* The per-step scalars changed: every active cell is advanced to the start of
  * step k with its old increment (its anchor moves to k) and predicted again
- * with the new one.  One thread per occupied slot. */
+ * with the new one.  One thread per occupied slot.  Codes of cells that
+ * completed before step k are turned into INACTIVE. */
 __global__ void __launch_bounds__(GM_BLOCK) k_reanchor(GmDev d, GmSweepArgs A0) {
     if (d.st->halt) return;
     const GmSweepArgs A = step_args(d, A0);
     const uint32_t nI = d.st->nI, n = nI + d.st->nB;
+    const uint32_t kc = A.k & (GM_CS_MOD - 1u);
     for (uint32_t i = blockIdx.x * GM_BLOCK + threadIdx.x; i < n; i += gridDim.x * GM_BLOCK) {
         const uint32_t s = occ_slot(d, nI, i);
         const uint32_t cs = d.cstep[s];
-        if (cs == GM_CS_INACTIVE) continue;
+        if (cs_inactive(cs, kc, d.horizon)) {
+            /* a cell that completed in an earlier step: its code becomes the sentinel */
+            if (cs != GM_CS_INACTIVE) d.cstep[s] = (gm_cs_t)GM_CS_INACTIVE;
+            continue;
+        }
         GmRec *r = d.rec + s;
         const RecA a = rec_a(r);
         const RecC c = rec_c(r);
@@ -192,7 +198,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_sweep(GmDev d, GmSweepArgs A0) {
 }
 
 /* pass 2: per-block offsets and the list lengths.  Cells the sweep met without
- * a valid prediction raise GM_ERRF_MISSED. */
+ * a prediction (UNKNOWN) raise GM_ERRF_MISSED. */
 __global__ void __launch_bounds__(1024) k_sweep_scan(GmDev d, uint32_t nb) {
     if (d.st->halt) return;
     uint32_t tot[5];
@@ -602,19 +608,20 @@ __global__ void k_apply_rlog(GmDev d, uint32_t *out) {
     }
 }
 
-/* exported map: cells held by active front records are INFECTED */
-__global__ void k_overlay_infected(GmDev d, uint32_t nI, uint32_t nB, uint32_t *out) {
-    const uint32_t n = nI + nB;
+/* exported map: cells held by active front records before step k are INFECTED */
+__global__ void k_overlay_infected(GmDev d, uint32_t nI, uint32_t nB, uint32_t k, uint32_t *out) {
+    const uint32_t n = nI + nB, kc = k & (GM_CS_MOD - 1u);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t s = occ_slot(d, nI, i);
-        if (d.cstep[s] != GM_CS_INACTIVE) out[d.rec[s].cell] = GM_INFECTED;
+        if (!cs_inactive(d.cstep[s], kc, d.horizon)) out[d.rec[s].cell] = GM_INFECTED;
     }
 }
 
-hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t *out, hipStream_t s) {
+hipError_t gm_launch_overlay_infected(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k, uint32_t *out,
+                                      hipStream_t s) {
     k_apply_rlog<<<8192, GM_BLOCK, 0, s>>>(d, out);
     if (nI + nB == 0) return hipGetLastError();
-    k_overlay_infected<<<fill_blocks(nI + nB), GM_BLOCK, 0, s>>>(d, nI, nB, out);
+    k_overlay_infected<<<fill_blocks(nI + nB), GM_BLOCK, 0, s>>>(d, nI, nB, k, out);
     return hipGetLastError();
 }
 
@@ -850,7 +857,7 @@ __global__ void k_materialize(GmDev d, uint32_t k, uint32_t base, uint32_t n, Gm
         o.cell[i] = c.cell;
         o.defg[i] = c.defg;
         o.rxg[i] = r->rxg;
-        if (cs == GM_CS_INACTIVE) {
+        if (cs_inactive(cs, k & (GM_CS_MOD - 1u), d.horizon)) {
             o.act[i] = 0;
             o.frac[i] = 0.0;
             continue;
@@ -906,11 +913,12 @@ hipError_t gm_launch_hash_insert(const GmDev &d, const unsigned long long *keys,
  * ACTIVE record left there is frozen here at its value before step k (anchor k,
  * increment 0), which is what the reference holds in that slot from then on. */
 
-__global__ void __launch_bounds__(GM_BLOCK) k_defrag_first(GmDev d, uint32_t base, uint32_t lo, uint32_t hi) {
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_first(GmDev d, uint32_t base, uint32_t lo, uint32_t hi,
+                                                          uint32_t kc) {
     const uint32_t lane = threadIdx.x & 63u, stride = gridDim.x * GM_BLOCK;
     for (uint32_t i0 = lo + blockIdx.x * GM_BLOCK + (threadIdx.x & ~63u); i0 < hi; i0 += stride) {
         const uint32_t i = i0 + lane;
-        const uint64_t m = __ballot(i < hi && d.cstep[base + i] == GM_CS_INACTIVE);
+        const uint64_t m = __ballot(i < hi && cs_inactive(d.cstep[base + i], kc, d.horizon));
         if (m) {
             /* later chunks of this wave only hold larger slots */
             if (lane == 0) atomicMin(&d.st->dfirst, i0 + (uint32_t)__ffsll((long long)m) - 1u);
@@ -920,19 +928,21 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_first(GmDev d, uint32_t bas
     }
 }
 
-hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, hipStream_t s) {
+hipError_t gm_launch_defrag_first_inactive(const GmDev &d, uint32_t base, uint32_t lo, uint32_t hi, uint32_t k,
+                                           hipStream_t s) {
     k_fill_u32<<<1, 64, 0, s>>>(&d.st->dfirst, hi, 1);
-    if (hi > lo) k_defrag_first<<<fill_blocks(hi - lo), GM_BLOCK, 0, s>>>(d, base, lo, hi);
+    if (hi > lo) k_defrag_first<<<fill_blocks(hi - lo), GM_BLOCK, 0, s>>>(d, base, lo, hi, k & (GM_CS_MOD - 1u));
     return hipGetLastError();
 }
 
 /* ACTIVE slots among list slots first+1+i, i in [0, n), per block chunk */
-__global__ void __launch_bounds__(GM_BLOCK) k_defrag_count(GmDev d, uint32_t base, uint32_t first, uint32_t n) {
+__global__ void __launch_bounds__(GM_BLOCK) k_defrag_count(GmDev d, uint32_t base, uint32_t first, uint32_t n,
+                                                          uint32_t kc) {
     uint32_t lo, hi;
     chunk_of(n, lo, hi);
     uint32_t c = 0;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += GM_BLOCK)
-        c += d.cstep[base + first + 1u + i] != GM_CS_INACTIVE ? 1u : 0u;
+        c += cs_inactive(d.cstep[base + first + 1u + i], kc, d.horizon) ? 0u : 1u;
     uint32_t tot;
     block_incl_scan(c, &tot);
     if (threadIdx.x == 0) d.bcnt[blockIdx.x] = tot;
@@ -957,7 +967,10 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t ba
         const uint32_t i = b0 + threadIdx.x;
         const uint32_t c = first + 1u + i;
         const uint32_t cs = i < hi ? d.cstep[base + c] : GM_CS_INACTIVE;
-        const uint32_t act = cs != GM_CS_INACTIVE ? 1u : 0u;
+        const uint32_t act = cs_inactive(cs, k & (GM_CS_MOD - 1u), d.horizon) ? 0u : 1u;
+        /* the code of a cell that completed in an earlier step becomes the
+         * sentinel, so that no past code stays behind the compacted block */
+        if (i < hi && !act && cs != GM_CS_INACTIVE) d.cstep[base + c] = (gm_cs_t)GM_CS_INACTIVE;
         uint32_t tot;
         const uint32_t incl = block_incl_scan(act, &tot);
         if (act) {
@@ -1006,7 +1019,7 @@ hipError_t gm_launch_defrag_move(const GmDev &d, uint32_t base, uint32_t first, 
     if (first + 1u >= nnever) return hipSuccess;
     const uint32_t n = nnever - first - 1u;
     const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
-    k_defrag_count<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n);
+    k_defrag_count<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n, k & (GM_CS_MOD - 1u));
     k_scan_row<<<1, 1024, 0, s>>>(d, nb);
     k_defrag_gather<<<nb, GM_BLOCK, 0, s>>>(d, base, first, n, k, trec, tcs);
     k_defrag_scatter<<<fill_blocks(n), GM_BLOCK, 0, s>>>(d, base, first, trec, tcs);
@@ -1024,12 +1037,12 @@ static __device__ __forceinline__ uint32_t item_slot(const GmDev &d, uint32_t nI
     return occ_slot(d, nI, i);
 }
 
-__global__ void __launch_bounds__(GM_BLOCK) k_status_count(GmDev d, uint32_t nI, uint32_t n) {
+__global__ void __launch_bounds__(GM_BLOCK) k_status_count(GmDev d, uint32_t nI, uint32_t n, uint32_t kc) {
     uint32_t lo, hi;
     chunk_of(n, lo, hi);
     uint32_t c = 0;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += GM_BLOCK)
-        c += d.cstep[item_slot(d, nI, i)] != GM_CS_INACTIVE ? 1u : 0u;
+        c += cs_inactive(d.cstep[item_slot(d, nI, i)], kc, d.horizon) ? 0u : 1u;
     uint32_t tot;
     block_incl_scan(c, &tot);
     if (threadIdx.x == 0) d.bcnt[blockIdx.x] = tot;
@@ -1044,7 +1057,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_status_write(GmDev d, uint32_t nI,
         const uint32_t i = b0 + threadIdx.x;
         const uint32_t s = i < hi ? item_slot(d, nI, i) : 0u;
         const uint32_t cs = i < hi ? d.cstep[s] : GM_CS_INACTIVE;
-        const uint32_t act = cs != GM_CS_INACTIVE ? 1u : 0u;
+        const uint32_t act = cs_inactive(cs, k & (GM_CS_MOD - 1u), d.horizon) ? 0u : 1u;
         uint32_t tot;
         const uint32_t incl = block_incl_scan(act, &tot);
         if (act) {
@@ -1069,7 +1082,7 @@ hipError_t gm_launch_status(const GmDev &d, uint32_t nI, uint32_t nB, uint32_t k
         return hipGetLastError();
     }
     const uint32_t nb = grid_for(n, 4 * GM_BLOCK);
-    k_status_count<<<nb, GM_BLOCK, 0, s>>>(d, nI, n);
+    k_status_count<<<nb, GM_BLOCK, 0, s>>>(d, nI, n, k & (GM_CS_MOD - 1u));
     k_scan_row<<<1, 1024, 0, s>>>(d, nb);
     k_status_write<<<nb, GM_BLOCK, 0, s>>>(d, nI, n, k, thr, ixyz, gid);
     return hipGetLastError();
