@@ -455,7 +455,7 @@ def main(argv=None) -> int:
             prm = dataclasses.replace(prm, n_regions=a.profile_regions)
         psecs, pres, pf, h = one_run(True)
         log(f"profiled run: {psecs:.3f}s, {pf['steps']} steps, grain map {h}; per-kernel ms "
-            f"{json.dumps({k: round(v, 3) for k, v in pf['ms_kernel'].items()})}")
+            f"{json.dumps({k: round(v, 3) for k, v in pf['ms_kernel'].items() if v > 0.0})}")
         return 0
 
     runs = []
@@ -496,22 +496,13 @@ def main(argv=None) -> int:
             tr = json.load(open(TRAFFIC_FILE))
             if tr.get("kernel") == dom_k:
                 traffic = tr.get("bytes_per_launch")
-        # effective bandwidth of the reference algorithm's bytes per phase
-        sweep_ref = 30 * pf["sum_sv"] + (pf["slots_scanned"] - pf["sum_sv"]) + 4 * (pf["n_full"] + pf["n_recycled"])
-        infect_ref = (pf["n_full"] * (26 * 4 + 30) + 4 * pf["n_attempts"] + pf["n_new"] * 34 + pf["n_full"] * 12)
-        sweep_ms = sum(kms[k] for k in ("k_reanchor", "k_sweep", "k_sweep_scan", "k_compact"))
-        infect_ms = sum(kms[k] for k in ("k_propose", "k_decide", "k_decide_scan", "k_commit"))
         roofline = {
             "bound": "hbm", "kernel": dom_k,
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "avg_launch_us": round(avg_us, 3), "alg_bytes_per_launch": int(nbytes / launches), "alg_bytes_units": units,
-            "kernel_ms_per_run": {k: round(v, 3) for k, v in kms.items()},
+            "kernel_ms_per_run": {k: round(v, 3) for k, v in kms.items() if v > 0.0},
             "device_ms_per_run": round(pf["ms_step"], 3), "host_loop_ms_per_run": round(pf["ms_host"], 3),
-            "reference_bytes_effective_GBs": {
-                "sweep": round(sweep_ref / (sweep_ms * 1e-3) / 1e9, 1),
-                "infection": round(infect_ref / (infect_ms * 1e-3) / 1e9, 1),
-                "step": round((sweep_ref + infect_ref) / (pf["ms_step"] * 1e-3) / 1e9, 1)},
             "counters_per_run": {k: pf[k] for k in ("steps", "sum_sv", "slots_scanned", "n_full", "n_recycled",
                                                     "n_attempts", "n_new", "n_predicted", "n_reanchor_steps",
                                                     "n_reanchored", "pending_resolved")},
