@@ -88,6 +88,38 @@ def copy_calibration() -> int:
     return 0
 
 
+def stream_reference(nbytes: int) -> dict:
+    """What one launch that streams nbytes reaches on this GPU: torch's float32 sum
+    over a buffer of that size (reads it once), timed with events on torch's
+    stream, as the yardstick of a pass that reads every occupied slot's code.
+    It runs in a child process: torch brings its own HIP runtime, which is not
+    mixed with the growth library's in one process."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-reference-bytes", str(int(nbytes))],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"streaming yardstick failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def stream_reference_run(nbytes: int, reps: int = 50) -> dict:
+    import torch
+    a = torch.ones(max(nbytes // 4, 1), dtype=torch.float32, device="cuda")
+    for _ in range(5):
+        torch.sum(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        torch.sum(a)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    del a
+    return {"what": "torch.sum over a float32 buffer of the same bytes, one call per launch",
+            "bytes": int(nbytes), "us": round(us, 2), "GBs": round(nbytes / (us * 1e-6) / 1e9, 1)}
+
+
 def ensemble_domains(seed: int, n: int):
     """The batched-ensemble workload: n domains, edges U{64..192}, edge^3/2000
     grains, nucleus density log-uniform in [1e-5, 1e-3] per voxel, T uniform in
@@ -373,6 +405,8 @@ def main(argv=None) -> int:
     ap.add_argument("--profile-run-only", action="store_true",
                     help="only the profiled run (one step per launch window, events between kernels); for rocprofv3")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--stream-reference-bytes", type=int, default=0,
+                    help="only the streaming yardstick (torch sum over this many bytes), as one JSON line")
     ap.add_argument("--profile-regions", type=int, default=0,
                     help="with --profile-run-only: profile the run with this many memory regions")
     ap.add_argument("--batch", type=int, default=0,
@@ -398,6 +432,9 @@ def main(argv=None) -> int:
         return 2
     if a.copy_calibration:
         return copy_calibration()
+    if a.stream_reference_bytes:
+        print(json.dumps(stream_reference_run(a.stream_reference_bytes)), flush=True)
+        return 0
     if a.batch:
         return batch_bench(a)
     if a.serve:
@@ -456,6 +493,8 @@ def main(argv=None) -> int:
         psecs, pres, pf, h = one_run(True)
         log(f"profiled run: {psecs:.3f}s, {pf['steps']} steps, grain map {h}; per-kernel ms "
             f"{json.dumps({k: round(v, 3) for k, v in pf['ms_kernel'].items() if v > 0.0})}")
+        ref = stream_reference(int(kernel_bytes("k_sweep", pf)[0] / pf["steps"]))
+        log(f"streaming yardstick over the sweep's bytes: {json.dumps(ref)}")
         return 0
 
     runs = []
@@ -507,6 +546,12 @@ def main(argv=None) -> int:
                                                     "n_attempts", "n_new", "n_predicted", "n_reanchor_steps",
                                                     "n_reanchored", "pending_resolved")},
         }
+        # the sweep beside a plain streaming pass over the same bytes
+        sw_bytes = kernel_bytes("k_sweep", pf)[0] / steps
+        sw_us = kms["k_sweep"] * 1e3 / steps
+        ref = stream_reference(int(sw_bytes))
+        roofline["sweep_vs_stream"] = {"sweep_bytes_per_launch": int(sw_bytes), "sweep_us": round(sw_us, 2),
+                                       "sweep_GBs": round(sw_bytes / (sw_us * 1e-6) / 1e9, 1), "stream": ref}
         log(f"profiled run: {psecs:.3f}s; dominant kernel {dom_k} {avg_us:.1f} us/launch, "
             f"{achieved:.0f} GB/s algorithmic; per-kernel ms {json.dumps(roofline['kernel_ms_per_run'])}")
 

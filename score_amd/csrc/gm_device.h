@@ -424,9 +424,20 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
     rec_set_c(rec, Pn, t, g);
     *reinterpret_cast<uint4 *>(&rec->rxg) = make_uint4(pq0.w, 0u, 0u, 0u);
     d.cstep[gslot] = cs;
-    if (cs == GM_CS_DEFERRED) {
-        const uint32_t i = atomicAdd(&d.st->ndeferred, 1u);
-        if (i < d.dlistCap) d.dlist[i] = gslot;
+    /* DEFERRED cells go to the deferred list with one atomic per wave: the
+     * list's counter is a single address, and atomics on one address are
+     * served one after another */
+    const bool deferred = cs == GM_CS_DEFERRED;
+    const uint64_t dm = __ballot(deferred);
+    if (dm) {
+        const uint32_t lead = (uint32_t)__ffsll((long long)dm) - 1u;
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == lead) base = atomicAdd(&d.st->ndeferred, (uint32_t)__popcll(dm));
+        base = rdlane(base, lead);
+        if (deferred) {
+            const uint32_t i = base + lanemask_lt_count(dm);
+            if (i < d.dlistCap) d.dlist[i] = gslot;
+        }
     }
     clear_deformed(d, tx, ty, tz);
 }

@@ -297,9 +297,12 @@ __global__ void __launch_bounds__(64) k_recycle(GmDev d) {
 }
 
 /* grid of the sweep, compaction and recycling passes for lists of about nI, nB */
+#ifndef GM_SW_CHUNKS
+#define GM_SW_CHUNKS 2 /* 512-slot chunks per sweep wave */
+#endif
 static uint32_t sweep_blocks(uint32_t nI, uint32_t nB) {
     const uint32_t T = gm_tiles_inside(nI) + (nB + 63) / 64;
-    return grid_for(T, GM_SW_TPC * (GM_BLOCK / 64));
+    return grid_for(T, GM_SW_TPC * (GM_BLOCK / 64) * GM_SW_CHUNKS);
 }
 
 hipError_t gm_launch_reanchor(const GmDev &d, const GmSweepArgs &a, hipStream_t s) {
