@@ -134,9 +134,23 @@ GM_HD double gm_radd_round(double &f, double inc, double budget) {
     return m + 1.0;
 }
 
+/* Additions done one by one before the closed form takes over: a round costs
+ * about as many instructions as a dozen plain additions.  Short replays are
+ * added up directly, and so are the first additions of a value below 1/4,
+ * whose binades a round crosses a few additions at a time (a new cell starts
+ * near 0 with an increment of up to 0.1). */
+#define GM_RADD_PLAIN 16u
+#define GM_RADD_LOW 0.25
+
 /* f after n additions of inc */
 GM_HD double gm_radd_n(double f, double inc, uint64_t n) {
-    double left = (double)n;
+    if (n <= GM_RADD_PLAIN) {
+        for (uint32_t i = 0; i < (uint32_t)n; i++) f = f + inc;
+        return f;
+    }
+    uint32_t i = 0;
+    for (; i < GM_RADD_PLAIN && f < GM_RADD_LOW; i++) f = f + inc;
+    double left = (double)(n - i);
     while (left > 0.0) {
         const double k = gm_radd_round(f, inc, left);
         if (k == 0.0) break;
@@ -159,13 +173,28 @@ GM_HD uint64_t gm_radd_until_one(double &f, double inc, uint64_t cap) {
         return 0; /* inc < 0: the value only decreases from here */
     }
     if (!(inc > 0.0)) return 0;
-    const double capd = (double)cap;
+    double g = f;
+    const uint64_t plim = cap < GM_RADD_PLAIN ? cap : GM_RADD_PLAIN;
+    uint64_t lim = 0;
+    while (lim < plim && g < GM_RADD_LOW) {
+        g = g + inc;
+        lim++;
+        if (g >= 1.0) {
+            f = g;
+            return lim;
+        }
+    }
+    if (lim == cap) return 0;
+    const double capd = (double)(cap - lim);
     double done = 0.0;
     while (done < capd) {
-        const double k = gm_radd_round(f, inc, capd - done);
+        const double k = gm_radd_round(g, inc, capd - done);
         if (k == 0.0) return 0;
         done += k;
-        if (f >= 1.0) return (uint64_t)done;
+        if (g >= 1.0) {
+            f = g;
+            return (uint64_t)done + lim;
+        }
     }
     return 0;
 }
