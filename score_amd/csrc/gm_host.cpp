@@ -1430,15 +1430,6 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(SCORE_GM_ERR_NODEVICE, "device is not gfx950");
     HIPCHK(hipSetDevice(device));
-    {
-        /* freed handle memory stays in the device's pool for later handles */
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-            uint64_t keep = UINT64_MAX;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        }
-        (void)hipGetLastError();
-    }
 
     score_gm_handle *h = new (std::nothrow) score_gm_handle();
     if (!h) return fail(SCORE_GM_ERR_OOM, "handle");

@@ -596,6 +596,50 @@ def test_mixed_settings_in_launch_windows(require_gpu, seed):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("seed,chunks", [(3, (17, 31, 9)), (8, (40,)), (11, (12, 25))])
+def test_chunks_of_several_windows(require_gpu, seed, chunks):
+    """Steps taken in chunks longer than a launch window, so that one call runs
+    several windows back to back and a chunk ends inside a window; fronts and
+    trajectory agree with the oracle at every chunk's end."""
+    dom = synth.make_voronoi_domain(48, 40, 32, n_grains=9, n_nuclei=25, seed=seed)
+    print(f"chunks of several windows seed {seed} chunks {chunks}", flush=True)
+    o, g = Oracle(dom, ParamsSpec()), _gm(dom, ParamsSpec())
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+
+
+@pytest.mark.parametrize("seed", range(0, 40, 3))
+def test_mixed_settings_in_long_chunks(require_gpu, seed):
+    """The seeded mixed settings stepped in chunks of 10-60 steps: several
+    windows per call across drag, defragmentation, grain-log points, arrivals
+    and the deferred weights of new pairs."""
+    dom, prm, _ = _mixed_case(seed)
+    rng = np.random.default_rng(7000 + seed)
+    chunks = tuple(int(c) for c in rng.integers(10, 61, 3))
+    print(f"mixed case {seed} in chunks {chunks}", flush=True)
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+
+
 def test_hand_built_pin_cases(require_gpu):
     """The two hand-built domains that pin the oracle bit for bit against the
     reference's expressions (an Inside and a Border nucleus sharing nine target

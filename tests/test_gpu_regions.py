@@ -93,6 +93,29 @@ def test_regions_in_launch_windows(require_gpu, T, chunks):
     g.close()
 
 
+@pytest.mark.parametrize("T,chunks", [(2, (20, 13)), (4, (33,)), (8, (11, 27))])
+def test_regions_chunks_of_several_windows(require_gpu, T, chunks):
+    """chunks longer than a launch window: several windows per call, and chunks
+    that end inside a window"""
+    dom = region_domain(T, seed=60 + T, n_grains=10, n_nuclei=45)
+    prm = ParamsSpec(n_regions=T)
+    print(f"regions chunks of several windows T={T} {dom.nx}x{dom.ny}x{dom.nz} chunks={chunks}", flush=True)
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        assert_lists_equal(o, g, T, f"chunk{k}")
+        if do:
+            break
+        k += 1
+    assert_final_equal_regions(o, g, T)
+    o.close()
+    g.close()
+
+
 @pytest.mark.parametrize("T", [2, 3])
 @pytest.mark.parametrize("first", ["below", "above"])
 def test_own_region_front_beats_the_halo(require_gpu, first, T):
