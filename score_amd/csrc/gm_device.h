@@ -150,7 +150,7 @@ static __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 
 /* ------------------------------------------------------------ front records */
 
-/* the record's 16-byte quarters: A = (frac0, s0, inf), B = (inc, fin),
+/* the record's 16-byte quarters: A = (frac0, s0, inf), B = (inc, spare),
  * C = (P, cell, defg), then rxg */
 struct RecA {
     double frac0;
@@ -172,8 +172,8 @@ static __device__ __forceinline__ void rec_set_a(GmRec *r, double frac0, uint32_
     const unsigned long long b = (unsigned long long)__double_as_longlong(frac0);
     *reinterpret_cast<uint4 *>(&r->frac0) = make_uint4((uint32_t)b, (uint32_t)(b >> 32), s0, inf);
 }
-static __device__ __forceinline__ void rec_set_b(GmRec *r, double inc, double fin) {
-    *reinterpret_cast<double2 *>(&r->inc) = make_double2(inc, fin);
+static __device__ __forceinline__ void rec_set_b(GmRec *r, double inc) {
+    *reinterpret_cast<double2 *>(&r->inc) = make_double2(inc, 0.0);
 }
 static __device__ __forceinline__ void rec_set_c(GmRec *r, double P, uint32_t cell, uint32_t defg) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(P);
@@ -208,35 +208,35 @@ static __device__ __forceinline__ bool cell_increment(const GmDev &d, const GmSw
 /* Completion step of a cell with value f before step k: the first step >= k
  * in which repeated rounded additions of its increment reach 1.0 (GM_STEP_NONE
  * when none up to the step cap does).  Sets the increment (0.0 for a cell SCORE
- * does not advance) and the value after the completing addition, whose excess
- * over 1.0 the infection carries on. */
+ * does not advance).  The value after the completing addition, whose excess
+ * over 1.0 the infection carries on, is replayed from the anchor when the cell
+ * completes and wins a target (parent_fin). */
 static __device__ __forceinline__ uint32_t predict_values(const GmDev &d, const GmSweepArgs &A, uint32_t k, double f,
-                                                          double P, uint32_t defg, uint32_t inf, double &inc,
-                                                          double &fin) {
+                                                          double P, uint32_t defg, uint32_t inf, double &inc) {
     uint32_t cs = GM_STEP_NONE;
-    fin = 0.0;
     if (cell_increment(d, A.sp, P, defg, inf, inc)) {
         if (k <= A.stepcap && A.next_only) {
             /* the step's one addition; a cell it leaves below 1.0 is re-anchored
              * and predicted again before the next step (GM_STEP_FAR_NEXT) */
-            const double g = f + inc;
-            if (g >= 1.0) {
-                cs = k;
-                fin = g;
-            } else {
-                cs = GM_STEP_NEXT;
-            }
+            cs = f + inc >= 1.0 ? k : GM_STEP_NEXT;
         } else if (k <= A.stepcap) {
-            const uint64_t j = gm_radd_until_one(f, inc, (uint64_t)(A.stepcap - k) + 1u);
-            if (j) {
-                cs = k + (uint32_t)(j - 1u);
-                fin = f;
-            }
+            const uint64_t j = gm_radd_count_to_one(f, inc, (uint64_t)(A.stepcap - k) + 1u);
+            if (j) cs = k + (uint32_t)(j - 1u);
         }
     } else {
         inc = 0.0;
     }
     return cs;
+}
+
+/* rxFrac of a parent after its completing addition in step k: n = k - s0 + 1
+ * additions of its increment to the value it was anchored with before step s0 */
+static __device__ __forceinline__ double parent_fin(const GmDev &d, uint32_t slot, uint32_t k) {
+    const GmRec *r = d.rec + slot;
+    const RecA a = rec_a(r);
+    const double inc = r->inc;
+    const uint32_t n = k - a.s0 + 1u;
+    return n == 1u ? a.frac0 + inc : gm_radd_n(a.frac0, inc, n);
 }
 
 /* the stored code of completion step cs predicted at step kp */
@@ -260,14 +260,14 @@ static __device__ __forceinline__ bool cs_inactive(uint32_t cs, uint32_t kc, uin
 }
 
 /* Predicts slot s from its value f before step A.k (its new anchor): rewrites
- * the record's first sector (anchor, increment, final value) and stores the
- * completion step's code. */
+ * the record's first sector (anchor, increment) and stores the completion
+ * step's code. */
 static __device__ __forceinline__ void predict_slot(const GmDev &d, const GmSweepArgs &A, uint32_t s, double f,
                                                     uint32_t s0, double P, uint32_t defg, uint32_t inf) {
-    double inc, fin;
-    const uint32_t cs = predict_values(d, A, A.k, f, P, defg, inf, inc, fin);
+    double inc;
+    const uint32_t cs = predict_values(d, A, A.k, f, P, defg, inf, inc);
     rec_set_a(d.rec + s, f, s0, inf);
-    rec_set_b(d.rec + s, inc, fin);
+    rec_set_b(d.rec + s, inc);
     d.cstep[s] = cs_code(d, cs, A.k);
 }
 
@@ -415,12 +415,12 @@ static __device__ __forceinline__ void infect_one(const GmDev &d, const GmPParam
     double Pn = pP;
     if (fresh_weight || pq0.z != g) Pn = pair_weight(d, pp, pq0.w, g);
     const uint32_t inf = c_code[r];
-    double inc = 0.0, fin = 0.0;
+    double inc = 0.0;
     gm_cs_t cs = (gm_cs_t)GM_CS_DEFERRED;
-    if (!is_pending(Pn)) cs = cs_code(d, predict_values(d, A, k + 1u, carry, Pn, g, inf, inc, fin), k + 1u);
+    if (!is_pending(Pn)) cs = cs_code(d, predict_values(d, A, k + 1u, carry, Pn, g, inf, inc), k + 1u);
     GmRec *rec = d.rec + gslot;
     rec_set_a(rec, carry, k + 1u, inf);
-    rec_set_b(rec, inc, fin);
+    rec_set_b(rec, inc);
     rec_set_c(rec, Pn, t, g);
     *reinterpret_cast<uint4 *>(&rec->rxg) = make_uint4(pq0.w, 0u, 0u, 0u);
     d.cstep[gslot] = cs;

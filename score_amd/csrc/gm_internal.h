@@ -27,9 +27,10 @@
  * inc = ((v*dt)*inv_dx) that SCORE adds every step (SCORE_Kernel.cpp:4092).  While
  * the per-step scalars stay bit-identical, rxFrac after step k is frac0 plus
  * (k-s0+1) rounded additions of inc, so the step in which it first reaches 1.0
- * (cstep) and the value it then holds (fin) are computed in advance
- * (gm_radd.h).  When any scalar changes, every active cell is re-anchored with
- * the old increment and predicted again with the new one.
+ * (cstep) is computed in advance (gm_radd.h), and the value it then holds is
+ * replayed from the anchor when the cell completes.  When any scalar changes,
+ * every active cell is re-anchored with the old increment and predicted again
+ * with the new one.
  */
 #ifndef SCORE_GM_INTERNAL_H
 #define SCORE_GM_INTERNAL_H
@@ -111,14 +112,14 @@ static inline __host__ __device__ uint32_t gm_tiles_inside(uint32_t nI) {
 #define GM_WMAX 8                 /* steps per launch window */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
- * anchor + increment + predicted final value.  The first 32-byte sector holds
+ * anchor + increment.  The first 32-byte sector holds
  * what a prediction or a re-anchoring rewrites, so those write one sector. */
 struct __attribute__((aligned(64))) GmRec {
     double frac0;   /* rxFrac before step s0 */
     uint32_t s0;    /* anchor step */
     uint32_t inf;   /* infector code 0..26 */
     double inc;     /* per-step increment, 0.0 when SCORE does not advance the cell */
-    double fin;     /* rxFrac after the predicted completing addition */
+    double spare;   /* unused, written as 0.0 with inc */
     double P;       /* mobility weight, or a pending marker */
     uint32_t cell;  /* global cell id x + y*nx + z*nx*ny */
     uint32_t defg;  /* deformed grain being consumed (mydefgid) */
@@ -127,16 +128,18 @@ struct __attribute__((aligned(64))) GmRec {
 };
 static_assert(sizeof(GmRec) == 64, "GmRec is one 64-byte record");
 /* the kernels move records as four 16-byte quarters */
-static_assert(__builtin_offsetof(GmRec, inc) == 16 && __builtin_offsetof(GmRec, fin) == 24 &&
+static_assert(__builtin_offsetof(GmRec, inc) == 16 && __builtin_offsetof(GmRec, spare) == 24 &&
                   __builtin_offsetof(GmRec, P) == 32 && __builtin_offsetof(GmRec, cell) == 40 &&
                   __builtin_offsetof(GmRec, defg) == 44 && __builtin_offsetof(GmRec, rxg) == 48,
-              "GmRec quarters: (frac0, s0, inf) | (inc, fin) | (P, cell, defg) | (rxg, pad)");
+              "GmRec quarters: (frac0, s0, inf) | (inc, spare) | (P, cell, defg) | (rxg, pad)");
 
 /* The completed cells of the current step, in FullRX list order (Inside list
  * first): what the infection kernels need of each parent, as three planes of
  * parentCap 16-byte entries, so that a wave's loads of one plane are contiguous.
  *   GM_PAR_IDS  (slot, cell, defg, rxg)
- *   GM_PAR_FP   (fin, P) as two doubles
+ *   GM_PAR_FP   (fin, P) as two doubles: rxFrac after the completing addition
+ *               (written by the decide pass for parents that won a target) and
+ *               the mobility weight
  *   GM_PAR_XYZ  (x, y, z, Border flag or list) */
 #define GM_PAR_IDS 0u
 #define GM_PAR_FP 1u

@@ -400,13 +400,13 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
             const uint32_t s = d.pslot[p];
             const GmRec *r = d.rec + s;
             const RecC c = rec_c(r);
-            const double fin = r->fin;
             const uint32_t rxg = r->rxg;
             const uint32_t cell = c.cell;
             const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
             q2 = make_uint4(x, y, z, s >= d.capI ? 1u : 0u);
             par_plane(d, GM_PAR_IDS)[p] = make_uint4(s, cell, c.defg, rxg);
-            reinterpret_cast<double2 *>(par_plane(d, GM_PAR_FP))[p] = make_double2(fin, c.P);
+            /* the value after the completing addition follows in k_decide */
+            reinterpret_cast<double2 *>(par_plane(d, GM_PAR_FP))[p] = make_double2(0.0, c.P);
             par_plane(d, GM_PAR_XYZ)[p] = q2;
             if (logged) d.rlog[nret + p] = make_uint2(cell, d.ndefg + rxg);
         }
@@ -437,10 +437,13 @@ __global__ void __launch_bounds__(GM_BLOCK) k_propose(GmDev d) {
 
 /* One lane per parent: which of its attempts won, as 26-bit masks per
  * destination list; per-block totals for the ranking scan and the attempts.
- * The partition into blocks is the commit pass's. */
+ * A parent that won a target gets its value after the completing addition, the
+ * carry's source, in the parent array.  The partition into blocks is the commit
+ * pass's. */
 __global__ void __launch_bounds__(64) k_decide(GmDev d) {
     if (d.st->halt) return;
     const uint32_t nfull = d.st->nfullI + d.st->nfullB;
+    const uint32_t k = d.st->kcur;
     uint32_t lo, hi;
     part_of(nfull, GM_PCHUNK, lo, hi);
     uint32_t cI = 0, cB = 0, cA = 0;
@@ -460,6 +463,7 @@ __global__ void __launch_bounds__(64) k_decide(GmDev d) {
         }
         d.pmI[p] = mI;
         d.pmB[p] = mB;
+        if (mI | mB) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, d.pslot[p], k);
         cI += (uint32_t)__popc(mI);
         cB += (uint32_t)__popc(mB);
         cA += (uint32_t)__popc(am);
@@ -776,7 +780,7 @@ __global__ void k_nuc_commit(GmDev d, const uint32_t *site, const double *nucP, 
         uint32_t g = d.grid[w];
         GmRec *r = d.rec + gslot;
         rec_set_a(r, 1.0, 0u, 26u);
-        rec_set_b(r, 0.0, 0.0);
+        rec_set_b(r, 0.0);
         rec_set_c(r, nucP[k], w, g);
         r->rxg = k;
         d.cstep[gslot] = (gm_cs_t)GM_CS_UNKNOWN;
@@ -993,7 +997,7 @@ __global__ void __launch_bounds__(GM_BLOCK) k_defrag_gather(GmDev d, uint32_t ba
                 if (cs != GM_CS_UNKNOWN) f = gm_radd_n(f, inc, k - q0.z);
                 const double P = resolved(d, __hiloint2double((int)q2.y, (int)q2.x));
                 rec_set_a(r, f, k, q0.w);
-                rec_set_b(r, 0.0, 0.0);
+                rec_set_b(r, 0.0);
                 rec_set_c(r, P, q2.z, q2.w);
                 d.cstep[base + c] = (gm_cs_t)GM_CS_NEVER;
             }
@@ -1153,11 +1157,11 @@ __global__ void __launch_bounds__(1024) k_arrivals(GmDev d, GmSweepArgs A0) {
                 const uint32_t gslot = in ? rel : d.capI + rel;
                 const uint32_t g = d.grid[c];
                 const double P = d.nucP[id];
-                double inc = 0.0, fin = 0.0;
-                const gm_cs_t cs = cs_code(d, predict_values(d, A, k, 1.0, P, g, 26u, inc, fin), k);
+                double inc = 0.0;
+                const gm_cs_t cs = cs_code(d, predict_values(d, A, k, 1.0, P, g, 26u, inc), k);
                 GmRec *r = d.rec + gslot;
                 rec_set_a(r, 1.0, k, 26u);
-                rec_set_b(r, inc, fin);
+                rec_set_b(r, inc);
                 rec_set_c(r, P, c, g);
                 *reinterpret_cast<uint4 *>(&r->rxg) = make_uint4(id, 0u, 0u, 0u);
                 d.cstep[gslot] = cs;

@@ -199,4 +199,47 @@ GM_HD uint64_t gm_radd_until_one(double &f, double inc, uint64_t cap) {
     return 0;
 }
 
+/* an estimate of num / den (den > 0), exact or nearly: the device refines the
+ * hardware reciprocal once */
+#if defined(__HIP_DEVICE_COMPILE__)
+GM_HD double gm_quot_estimate(double num, double den) {
+    const double r0 = __builtin_amdgcn_rcp(den);
+    return num * __builtin_fma(r0, __builtin_fma(-den, r0, 1.0), r0);
+}
+#else
+GM_HD double gm_quot_estimate(double num, double den) { return num / den; }
+#endif
+
+/* The count gm_radd_until_one returns, without the value reached.  While the
+ * additions stay below 1.0, each rounds by at most half an ulp of a value below
+ * 1.0, 2^-54, so the n-th one lies within n * 2^-54 of the real sum
+ * S(n) = f + n*inc.  Hence, for the estimate n = ceil((1 - f) / inc):
+ * S(n-1) + (n-1) * 2^-54 < 1 puts every addition up to the (n-1)-th below 1.0
+ * (the exact sum of the next one, below 1 - 2^-54, rounds below 1.0), and
+ * S(n) - (n-1) * 2^-54 >= 1 puts the exact sum of the n-th at or above 1.0, so
+ * n is the count.  The sums are formed here from non-negative terms below 2 in
+ * four roundings of at most 2^-53 each, which the margin 2^-50 covers.  When
+ * either test fails (1.0 lies within about n * 2^-54 of a real sum, or the
+ * estimate is off by one), or for a negative start or an increment of 1/2 or
+ * more, the rounds decide. */
+GM_HD uint64_t gm_radd_count_to_one(double f, double inc, uint64_t cap) {
+    if (cap == 0) return 0;
+    if (f >= 1.0) return f + inc >= 1.0 ? 1 : 0;
+    if (!(inc > 0.0)) return 0;
+    if (f >= 0.0 && inc < 0.5) {
+        const double e = 0x1p-54, M = 0x1p-50, capd = (double)cap;
+        double n = __builtin_ceil(gm_quot_estimate(1.0 - f, inc));
+        if (!(n >= 1.0)) n = 1.0;
+        if (n > capd) {
+            /* the cap-th addition still lies below 1.0 */
+            if (((f + capd * inc) + capd * e) + M < 1.0) return 0;
+        } else {
+            const double m = n - 1.0;
+            if (((f + m * inc) + m * e) + M < 1.0 && ((f + n * inc) - m * e) - M >= 1.0) return (uint64_t)n;
+        }
+    }
+    double g = f;
+    return gm_radd_until_one(g, inc, cap);
+}
+
 #endif

@@ -554,13 +554,13 @@ __global__ void __launch_bounds__(GM_BLOCK) k_rg_propose(GmDev d) {
             const uint32_t s = d.pslot[p];
             const GmRec *r = d.rec + s;
             const RecC c = rec_c(r);
-            const double fin = r->fin;
             const uint32_t rxg = r->rxg;
             const uint32_t cell = c.cell;
             const uint32_t z = cell / d.nxy, rem = cell - z * d.nxy, y = rem / d.nx, x = rem - y * d.nx;
             q2 = make_uint4(x, y, z, d.plist[p]);
             par_plane(d, GM_PAR_IDS)[p] = make_uint4(s, cell, c.defg, rxg);
-            reinterpret_cast<double2 *>(par_plane(d, GM_PAR_FP))[p] = make_double2(fin, c.P);
+            /* the value after the completing addition follows in k_rg_decide */
+            reinterpret_cast<double2 *>(par_plane(d, GM_PAR_FP))[p] = make_double2(0.0, c.P);
             par_plane(d, GM_PAR_XYZ)[p] = q2;
             if (logged) d.rlog[nret + p] = make_uint2(cell, d.ndefg + rxg);
         }
@@ -590,10 +590,12 @@ __global__ void __launch_bounds__(GM_BLOCK) k_rg_propose(GmDev d) {
 
 /* One lane per parent: the attempts that hold their target's smallest key, as
  * masks by destination (own Inside, own Border, a halo), counted per block by
- * row (GM_RW_*); the partition into blocks (one region per block) is the
- * commit pass's. */
+ * row (GM_RW_*); a parent that won a target gets its value after the completing
+ * addition in the parent array.  The partition into blocks (one region per
+ * block) is the commit pass's. */
 __global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
     if (d.st->halt) return;
+    const uint32_t k = d.st->kcur;
     uint32_t lo, hi, reg;
     rg_part(d, lo, hi, reg);
     const RgBox box = rg_box(d, reg);
@@ -622,6 +624,7 @@ __global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
         d.pmI[p] = mI;
         d.pmB[p] = mB;
         d.pmH[p] = mH;
+        if (mI | mB | mH) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, d.pslot[p], k);
         cnt[GM_RW_I] += (uint32_t)__popc(mI);
         cnt[GM_RW_B] += (uint32_t)__popc(mB);
         cnt[GM_RW_A] += (uint32_t)__popc(am);

@@ -3,7 +3,8 @@
  *
  * Every thread takes one (f0, inc, n) case, adds inc to f0 n times with the
  * hardware addition, and compares the value and the first step that reaches 1.0
- * with gm_radd_n and gm_radd_until_one; a second kernel compares gm_floor_div
+ * with gm_radd_n, gm_radd_until_one and gm_radd_count_to_one; a second kernel
+ * compares gm_floor_div
  * with 64-bit integer division.  The cases are drawn as tests/test_radd.py
  * draws them for the host build.  Prints "bad <count>" and exits non-zero on a
  * mismatch.
@@ -50,6 +51,8 @@ __global__ void k_radd(const Case *cs, int ncase, unsigned *bad) {
     const uint64_t jd = gm_radd_until_one(d, c.inc, c.n);
     bool ok = gm_dbits(a) == gm_dbits(b) && jn == jd;
     if (ok && jn) ok = gm_dbits(first) == gm_dbits(d);
+    /* the count alone, certified from the real sums where it can be */
+    if (ok) ok = gm_radd_count_to_one(c.f0, c.inc, c.n) == jn;
     if (!ok) atomicAdd(bad, 1u);
 }
 
@@ -71,7 +74,7 @@ static double ur() { return (double)(nx() >> 11) * (1.0 / 9007199254740992.0); }
 
 static Case draw() {
     double f0, inc;
-    const int kind = (int)(nx() % 9);
+    const int kind = (int)(nx() % 11);
     if (kind == 0) {
         f0 = ur() * 0.01;
         inc = pow(10.0, -5.0 * ur()) * 0.1;
@@ -106,6 +109,13 @@ static Case draw() {
         const double q = (double)(1 + nx() % (1u << 20));
         inc = q * u + u * (0.5 + 0.5 * ur());
         f0 = (9007199254740991.0 - q - (double)(nx() % 50) * (q + 1.0)) * u;
+    } else if (kind == 9) { /* real sums that reach 1.0 within rounding of an integer count */
+        f0 = (nx() & 1) ? 0.0 : ur() * 0.5;
+        inc = (1.0 - f0) / (double)(1 + nx() % 2500);
+        if (nx() & 1) inc = gm_dfrom(gm_dbits(inc) + (nx() % 5) - 2);
+    } else if (kind == 10) { /* the slowest cells: counts around and beyond the cap */
+        f0 = ur() * 0.9;
+        inc = (1.0 - f0) / (1500.0 + 3000.0 * ur());
     } else { /* carries and increments as the growth produces them */
         f0 = ur() * 0.0068 * ur();
         inc = 0.1 * ur() * (0.1 + 0.9 * ur()) * (0.6494 + 0.3506 * ur());

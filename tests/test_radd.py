@@ -39,6 +39,13 @@ static int check(double f0, double inc, uint64_t n) {
                (unsigned long long)jd, d);
         return 1;
     }
+    /* the count alone, certified from the real sums where it can be */
+    uint64_t jc = gm_radd_count_to_one(f0, inc, n);
+    if (jc != jn) {
+        printf("C %a %a %llu: %llu vs %llu\n", f0, inc, (unsigned long long)n, (unsigned long long)jn,
+               (unsigned long long)jc);
+        return 1;
+    }
     return 0;
 }
 
@@ -47,7 +54,7 @@ int main(int argc, char **argv) {
     int bad = 0;
     for (long k = 0; k < cases && bad < 5; k++) {
         double f0, inc;
-        int kind = (int)(nx() % 9);
+        int kind = (int)(nx() % 11);
         if (kind == 0) { f0 = ur() * 0.01; inc = pow(10.0, -5.0 * ur()) * 0.1; }
         else if (kind == 1) { f0 = 0.0; inc = ur() * 0.1; }
         else if (kind == 2) { f0 = 1.0; inc = ur() * 0.1; }
@@ -77,6 +84,13 @@ int main(int argc, char **argv) {
             inc = q * u + u * (0.5 + 0.5 * ur());
             double F0 = 9007199254740991.0 - q - (double)(nx() % 50) * (q + 1.0);
             f0 = F0 * u;
+        } else if (kind == 9) {          /* real sums that reach 1.0 within rounding of an integer count */
+            f0 = (nx() & 1) ? 0.0 : ur() * 0.5;
+            inc = (1.0 - f0) / (double)(1 + nx() % 2500);
+            if (nx() & 1) inc = gm_dfrom(gm_dbits(inc) + (nx() % 5) - 2);
+        } else if (kind == 10) {         /* the slowest cells: counts around and beyond the cap */
+            f0 = ur() * 0.9;
+            inc = (1.0 - f0) / (1500.0 + 3000.0 * ur());
         } else {                         /* carries and increments as the growth produces them */
             f0 = ur() * 0.0068 * ur();
             inc = 0.1 * ur() * (0.1 + 0.9 * ur()) * (0.6494 + 0.3506 * ur());
