@@ -24,14 +24,17 @@
  *                  atomicMin(claim[target], p*32 + r): the key orders attempts
  *                  exactly as the serial loop issues them.
  *   k_decide       an attempt wins iff its key is the minimum; winners per parent
- *                  and destination list as bit masks, counted per block.
+ *                  and destination list as bit masks, counted per block; a
+ *                  parent that won gets its rxFrac after the completing
+ *                  addition, replayed from its anchor.
  *   k_decide_scan  one block scans those counts: rank k into a list takes the
  *                  k-th recycled slot, else appends.
  *   k_recycle      writes the recycled slots that will be handed out: the first
  *                  min(winners, cap) INACTIVE slots of each list, ascending.
  *   k_commit       one thread per winner writes the new record with its
- *                  increment and predicted completion step, and clears the
- *                  cell's bitmap bit; then the parents retire.
+ *                  increment and completion step (certified from the real
+ *                  sums, gm_radd_count_to_one), and clears the cell's bitmap
+ *                  bit; then the parents retire.
  *
  * No kernel waits on another block: every cross-block dependency is a kernel
  * boundary, so the passes use only relaxed device atomics.  FP64 throughout,
