@@ -453,6 +453,7 @@ __global__ void __launch_bounds__(64) k_decide(GmDev d) {
     for (uint32_t p = lo + threadIdx.x; p < hi; p += 64) {
         const uint4 q2 = par_plane(d, GM_PAR_XYZ)[p];
         const uint32_t am = d.pmI[p];
+        const uint32_t ps = d.pslot[p]; /* loaded with the rest, used if the parent wins */
         uint32_t mI = 0, mB = 0;
         for (uint32_t m = am; m; m &= m - 1u) {
             const uint32_t rk = (uint32_t)__ffs(m) - 1u;
@@ -466,7 +467,7 @@ __global__ void __launch_bounds__(64) k_decide(GmDev d) {
         }
         d.pmI[p] = mI;
         d.pmB[p] = mB;
-        if (mI | mB) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, d.pslot[p], k);
+        if (mI | mB) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, ps, k);
         cI += (uint32_t)__popc(mI);
         cB += (uint32_t)__popc(mB);
         cA += (uint32_t)__popc(am);

@@ -605,6 +605,7 @@ __global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
     for (uint32_t p = lo + threadIdx.x; p < hi; p += 64) {
         const uint4 q2 = par_plane(d, GM_PAR_XYZ)[p];
         const uint32_t am = d.pmI[p];
+        const uint32_t ps = d.pslot[p]; /* loaded with the rest, used if the parent wins */
         uint32_t mI = 0, mB = 0, mH = 0;
         for (uint32_t m = am; m; m &= m - 1u) {
             const uint32_t rk = (uint32_t)__ffs(m) - 1u;
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(64) k_rg_decide(GmDev d) {
         d.pmI[p] = mI;
         d.pmB[p] = mB;
         d.pmH[p] = mH;
-        if (mI | mB | mH) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, d.pslot[p], k);
+        if (mI | mB | mH) reinterpret_cast<double *>(par_plane(d, GM_PAR_FP))[2 * (size_t)p] = parent_fin(d, ps, k);
         cnt[GM_RW_I] += (uint32_t)__popc(mI);
         cnt[GM_RW_B] += (uint32_t)__popc(mB);
         cnt[GM_RW_A] += (uint32_t)__popc(am);
