@@ -19,8 +19,8 @@ arrivals, edges 64-320, p50/p99 time to X = 0.99).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for PMC scaling
-    python bench.py --batch 512 --inflight 8 # ensemble of independent domains, domains/hour
-    python bench.py --serve 64 --inflight 8  # Poisson stream of requests, p50/p99 time to X=0.99
+    python bench.py --batch 512 --inflight 12 # ensemble of independent domains, domains/hour
+    python bench.py --serve 64 --inflight 12  # Poisson stream of requests, p50/p99 time to X=0.99
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -411,7 +411,7 @@ def main(argv=None) -> int:
                     help="with --profile-run-only: profile the run with this many memory regions")
     ap.add_argument("--batch", type=int, default=0,
                     help="ensemble mode: this many independent domains, domains/hour batched vs one at a time")
-    ap.add_argument("--inflight", type=int, default=8, help="domains in flight in ensemble and served modes")
+    ap.add_argument("--inflight", type=int, default=12, help="domains in flight in ensemble and served modes")
     ap.add_argument("--serve", type=int, default=0,
                     help="served mode: this many requests arriving as a Poisson stream; p50/p99 time to X=0.99")
     ap.add_argument("--rate", type=float, default=0.0, help="arrival rate of the served mode, requests/s")
