@@ -525,7 +525,9 @@ static __device__ __forceinline__ bool top_exhausted(const GmDev &d, const GmWin
 }
 
 /* The end of a step, run by one wave after the commit pass: the step's row and
- * whether the next step may run. */
+ * whether the next step may run; the window's progress, (steps finished << 1) |
+ * halt, goes to the host's mapped word, which paces the launches (gm_host.cpp,
+ * step_window). */
 static __device__ void step_tail(const GmDev &d, const GmWindow &w) {
     GmStats *st = d.st;
     const uint32_t j = st->nexec;
@@ -554,6 +556,7 @@ static __device__ void step_tail(const GmDev &d, const GmWindow &w) {
     st->pone = 0;
     st->arr_pkey = 0ull;
     step_finish(d, w, r, any_top);
+    if (d.prog) __hip_atomic_store(d.prog, (st->nexec << 1) | st->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 /* orders a wave's LDS writes before its later LDS reads (the lanes of a wave

@@ -148,6 +148,8 @@ struct score_gm_handle {
     GmDev d{};
     GmStats *hst = nullptr; /* pinned mirror of d.st (mapped, coherent), written by k_stats_out */
     GmStats *hst_dev = nullptr; /* its device address */
+    uint32_t *hprog = nullptr;  /* pinned progress word (mapped, coherent), written by k_step_tail */
+    uint32_t *hprog_dev = nullptr;
     score_gm_params p{};
     std::vector<double> sched_t, sched_T, defrag_at;
     std::vector<double> defq, rxq;
@@ -325,6 +327,9 @@ void free_device(score_gm_handle *h) {
     if (h->hst) (void)hipHostFree(h->hst);
     h->hst = nullptr;
     h->hst_dev = nullptr;
+    if (h->hprog) (void)hipHostFree(h->hprog);
+    h->hprog = nullptr;
+    h->hprog_dev = nullptr;
     if (h->hpend_vals) (void)hipHostFree(h->hpend_vals);
     h->hpend_vals = nullptr;
     h->hpend_dev = nullptr;
@@ -1121,6 +1126,28 @@ int log_grainevo(score_gm_handle *h, uint32_t sv) {
 /* Iterations of SCORE's main loop (SCORE_Kernel.cpp:6615-6747), up to wmax of
  * them launched back to back as one window.  Returns 1 when the termination
  * test fired before any step, 0 after *nsteps >= 1 steps, < 0 on error. */
+/* steps queued ahead of the device within a window */
+#define GM_LEAD 2u
+
+/* Waits until the device has finished `need` steps of the window or halted it,
+ * reading the progress word k_step_tail writes into mapped host memory; false
+ * when the window halted.  After a second without getting there it stops
+ * waiting (the window's end reports whatever held the device up). */
+bool wait_steps(score_gm_handle *h, uint32_t need) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t it = 1;; it++) {
+        const uint32_t v = __atomic_load_n(h->hprog, __ATOMIC_ACQUIRE);
+        if (v & 1u) return false;
+        if ((v >> 1) >= need) return true;
+        if ((it & 63u) == 0u) {
+            std::this_thread::yield();
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) return true;
+        } else {
+            __builtin_ia32_pause();
+        }
+    }
+}
+
 int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     *nsteps = 0;
     if (h->err) return h->err;
@@ -1184,9 +1211,17 @@ int step_window(score_gm_handle *h, uint32_t wmax, uint32_t *nsteps) {
     hipEvent_t *ev = prof ? h->ev : nullptr;
     const auto th0 = std::chrono::steady_clock::now();
     hipError_t e;
+    /* The steps are launched GM_LEAD ahead of the device: before step j is
+     * queued the device has finished j - GM_LEAD of them, so that after a step
+     * that halts the window (later steps would return at once) no more are
+     * queued.  Steps not launched are exactly those that would have done
+     * nothing, whatever the host sees in time. */
+    const bool paced = !prof && W > GM_LEAD + 1;
+    if (paced) __atomic_store_n(h->hprog, 0u, __ATOMIC_RELAXED);
     if ((e = gm_launch_window_begin(d, h->step, h->Xcells, h->stream)) != hipSuccess)
         return h->err = hip_fail(e, "growth step");
     for (uint32_t j = 0; j < W; j++) {
+        if (paced && j > GM_LEAD && !wait_steps(h, j - GM_LEAD)) break;
         GmSweepArgs aj = a;
         if (j) aj.reanchor = 0;
         if (d.nreg > 1) {
@@ -1614,11 +1649,13 @@ int score_gm_create(const score_gm_domain *dom, const score_gm_params *prm, int 
         delete h;
         return rc;
     }
-    if ((rc = pinned_mapped(h, (void **)&h->hst, (void **)&h->hst_dev, sizeof(GmStats)))) {
+    if ((rc = pinned_mapped(h, (void **)&h->hst, (void **)&h->hst_dev, sizeof(GmStats))) ||
+        (rc = pinned_mapped(h, (void **)&h->hprog, (void **)&h->hprog_dev, 64))) {
         free_device(h);
         delete h;
         return rc;
     }
+    d.prog = h->hprog_dev;
     uint32_t *d_site = nullptr;
     double *d_nucP = nullptr;
     auto setup = [&]() -> int {

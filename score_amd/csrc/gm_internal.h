@@ -305,6 +305,7 @@ struct GmDev {
     uint32_t *nflag, *nI_, *nB_;
     uint32_t *scan_bsum;        /* 2*GM_SCAN_BLOCKS */
     GmStats *st;
+    uint32_t *prog;             /* mapped host word: (steps the window has finished << 1) | halt */
     /* several memory regions (nreg > 1); with one region none of this is used */
     uint32_t nreg, nlists;      /* T and 2T */
     uint32_t npy, npz, nry, nrz;/* region grid and slab thickness along y and z */
