@@ -109,7 +109,7 @@ static inline __host__ __device__ uint32_t gm_tiles_inside(uint32_t nI) {
 #define GM_ZEROED_INLINE 1022     /* exhausted grain ids returned with the per-step stats */
 #define GM_PEND_INLINE 2048       /* new P-cache keys returned with the per-step stats */
 #define GM_NK 9                   /* kernels per step (SCORE_GM_K_* order) */
-#define GM_WMAX 8                 /* steps per launch window */
+#define GM_WMAX 16                /* steps per launch window at most (also below imin - 1) */
 
 /* One front record: SCORE's `cell` (SCORE_Kernel.h:124-137) with rxFrac held as
  * anchor + increment.  The first 32-byte sector holds

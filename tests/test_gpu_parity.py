@@ -618,6 +618,29 @@ def test_chunks_of_several_windows(require_gpu, seed, chunks):
     assert_final_equal(o, g)
 
 
+@pytest.mark.parametrize("maxfill,chunks", [(0.03, (23, 40, 17)), (0.05, (35,)), (0.07, (9, 30))])
+def test_longest_windows_at_small_maxfill(require_gpu, maxfill, chunks):
+    """A smaller maxfill lets a launch window run longer (up to 16 steps while
+    a new cell needs more steps than that to fill); fronts and trajectory agree
+    with the oracle at every chunk's end."""
+    dom = synth.make_voronoi_domain(40, 36, 30, n_grains=7, n_nuclei=20, seed=21)
+    prm = ParamsSpec(maxfill=maxfill)
+    print(f"long windows maxfill {maxfill} chunks {chunks}", flush=True)
+    o, g = Oracle(dom, prm), _gm(dom, prm)
+    k = 0
+    while True:
+        n = chunks[k % len(chunks)]
+        do, dg = o.step(n), g.step(n)
+        assert do == dg, k
+        assert_traj_equal(o.trajectory(), g.trajectory())
+        for lst in (abi.LIST_INSIDE, abi.LIST_BORDER):
+            assert_front_equal(o.front(lst), g.front(lst), f"chunk{k} list{lst}")
+        if do:
+            break
+        k += 1
+    assert_final_equal(o, g)
+
+
 @pytest.mark.parametrize("seed", range(0, 40, 3))
 def test_mixed_settings_in_long_chunks(require_gpu, seed):
     """The seeded mixed settings stepped in chunks of 10-60 steps: several
