@@ -1127,7 +1127,7 @@ int log_grainevo(score_gm_handle *h, uint32_t sv) {
  * them launched back to back as one window.  Returns 1 when the termination
  * test fired before any step, 0 after *nsteps >= 1 steps, < 0 on error. */
 /* steps queued ahead of the device within a window */
-#define GM_LEAD 2u
+#define GM_LEAD 1u
 
 /* Waits until the device has finished `need` steps of the window or halted it,
  * reading the progress word k_step_tail writes into mapped host memory; false
